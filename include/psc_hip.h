@@ -1,0 +1,159 @@
+/*
+ * psc_hip.h -- C ABI of the MI355X (gfx950) D3Q19 partially-saturated-cell
+ * immersed-boundary (PSC-IMB) lattice-Boltzmann library, libpsc_hip.so.
+ *
+ * The reference (wangjianfttt/particuology_Hydraulic-penalty-) exposes this
+ * path as the Python class API of src/lbm_dem/psc3d.py; it has no foreign
+ * function interface of its own.  Each entry point below replaces one piece
+ * of that module (cited per function) and is what the Python facade
+ * (package module psc3d.py) binds through ctypes.
+ *
+ * Conventions
+ *   - Every function returns int: 0 on success, a negative PSC_E* code on a
+ *     bad argument, or a positive hipError_t from the HIP runtime.
+ *     psc_last_error() gives the message of the last failure on this thread.
+ *   - Pointers named d_* are device addresses (hipMalloc / torch tensors);
+ *     nothing here allocates, frees or synchronises, so every launch is
+ *     asynchronous on `stream` (a hipStream_t, NULL = default stream).
+ *   - Lattice fields are structure-of-arrays: direction-major planes of
+ *     N = nx*ny*nz cells, cell id = (ix*ny + iy)*nz + iz (iz fastest), the
+ *     C order of the reference's (19, n, n, n) arrays.
+ *   - Distribution storage: PSC_F32_DEV stores g_i = f_i - w_i in FP32,
+ *     PSC_F64_PLAIN stores f_i in FP64 with the reference's exact operation
+ *     order.  Stream layout PSC_PUSH keeps the post-stream state (the
+ *     reference's `f`); PSC_PULL keeps the post-collision state and gathers
+ *     on the next step.
+ *   - Particle table: P rows of 10 doubles (x, y, z, r, vx, vy, vz, wx, wy,
+ *     wz), lattice units, the fields of Particle3D (psc3d.py:46-57).
+ *   - Box table: P rows of 8 int32 (x0, y0, z0, ex, ey, ez, entry_offset, 0):
+ *     the cell bounding box [x0, x0+ex) ... of psc3d.py:73-78 and the offset
+ *     of the particle's first per-cell count in the count array.
+ *   - Chunk table: rows of 2 int32 (particle, first box entry); each chunk
+ *     covers up to `chunk_cells` consecutive box entries (ix, iy, iz
+ *     lexicographic) of one particle.
+ */
+#ifndef PSC_HIP_H
+#define PSC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSC_OK 0
+#define PSC_EINVAL (-1) /* bad argument (null pointer, size, range)          */
+#define PSC_ELAUNCH (-2) /* launch configuration outside the kernel's limits  */
+
+#define PSC_F32_DEV 1   /* FP32 deviation storage g = f - w                  */
+#define PSC_F64_PLAIN 2 /* FP64 plain storage f, reference operation order   */
+
+#define PSC_PUSH 0
+#define PSC_PULL 1
+
+#define PSC_NSUM 13 /* number of FP64 sums produced by psc_moments           */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int psc_version(void);
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* psc_last_error(void);
+
+/* D3Q19 velocity set, weights and opposite table (psc3d.py:17-43), host
+ * memory: c[19*3] int32, w[19] double, opp[19] int32. Host-only, no GPU. */
+int psc_lattice_constants(int32_t* c, double* w, int32_t* opp);
+
+/* Zero nbytes of device memory at d_ptr with a fill kernel (graph-safe).   */
+int psc_clear(void* d_ptr, int64_t nbytes, void* stream);
+
+/* Initial state rho = 1, u = 0 (psc3d.py:136-140 / :312-316): f = w in
+ * PSC_F64_PLAIN, g = 0 in PSC_F32_DEV.  ncell = N. */
+int psc_init_state(void* d_state, int32_t storage, int64_t ncell, void* stream);
+
+/* K1 -- sphere-to-lattice solid fraction by s^3 sub-sample counting.
+ * Replaces _sparse_particle_alpha_task (psc3d.py:60-94) with the sum of
+ * SparsePscD3Q19.rebuild_geometry (psc3d.py:328-345) / the dense
+ * PscD3Q19.rebuild_geometry (psc3d.py:151-179), and the numerator of
+ * solid_velocity (psc3d.py:181-206, :347-373).
+ *   d_counts[entry]   = number of samples inside (uint16) for every box entry
+ *   d_count_sum[cell] += count              (must be cleared before the call)
+ *   d_us_num[a*N+cell] += (count/s^3)*(U + omega x r)_a   if d_us_num != NULL
+ *                     (FP32 if us_precision == 1, FP64 if 2; must be cleared)
+ * Box cells are wrapped into the lattice, so clamped boxes (the reference,
+ * psc3d.py:73-78) and unclamped periodic boxes use the same call; lever
+ * arms use the unwrapped cell position. */
+int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
+                 int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz,
+                 int32_t subsamples, uint16_t* d_counts, uint32_t* d_count_sum,
+                 void* d_us_num, int32_t us_precision, void* stream);
+
+/* K2 -- fused PSC collision + streaming, one lattice step.
+ * Replaces PscD3Q19.collide_stream_psc (psc3d.py:208-246) with equilibrium
+ * (psc3d.py:97-103) and guo_force (psc3d.py:106-114) inlined.  The solid
+ * fraction of a cell is min(count_sum/s^3, 1); B = a/(a + tau(1-a) + 1e-14);
+ * the solid velocity is us_num/(count_sum/s^3) (zero if d_us_num is NULL).
+ * The body force (fx, fy, fz) generalises body_force_z; (0, 0, Fz) is the
+ * reference.  d_imb (3 planes, optional) receives sum_i (B Omega_s)_i c_i
+ * (psc3d.py:234-236) in every cell with count_sum > 0; other cells are not
+ * written.  d_in and d_out must not alias.  block_z/block_y select the
+ * workgroup shape (0 = default); storage/layout as above.  Precision of
+ * d_in/d_out/d_us_num/d_imb follows `storage`. */
+int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const void* d_us_num,
+             void* d_imb, int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz,
+             int32_t subsamples, double tau, double fx, double fy, double fz, int32_t block_z,
+             int32_t block_y, void* stream);
+
+/* K3 -- node-summed hydrodynamic force and torque per particle.
+ * Replaces accumulate_particle_loads (psc3d.py:255-281 dense, :375-407
+ * sparse): owner = (count/s^3)/max(count_sum/s^3, 1e-14), node force
+ * -owner*imb, torque about the centre with lever arm (cell + 1/2) - centre.
+ * Two launches: per-chunk FP64 partial sums (d_partials, nchunks*6 doubles)
+ * then a fixed-order per-particle sum over d_chunk_first[p]..[p+1] into
+ * d_loads (P*6 doubles: Fx Fy Fz Tx Ty Tz).  Deterministic.
+ * imb_precision: 1 = FP32 imb, 2 = FP64 imb. */
+int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
+                       const int32_t* d_chunk_first, int32_t nparticles, int32_t nchunks,
+                       int32_t chunk_cells, const uint16_t* d_counts, const uint32_t* d_count_sum,
+                       const void* d_imb, int32_t imb_precision, int32_t nx, int32_t ny, int32_t nz,
+                       int32_t subsamples, double* d_partials, double* d_loads, void* stream);
+
+/* K4 -- post-stream moments and FP64 readout sums.
+ * Replaces the tail of collide_stream_psc (psc3d.py:243-253) and the
+ * validation readouts (run_precision_validation.py:142-147,438;
+ * run_psc_permeability_resolution.py:144-160).  For every cell it forms
+ * rho = sum f, u = (sum f c + F/2)/rho of the post-stream state; if d_rho_u
+ * is not NULL it stores rho, ux, uy, uz as 4 FP64 planes.  d_sums receives
+ * PSC_NSUM FP64 values in this order:
+ *   0 sum solid, 1 sum (1-solid), 2-4 sum u, 5-7 sum (1-solid) u,
+ *   8-10 sum imb over cells with count_sum > 0, 11 sum rho,
+ *   12 number of cells with a non-finite moment.
+ * solid = min(count_sum/s^3, 1).  d_partials needs psc_moments_blocks(N)*
+ * PSC_NSUM doubles.  Deterministic (fixed-order two-pass reduction).
+ * d_imb may be NULL (sums 8-10 are then 0). */
+int psc_moments(const void* d_state, int32_t storage, int32_t layout, const uint32_t* d_count_sum,
+                const void* d_imb, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double fx,
+                double fy, double fz, double* d_rho_u, double* d_partials, double* d_sums,
+                void* stream);
+
+/* Number of partial-sum blocks psc_moments uses for N cells. */
+int64_t psc_moments_blocks(int64_t ncell);
+
+/* Post-stream distributions f as FP64 (19 planes), the reference's `f`
+ * attribute (psc3d.py:239-242): w + g for PSC_F32_DEV, gathered along c_i
+ * for PSC_PULL. */
+int psc_export_f64(const void* d_state, int32_t storage, int32_t layout, int32_t nx, int32_t ny,
+                   int32_t nz, double* d_f, void* stream);
+
+/* Inverse of psc_export_f64: set the state from post-stream FP64 f. */
+int psc_import_f64(const double* d_f, int32_t storage, int32_t layout, int32_t nx, int32_t ny,
+                   int32_t nz, void* d_state, void* stream);
+
+/* Plain device copy of n floats, the bandwidth reference for the roofline
+ * and for scaling HBM byte counters (no counterpart in the reference). */
+int psc_copy_f32(const float* d_src, float* d_dst, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSC_HIP_H */

@@ -1,0 +1,84 @@
+"""ctypes binding of libpsc_hip.so (the C ABI declared in include/psc_hip.h).
+
+The library is built in-tree by ``csrc/build.py``; there is no fallback: if
+it is missing or cannot be loaded, every solver constructor raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpsc_hip.so")
+
+PSC_OK = 0
+PSC_F32_DEV = 1
+PSC_F64_PLAIN = 2
+PSC_PUSH = 0
+PSC_PULL = 1
+PSC_NSUM = 13
+
+# Names and argument types of every exported entry point, in header order.
+_P = ct.c_void_p
+_I32 = ct.c_int32
+_I64 = ct.c_int64
+_D = ct.c_double
+SIGNATURES = {
+    "psc_version": ([], ct.c_int),
+    "psc_last_error": ([], ct.c_char_p),
+    "psc_lattice_constants": ([_P, _P, _P], ct.c_int),
+    "psc_clear": ([_P, _I64, _P], ct.c_int),
+    "psc_init_state": ([_P, _I32, _I64, _P], ct.c_int),
+    "psc_voxelize": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I32, _P], ct.c_int),
+    "psc_step": ([_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _P],
+                 ct.c_int),
+    "psc_particle_loads": ([_P, _P, _P, _P, _I32, _I32, _I32, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P,
+                            _P], ct.c_int),
+    "psc_moments": ([_P, _I32, _I32, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _P, _P, _P, _P], ct.c_int),
+    "psc_moments_blocks": ([_I64], ct.c_int64),
+    "psc_export_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
+    "psc_import_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
+    "psc_copy_f32": ([_P, _P, _I64, _P], ct.c_int),
+}
+
+
+class PscError(RuntimeError):
+    """A libpsc_hip call returned a non-zero status."""
+
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load (once) and return the ctypes library with typed entry points."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise PscError(
+            f"{p} is missing: build it with `python particuology_hydraulic-penalty-_amd/csrc/build.py` "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    lib = ct.CDLL(p)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != PSC_OK:
+        msg = load().psc_last_error().decode(errors="replace")
+        raise PscError(f"{what} failed with status {rc}: {msg}")
+
+
+def call(name: str, *args) -> int:
+    """Call an entry point and raise PscError on a non-zero status."""
+    fn = getattr(load(), name)
+    rc = fn(*args)
+    check(rc, name)
+    return rc

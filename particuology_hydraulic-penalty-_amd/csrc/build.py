@@ -1,0 +1,74 @@
+"""Build libpsc_hip.so for gfx950 with hipcc, in-tree.
+
+    python particuology_hydraulic-penalty-_amd/csrc/build.py
+
+Three translation units:
+  psc_exact.hip  FP64 kernels in the reference's operation order, compiled
+                 with -ffp-contract=off (no fused multiply-add), plus the
+                 voxelizer, particle loads, moments and f import/export;
+  psc_fast.hip   the FP32 deviation step, -ffp-contract=fast;
+  psc_abi.hip    the extern "C" layer.
+Packed FP32 instructions are disabled on every device compile.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+ROOT = os.path.dirname(PKG)
+LIB_NAME = "libpsc_hip.so"
+ARCH = "gfx950"
+
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    f"-I{os.path.join(ROOT, 'include')}",
+]
+UNITS = [
+    ("psc_exact.hip", ["-ffp-contract=off"]),
+    ("psc_fast.hip", ["-ffp-contract=fast"]),
+    ("psc_abi.hip", ["-ffp-contract=off"]),
+]
+
+
+def hipcc() -> str:
+    cand = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(cand):
+        raise RuntimeError("hipcc not found; the HIP library cannot be built")
+    return cand
+
+
+def build(verbose: bool = False, out_dir: str | None = None) -> str:
+    """Compile every unit and link the shared library; return its path."""
+    cc = hipcc()
+    out_dir = out_dir or PKG
+    target = os.path.join(out_dir, LIB_NAME)
+    with tempfile.TemporaryDirectory(prefix="psc_build_") as tmp:
+        objs = []
+        for src, extra in UNITS:
+            obj = os.path.join(tmp, src.replace(".hip", ".o"))
+            cmd = [cc, "-c", os.path.join(HERE, src), "-o", obj] + COMMON + extra
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
+            objs.append(obj)
+        tmp_lib = os.path.join(tmp, LIB_NAME)
+        cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp_lib] + objs
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        shutil.copyfile(tmp_lib, target + ".tmp")
+        os.replace(target + ".tmp", target)
+    return target
+
+
+if __name__ == "__main__":
+    path = build(verbose="-q" not in sys.argv)
+    print("built", path)

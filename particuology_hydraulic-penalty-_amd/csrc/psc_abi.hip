@@ -1,0 +1,241 @@
+// extern "C" entry points of libpsc_hip.so: argument checking, error
+// reporting and dispatch to the kernel launchers of psc_exact.hip and
+// psc_fast.hip.  The declarations and their reference citations are in
+// include/psc_hip.h.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <string>
+
+#include "psc_common.h"
+#include "../../include/psc_hip.h"
+
+namespace psc {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error((int)e, "%s: %s", what, hipGetErrorString(e));
+  return PSC_OK;
+}
+
+// launchers defined in the kernel translation units
+int launch_step_f64(const double*, double*, const uint32_t*, const double*, double*, int, int, int, int, int, double,
+                    double, double, double, int, int, hipStream_t);
+int launch_step_f32(const float*, float*, const uint32_t*, const float*, float*, int, int, int, int, int, double,
+                    double, double, double, int, int, hipStream_t);
+int launch_voxelize(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, uint16_t*, uint32_t*,
+                    void*, int, hipStream_t);
+int launch_loads(const double*, const int32_t*, const int32_t*, const int32_t*, int, int, int, const uint16_t*,
+                 const uint32_t*, const void*, int, int, int, int, int, double*, double*, hipStream_t);
+int launch_moments(const void*, int, int, const uint32_t*, const void*, int, int, int, int, double, double, double,
+                   double*, double*, double*, hipStream_t);
+int64_t moments_blocks(int64_t);
+int launch_export(const void*, int, int, int, int, int, double*, hipStream_t);
+int launch_import(const double*, int, int, int, int, int, void*, hipStream_t);
+int launch_init_plain(double*, size_t, hipStream_t);
+
+namespace {
+
+__global__ void k_clear16(uint4* __restrict__ p, size_t n16) {
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = z;
+}
+
+__global__ void k_clear1(unsigned char* __restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+__global__ void k_copy4(const float4* __restrict__ s, float4* __restrict__ d, size_t n4) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
+__global__ void k_copy1(const float* __restrict__ s, float* __restrict__ d, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
+int grid_cap(size_t n, int threads) {
+  size_t g = (n + threads - 1) / threads;
+  if (g > 16384) g = 16384;
+  return g < 1 ? 1 : (int)g;
+}
+
+bool dims_ok(int nx, int ny, int nz) {
+  if (nx < 1 || ny < 1 || nz < 1) return false;
+  const unsigned long long n = (unsigned long long)nx * ny * nz;
+  return n <= 0x7fffffffULL;
+}
+
+}  // namespace
+}  // namespace psc
+
+using namespace psc;
+
+extern "C" {
+
+int psc_version(void) { return 10000; }
+
+const char* psc_last_error(void) { return g_last_error.c_str(); }
+
+int psc_lattice_constants(int32_t* c, double* w, int32_t* opp) {
+  if (c == nullptr || w == nullptr || opp == nullptr) return set_error(PSC_EINVAL, "psc_lattice_constants: null");
+  for (int i = 0; i < Q; ++i) {
+    c[3 * i] = cx_of(i);
+    c[3 * i + 1] = cy_of(i);
+    c[3 * i + 2] = cz_of(i);
+    w[i] = w_of(i);
+    opp[i] = opp_of(i);
+  }
+  return PSC_OK;
+}
+
+int psc_clear(void* d_ptr, int64_t nbytes, void* stream) {
+  if (nbytes < 0) return set_error(PSC_EINVAL, "psc_clear: negative size");
+  if (nbytes == 0) return PSC_OK;
+  if (d_ptr == nullptr) return set_error(PSC_EINVAL, "psc_clear: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const uintptr_t a = (uintptr_t)d_ptr;
+  if ((a & 15u) == 0) {
+    const size_t n16 = (size_t)nbytes / 16;
+    if (n16) k_clear16<<<grid_cap(n16, 256), 256, 0, s>>>((uint4*)d_ptr, n16);
+    const size_t tail = (size_t)nbytes - n16 * 16;
+    if (tail) k_clear1<<<1, 64, 0, s>>>((unsigned char*)d_ptr + n16 * 16, tail);
+  } else {
+    k_clear1<<<grid_cap((size_t)nbytes, 256), 256, 0, s>>>((unsigned char*)d_ptr, (size_t)nbytes);
+  }
+  return check_launch("psc_clear");
+}
+
+int psc_init_state(void* d_state, int32_t storage, int64_t ncell, void* stream) {
+  if (d_state == nullptr || ncell <= 0) return set_error(PSC_EINVAL, "psc_init_state: bad state or size");
+  if (storage == PSC_F32_DEV) return psc_clear(d_state, (int64_t)Q * ncell * 4, stream);
+  if (storage == PSC_F64_PLAIN) return launch_init_plain((double*)d_state, (size_t)ncell, (hipStream_t)stream);
+  return set_error(PSC_EINVAL, "psc_init_state: storage %d", storage);
+}
+
+int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks, int32_t nchunks,
+                 int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, uint16_t* d_counts,
+                 uint32_t* d_count_sum, void* d_us_num, int32_t us_precision, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_voxelize: bad lattice %dx%dx%d", nx, ny, nz);
+  if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
+    return set_error(PSC_EINVAL, "psc_voxelize: subsamples %d outside [1, 40]", subsamples);
+  if (nchunks < 0 || chunk_cells < 1) return set_error(PSC_EINVAL, "psc_voxelize: bad chunking");
+  if (nchunks > 0 && (d_particles == nullptr || d_boxes == nullptr || d_chunks == nullptr || d_counts == nullptr ||
+                      d_count_sum == nullptr))
+    return set_error(PSC_EINVAL, "psc_voxelize: null pointer");
+  if (d_us_num != nullptr && us_precision != 1 && us_precision != 2)
+    return set_error(PSC_EINVAL, "psc_voxelize: us_precision %d", us_precision);
+  return launch_voxelize(d_particles, d_boxes, d_chunks, nchunks, chunk_cells, nx, ny, nz, subsamples, d_counts,
+                         d_count_sum, d_us_num, d_us_num ? us_precision : 0, (hipStream_t)stream);
+}
+
+int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const void* d_us_num, void* d_imb,
+             int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double tau,
+             double fx, double fy, double fz, int32_t block_z, int32_t block_y, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_step: bad lattice %dx%dx%d", nx, ny, nz);
+  if (d_in == nullptr || d_out == nullptr || d_count_sum == nullptr) return set_error(PSC_EINVAL, "psc_step: null");
+  if (d_in == d_out) return set_error(PSC_EINVAL, "psc_step: input and output alias");
+  if (layout != PSC_PUSH && layout != PSC_PULL) return set_error(PSC_EINVAL, "psc_step: layout %d", layout);
+  if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
+    return set_error(PSC_EINVAL, "psc_step: subsamples %d", subsamples);
+  if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_step: tau %g must exceed 1/2", tau);
+  int bz = block_z, by = block_y;
+  if (bz <= 0) bz = nz >= 64 ? 64 : 32;
+  if (by <= 0) by = 256 / bz > 0 ? 256 / bz : 1;
+  if (bz * by > 256 || bz * by < 1 || ((bz * by) % 64) != 0)
+    return set_error(PSC_ELAUNCH, "psc_step: block %dx%d must hold a multiple of 64 threads, at most 256", bz, by);
+  hipStream_t s = (hipStream_t)stream;
+  if (storage == PSC_F32_DEV)
+    return launch_step_f32((const float*)d_in, (float*)d_out, d_count_sum, (const float*)d_us_num, (float*)d_imb,
+                           layout, nx, ny, nz, subsamples, tau, fx, fy, fz, bz, by, s);
+  if (storage == PSC_F64_PLAIN)
+    return launch_step_f64((const double*)d_in, (double*)d_out, d_count_sum, (const double*)d_us_num,
+                           (double*)d_imb, layout, nx, ny, nz, subsamples, tau, fx, fy, fz, bz, by, s);
+  return set_error(PSC_EINVAL, "psc_step: storage %d", storage);
+}
+
+int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
+                       const int32_t* d_chunk_first, int32_t nparticles, int32_t nchunks, int32_t chunk_cells,
+                       const uint16_t* d_counts, const uint32_t* d_count_sum, const void* d_imb,
+                       int32_t imb_precision, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples,
+                       double* d_partials, double* d_loads, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_particle_loads: bad lattice");
+  if (nparticles < 0 || nchunks < 0 || chunk_cells < 1) return set_error(PSC_EINVAL, "psc_particle_loads: sizes");
+  if (imb_precision != 1 && imb_precision != 2) return set_error(PSC_EINVAL, "psc_particle_loads: imb precision");
+  if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
+    return set_error(PSC_EINVAL, "psc_particle_loads: subsamples %d", subsamples);
+  if (nparticles > 0 && (d_chunk_first == nullptr || d_loads == nullptr))
+    return set_error(PSC_EINVAL, "psc_particle_loads: null output");
+  if (nchunks > 0 && (d_particles == nullptr || d_boxes == nullptr || d_chunks == nullptr || d_counts == nullptr ||
+                      d_count_sum == nullptr || d_imb == nullptr || d_partials == nullptr))
+    return set_error(PSC_EINVAL, "psc_particle_loads: null input");
+  return launch_loads(d_particles, d_boxes, d_chunks, d_chunk_first, nparticles, nchunks, chunk_cells, d_counts,
+                      d_count_sum, d_imb, imb_precision, nx, ny, nz, subsamples, d_partials, d_loads,
+                      (hipStream_t)stream);
+}
+
+int64_t psc_moments_blocks(int64_t ncell) { return moments_blocks(ncell); }
+
+int psc_moments(const void* d_state, int32_t storage, int32_t layout, const uint32_t* d_count_sum, const void* d_imb,
+                int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double fx, double fy, double fz,
+                double* d_rho_u, double* d_partials, double* d_sums, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_moments: bad lattice");
+  if (storage != PSC_F32_DEV && storage != PSC_F64_PLAIN) return set_error(PSC_EINVAL, "psc_moments: storage");
+  if (layout != PSC_PUSH && layout != PSC_PULL) return set_error(PSC_EINVAL, "psc_moments: layout");
+  if (d_count_sum == nullptr || d_partials == nullptr || d_sums == nullptr)
+    return set_error(PSC_EINVAL, "psc_moments: null");
+  if (d_state == nullptr && d_rho_u != nullptr) return set_error(PSC_EINVAL, "psc_moments: fields need a state");
+  if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
+    return set_error(PSC_EINVAL, "psc_moments: subsamples %d", subsamples);
+  return launch_moments(d_state, storage, layout, d_count_sum, d_imb, nx, ny, nz, subsamples, fx, fy, fz, d_rho_u,
+                        d_partials, d_sums, (hipStream_t)stream);
+}
+
+int psc_export_f64(const void* d_state, int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz,
+                   double* d_f, void* stream) {
+  if (!dims_ok(nx, ny, nz) || d_state == nullptr || d_f == nullptr)
+    return set_error(PSC_EINVAL, "psc_export_f64: bad arguments");
+  if ((storage != PSC_F32_DEV && storage != PSC_F64_PLAIN) || (layout != PSC_PUSH && layout != PSC_PULL))
+    return set_error(PSC_EINVAL, "psc_export_f64: storage/layout");
+  if ((const void*)d_f == d_state) return set_error(PSC_EINVAL, "psc_export_f64: aliasing");
+  return launch_export(d_state, storage, layout, nx, ny, nz, d_f, (hipStream_t)stream);
+}
+
+int psc_import_f64(const double* d_f, int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz,
+                   void* d_state, void* stream) {
+  if (!dims_ok(nx, ny, nz) || d_state == nullptr || d_f == nullptr)
+    return set_error(PSC_EINVAL, "psc_import_f64: bad arguments");
+  if ((storage != PSC_F32_DEV && storage != PSC_F64_PLAIN) || (layout != PSC_PUSH && layout != PSC_PULL))
+    return set_error(PSC_EINVAL, "psc_import_f64: storage/layout");
+  if ((const void*)d_f == d_state) return set_error(PSC_EINVAL, "psc_import_f64: aliasing");
+  return launch_import(d_f, storage, layout, nx, ny, nz, d_state, (hipStream_t)stream);
+}
+
+int psc_copy_f32(const float* d_src, float* d_dst, int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && (d_src == nullptr || d_dst == nullptr))) return set_error(PSC_EINVAL, "psc_copy_f32: args");
+  if (n == 0) return PSC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if ((((uintptr_t)d_src | (uintptr_t)d_dst) & 15u) == 0 && (n % 4) == 0)
+    k_copy4<<<grid_cap((size_t)n / 4, 256), 256, 0, s>>>((const float4*)d_src, (float4*)d_dst, (size_t)n / 4);
+  else
+    k_copy1<<<grid_cap((size_t)n, 256), 256, 0, s>>>(d_src, d_dst, (size_t)n);
+  return check_launch("psc_copy_f32");
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// Shared device definitions of the D3Q19 PSC-IMB kernels (gfx950).
+//
+// Lattice constants follow src/lbm_dem/psc3d.py:17-43 of the reference:
+// direction 0 is rest, 1-6 the axis directions (+x,-x,+y,-y,+z,-z), 7-18
+// the face diagonals in (+,+),(-,-),(+,-),(-,+) pairs over xy, xz, yz; the
+// opposite of direction i is its pair partner.  Weights 1/3, 1/18, 1/36,
+// cs^2 = 1/3.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace psc {
+
+constexpr int Q = 19;
+
+__host__ __device__ constexpr int cx_of(int i) {
+  constexpr int t[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  return t[i];
+}
+__host__ __device__ constexpr int cy_of(int i) {
+  constexpr int t[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+  return t[i];
+}
+__host__ __device__ constexpr int cz_of(int i) {
+  constexpr int t[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+  return t[i];
+}
+__host__ __device__ constexpr int c_of(int i, int axis) {
+  return axis == 0 ? cx_of(i) : (axis == 1 ? cy_of(i) : cz_of(i));
+}
+__host__ __device__ constexpr int opp_of(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
+
+// Weights as the reference's Python floats 1/3, 1/18, 1/36 (correctly rounded).
+__host__ __device__ constexpr double w_of(int i) {
+  return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
+}
+constexpr double kCs2 = 1.0 / 3.0;
+constexpr double kCs2Sq = kCs2 * kCs2;
+
+// Periodic neighbour coordinate: d in {-1, 0, 1}.
+__device__ __forceinline__ int wrap_step(int v, int d, int n) {
+  int r = v + d;
+  r = (r < 0) ? r + n : r;
+  r = (r >= n) ? r - n : r;
+  return r;
+}
+
+// Wrap an arbitrary (possibly negative) coordinate into [0, n).
+__device__ __forceinline__ int wrap_any(int v, int n) {
+  int r = v % n;
+  return r < 0 ? r + n : r;
+}
+
+// Sum of K doubles per thread over the workgroup: wave64 butterfly then LDS across waves.
+// Result valid in thread 0.  blockDim.x must be a multiple of 64, <= 1024.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double* lds /* K*16 doubles */) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwave = blockDim.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[wave * K + k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double acc = lds[k];
+      for (int w = 1; w < nwave; ++w) acc += lds[w * K + k];
+      v[k] = acc;
+    }
+  }
+}
+
+}  // namespace psc
+
+// Error reporting shared by the translation units (defined in psc_abi.hip).
+namespace psc {
+int set_error(int code, const char* fmt, ...);
+int check_launch(const char* what);
+}  // namespace psc

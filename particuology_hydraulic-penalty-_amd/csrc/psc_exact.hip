@@ -1,0 +1,722 @@
+// FP64 kernels with the reference's operation order: the plain FP64 step,
+// sub-sample voxelization, particle loads, moments/readouts and the f
+// import/export.  This translation unit is compiled with -ffp-contract=off so
+// that every multiply and add rounds separately, as NumPy does; the FP64 step
+// then reproduces src/lbm_dem/psc3d.py bit for bit on cells owned by one
+// particle.
+#include "psc_common.h"
+#include "../../include/psc_hip.h"
+
+namespace psc {
+namespace {
+
+// ((cx*ux + cy*uy) + cz*uz) for cx, cy, cz in {-1, 0, 1}, as NumPy forms it
+// (equilibrium, psc3d.py:101).  Products with 0 only contribute signed zeros,
+// so the sum of the non-zero terms in axis order rounds identically.
+template <int I>
+__device__ __forceinline__ double cdot(double ux, double uy, double uz) {
+  constexpr int a = cx_of(I), b = cy_of(I), c = cz_of(I);
+  double acc = 0.0;
+  bool first = true;
+  if (a != 0) { acc = a > 0 ? ux : -ux; first = false; }
+  if (b != 0) { const double t = b > 0 ? uy : -uy; acc = first ? t : acc + t; first = false; }
+  if (c != 0) { const double t = c > 0 ? uz : -uz; acc = first ? t : acc + t; first = false; }
+  return acc;
+}
+
+// sum_i v_i c_{i,AX} accumulated in direction order (np.sum(..., axis=0)).
+template <int AX>
+__device__ __forceinline__ double moment(const double (&v)[Q]) {
+  double acc = 0.0;
+  bool first = true;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int c = c_of(i, AX);
+    if (c == 0) continue;
+    const double t = c > 0 ? v[i] : -v[i];
+    acc = first ? t : acc + t;
+    first = false;
+  }
+  return acc;
+}
+
+template <int I>
+__device__ __forceinline__ double feq_term(double rho, double ux, double uy, double uz, double uu) {
+  const double cu = cdot<I>(ux, uy, uz);
+  return (w_of(I) * rho) * (((1.0 + cu / kCs2) + ((0.5 * cu) * cu) / kCs2Sq) - (0.5 * uu) / kCs2);
+}
+
+template <int... Is>
+struct Seq {};
+
+template <int N, int... Is>
+struct MakeSeq : MakeSeq<N - 1, N - 1, Is...> {};
+template <int... Is>
+struct MakeSeq<0, Is...> {
+  using type = Seq<Is...>;
+};
+using Dirs = MakeSeq<Q>::type;
+
+template <int... Is>
+__device__ __forceinline__ void equilibrium_all(Seq<Is...>, double rho, double ux, double uy, double uz,
+                                                double (&out)[Q]) {
+  const double uu = (ux * ux + uy * uy) + uz * uz;
+  ((out[Is] = feq_term<Is>(rho, ux, uy, uz, uu)), ...);
+}
+
+template <int I>
+__device__ __forceinline__ double guo_term(double pref, double uf, double ux, double uy, double uz,
+                                           double fx, double fy, double fz) {
+  const double cu = cdot<I>(ux, uy, uz);
+  const double cf = cdot<I>(fx, fy, fz);
+  return (pref * w_of(I)) * (((cf - uf) / kCs2) + ((cu * cf) / kCs2Sq));
+}
+
+template <int... Is>
+__device__ __forceinline__ void guo_all(Seq<Is...>, double pref, double ux, double uy, double uz, double fx,
+                                        double fy, double fz, double (&out)[Q]) {
+  const double uf = (ux * fx + uy * fy) + uz * fz;
+  ((out[Is] = guo_term<Is>(pref, uf, ux, uy, uz, fx, fy, fz)), ...);
+}
+
+struct Dims {
+  int nx, ny, nz;
+};
+
+// Cell of the thread in a (z-tiles, y-tiles, x) grid.
+__device__ __forceinline__ bool thread_cell(const Dims d, int& ix, int& iy, int& iz) {
+  iz = blockIdx.x * blockDim.x + threadIdx.x;
+  iy = blockIdx.y * blockDim.y + threadIdx.y;
+  ix = blockIdx.z;
+  return iz < d.nz && iy < d.ny;
+}
+
+__device__ __forceinline__ uint32_t cell_id(const Dims d, int x, int y, int z) {
+  return ((uint32_t)x * (uint32_t)d.ny + (uint32_t)y) * (uint32_t)d.nz + (uint32_t)z;
+}
+
+// Address of direction I's value for cell (ix,iy,iz) shifted by s*c_I.
+template <int I>
+__device__ __forceinline__ uint32_t shifted(const Dims d, int ix, int iy, int iz, int s) {
+  return cell_id(d, wrap_step(ix, s * cx_of(I), d.nx), wrap_step(iy, s * cy_of(I), d.ny),
+                 wrap_step(iz, s * cz_of(I), d.nz));
+}
+
+template <bool PULL, int... Is>
+__device__ __forceinline__ void load_all(Seq<Is...>, const double* __restrict__ in, size_t N, const Dims d,
+                                         int ix, int iy, int iz, double (&f)[Q]) {
+  ((f[Is] = in[(size_t)Is * N + (PULL ? shifted<Is>(d, ix, iy, iz, -1) : cell_id(d, ix, iy, iz))]), ...);
+}
+
+template <bool PULL, int... Is>
+__device__ __forceinline__ void store_all(Seq<Is...>, double* __restrict__ out, size_t N, const Dims d, int ix,
+                                          int iy, int iz, const double (&f)[Q]) {
+  ((out[(size_t)Is * N + (PULL ? cell_id(d, ix, iy, iz) : shifted<Is>(d, ix, iy, iz, 1))] = f[Is]), ...);
+}
+
+// ---------------------------------------------------------------------------
+// K2 (FP64 plain): psc3d.py:211-242 in the reference's operation order.
+// ---------------------------------------------------------------------------
+template <bool PULL, bool MOVING, bool FORCE, bool IMB>
+__global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin, double* __restrict__ fout,
+                                                  const uint32_t* __restrict__ csum,
+                                                  const double* __restrict__ usn, double* __restrict__ imb,
+                                                  Dims d, double s3, double tau, double pref, double fx,
+                                                  double fy, double fz) {
+  int ix, iy, iz;
+  if (!thread_cell(d, ix, iy, iz)) return;
+  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  const uint32_t cell = cell_id(d, ix, iy, iz);
+  double f[Q];
+  load_all<PULL>(Dirs{}, fin, N, d, ix, iy, iz, f);
+
+  // rho, u with the half-force shift (psc3d.py:211-214)
+  double rho = f[0];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) rho = rho + f[i];
+  const double ux = (moment<0>(f) + 0.5 * fx) / rho;
+  const double uy = (moment<1>(f) + 0.5 * fy) / rho;
+  const double uz = (moment<2>(f) + 0.5 * fz) / rho;
+  // second half-force shift (psc3d.py:217-219)
+  const double uxe = ux + 0.5 * fx / rho;
+  const double uye = uy + 0.5 * fy / rho;
+  const double uze = uz + 0.5 * fz / rho;
+
+  double feqf[Q];
+  equilibrium_all(Dirs{}, rho, uxe, uye, uze, feqf);
+
+  // solid fraction and B (psc3d.py:174-175, 228-229)
+  const uint32_t cnt = csum[cell];
+  const double asum = (double)cnt / s3;
+  const double solid = asum < 1.0 ? asum : 1.0;
+  const double b = solid / ((solid + tau * (1.0 - solid)) + 1.0e-14);
+
+  double fnew[Q];
+  double gterm[Q];
+  if (FORCE) guo_all(Dirs{}, pref, uxe, uye, uze, fx, fy, fz, gterm);
+
+  if (cnt == 0) {
+    // B = 0: collision = 1*Omega_bgk + 0*Omega_s
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const double obgk = -((f[i] - feqf[i]) / tau);
+      fnew[i] = FORCE ? f[i] + (obgk + gterm[i]) : f[i] + obgk;
+    }
+  } else {
+    double feqs[Q];
+    if (MOVING) {
+      // solid velocity: numerator / alpha_sum (psc3d.py:202-205)
+      double usx = usn[cell], usy = usn[N + cell], usz = usn[2 * N + cell];
+      if (asum > 1.0e-14) {
+        usx = usx / asum;
+        usy = usy / asum;
+        usz = usz / asum;
+      }
+      equilibrium_all(Dirs{}, rho, usx, usy, usz, feqs);
+    } else {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) feqs[i] = w_of(i) * rho;
+    }
+    double bos[Q];
+    const double omb = 1.0 - b;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int o = opp_of(i);
+      const double obgk = -((f[i] - feqf[i]) / tau);
+      const double os = ((f[o] - f[i]) + feqs[i]) - feqf[o];
+      bos[i] = b * os;
+      const double coll = omb * obgk + bos[i];
+      fnew[i] = FORCE ? f[i] + (coll + gterm[i]) : f[i] + coll;
+    }
+    if (IMB) {
+      imb[cell] = moment<0>(bos);
+      imb[N + cell] = moment<1>(bos);
+      imb[2 * N + cell] = moment<2>(bos);
+    }
+  }
+  store_all<PULL>(Dirs{}, fout, N, d, ix, iy, iz, fnew);
+}
+
+// ---------------------------------------------------------------------------
+// K1: sub-sample voxelization (psc3d.py:60-94 / 151-173) and the solid
+// velocity numerator (psc3d.py:190-201 / 352-367).
+// ---------------------------------------------------------------------------
+struct Particle {
+  double x, y, z, r, vx, vy, vz, wx, wy, wz;
+};
+
+__device__ __forceinline__ Particle load_particle(const double* __restrict__ p, int i) {
+  const double* q = p + 10 * (size_t)i;
+  return Particle{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9]};
+}
+
+// Count of samples (gx+o_a, gy+o_b, gz+o_c), o_k = (k+0.5)/s, with
+// (dx*dx + dy*dy) + dz*dz <= r*r.  Cells whose extreme samples decide the
+// answer for every sample are resolved without the s^3 loop: correctly
+// rounded subtraction, squaring and addition are monotonic, so the farthest
+// (nearest) sample bound computed with the same operations is exact.
+template <int S>
+__device__ __forceinline__ int sample_count_t(int gx, int gy, int gz, const Particle& p, double r2) {
+  double dx2[S], dy2[S], dz2[S];
+  double dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0, dz0 = 0, dz1 = 0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const double o = ((double)k + 0.5) / (double)S;
+    const double dx = ((double)gx + o) - p.x;
+    const double dy = ((double)gy + o) - p.y;
+    const double dz = ((double)gz + o) - p.z;
+    dx2[k] = dx * dx;
+    dy2[k] = dy * dy;
+    dz2[k] = dz * dz;
+    if (k == 0) { dx0 = dx; dy0 = dy; dz0 = dz; }
+    if (k == S - 1) { dx1 = dx; dy1 = dy; dz1 = dz; }
+  }
+  const double mx = fmax(dx2[0], dx2[S - 1]), my = fmax(dy2[0], dy2[S - 1]), mz = fmax(dz2[0], dz2[S - 1]);
+  if ((mx + my) + mz <= r2) return S * S * S;
+  const double nxm = (dx0 <= 0.0 && dx1 >= 0.0) ? 0.0 : fmin(dx2[0], dx2[S - 1]);
+  const double nym = (dy0 <= 0.0 && dy1 >= 0.0) ? 0.0 : fmin(dy2[0], dy2[S - 1]);
+  const double nzm = (dz0 <= 0.0 && dz1 >= 0.0) ? 0.0 : fmin(dz2[0], dz2[S - 1]);
+  if ((nxm + nym) + nzm > r2) return 0;
+  int count = 0;
+#pragma unroll
+  for (int a = 0; a < S; ++a)
+#pragma unroll
+    for (int bb = 0; bb < S; ++bb) {
+      const double sxy = dx2[a] + dy2[bb];
+#pragma unroll
+      for (int c = 0; c < S; ++c) count += (sxy + dz2[c] <= r2) ? 1 : 0;
+    }
+  return count;
+}
+
+__device__ int sample_count_any(int gx, int gy, int gz, const Particle& p, double r2, int s) {
+  const double ds = (double)s;
+  const double o0 = 0.5 / ds, o1 = ((double)(s - 1) + 0.5) / ds;
+  const double dx0 = ((double)gx + o0) - p.x, dx1 = ((double)gx + o1) - p.x;
+  const double dy0 = ((double)gy + o0) - p.y, dy1 = ((double)gy + o1) - p.y;
+  const double dz0 = ((double)gz + o0) - p.z, dz1 = ((double)gz + o1) - p.z;
+  const double mx = fmax(dx0 * dx0, dx1 * dx1), my = fmax(dy0 * dy0, dy1 * dy1), mz = fmax(dz0 * dz0, dz1 * dz1);
+  if ((mx + my) + mz <= r2) return s * s * s;
+  const double nxm = (dx0 <= 0.0 && dx1 >= 0.0) ? 0.0 : fmin(dx0 * dx0, dx1 * dx1);
+  const double nym = (dy0 <= 0.0 && dy1 >= 0.0) ? 0.0 : fmin(dy0 * dy0, dy1 * dy1);
+  const double nzm = (dz0 <= 0.0 && dz1 >= 0.0) ? 0.0 : fmin(dz0 * dz0, dz1 * dz1);
+  if ((nxm + nym) + nzm > r2) return 0;
+  int count = 0;
+  for (int a = 0; a < s; ++a) {
+    const double dx = ((double)gx + ((double)a + 0.5) / ds) - p.x;
+    for (int bb = 0; bb < s; ++bb) {
+      const double dy = ((double)gy + ((double)bb + 0.5) / ds) - p.y;
+      const double sxy = dx * dx + dy * dy;
+      for (int c = 0; c < s; ++c) {
+        const double dz = ((double)gz + ((double)c + 0.5) / ds) - p.z;
+        count += (sxy + dz * dz <= r2) ? 1 : 0;
+      }
+    }
+  }
+  return count;
+}
+
+__device__ __forceinline__ int sample_count(int gx, int gy, int gz, const Particle& p, double r2, int s) {
+  switch (s) {
+    case 1: return sample_count_t<1>(gx, gy, gz, p, r2);
+    case 2: return sample_count_t<2>(gx, gy, gz, p, r2);
+    case 3: return sample_count_t<3>(gx, gy, gz, p, r2);
+    case 4: return sample_count_t<4>(gx, gy, gz, p, r2);
+    case 5: return sample_count_t<5>(gx, gy, gz, p, r2);
+    case 6: return sample_count_t<6>(gx, gy, gz, p, r2);
+    default: return sample_count_any(gx, gy, gz, p, r2, s);
+  }
+}
+
+struct Box {
+  int x0, y0, z0, ex, ey, ez, off;
+};
+__device__ __forceinline__ Box load_box(const int32_t* __restrict__ b, int p) {
+  const int32_t* q = b + 8 * (size_t)p;
+  return Box{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
+}
+
+template <int USP>
+__global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ parts, const int32_t* __restrict__ boxes,
+                                                  const int32_t* __restrict__ chunks, int chunk_cells, Dims d,
+                                                  int s, uint16_t* __restrict__ counts,
+                                                  uint32_t* __restrict__ csum, void* __restrict__ usn) {
+  const int pid = chunks[2 * blockIdx.x];
+  const int e0 = chunks[2 * blockIdx.x + 1];
+  const Box bx = load_box(boxes, pid);
+  const Particle p = load_particle(parts, pid);
+  const int vol = bx.ex * bx.ey * bx.ez;
+  const int e1 = min(e0 + chunk_cells, vol);
+  const double r2 = p.r * p.r;
+  const double s3 = (double)(s * s * s);
+  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+    const int lz = e % bx.ez;
+    const int t = e / bx.ez;
+    const int ly = t % bx.ey;
+    const int lx = t / bx.ey;
+    const int gx = bx.x0 + lx, gy = bx.y0 + ly, gz = bx.z0 + lz;
+    const int cnt = sample_count(gx, gy, gz, p, r2, s);
+    counts[(size_t)bx.off + e] = (uint16_t)cnt;
+    if (cnt == 0) continue;
+    const uint32_t cell = cell_id(d, wrap_any(gx, d.nx), wrap_any(gy, d.ny), wrap_any(gz, d.nz));
+    atomicAdd(csum + cell, (uint32_t)cnt);
+    if (USP != 0) {
+      const double a = (double)cnt / s3;
+      const double rx = ((double)gx + 0.5) - p.x;
+      const double ry = ((double)gy + 0.5) - p.y;
+      const double rz = ((double)gz + 0.5) - p.z;
+      const double vx = (p.vx + p.wy * rz) - p.wz * ry;
+      const double vy = (p.vy + p.wz * rx) - p.wx * rz;
+      const double vz = (p.vz + p.wx * ry) - p.wy * rx;
+      if (USP == 2) {
+        double* u = (double*)usn;
+        unsafeAtomicAdd(u + cell, a * vx);
+        unsafeAtomicAdd(u + N + cell, a * vy);
+        unsafeAtomicAdd(u + 2 * N + cell, a * vz);
+      } else {
+        float* u = (float*)usn;
+        unsafeAtomicAdd(u + cell, (float)(a * vx));
+        unsafeAtomicAdd(u + N + cell, (float)(a * vy));
+        unsafeAtomicAdd(u + 2 * N + cell, (float)(a * vz));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K3: node-summed force/torque per particle (psc3d.py:255-281, 375-407).
+// ---------------------------------------------------------------------------
+template <typename TI>
+__global__ __launch_bounds__(256) void k_loads_partial(const double* __restrict__ parts,
+                                                       const int32_t* __restrict__ boxes,
+                                                       const int32_t* __restrict__ chunks, int chunk_cells,
+                                                       const uint16_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ csum,
+                                                       const TI* __restrict__ imb, Dims d, double s3,
+                                                       double* __restrict__ partials) {
+  __shared__ double lds[6 * 16];
+  const int pid = chunks[2 * blockIdx.x];
+  const int e0 = chunks[2 * blockIdx.x + 1];
+  const Box bx = load_box(boxes, pid);
+  const Particle p = load_particle(parts, pid);
+  const int vol = bx.ex * bx.ey * bx.ez;
+  const int e1 = min(e0 + chunk_cells, vol);
+  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+    const int cnt = counts[(size_t)bx.off + e];
+    if (cnt == 0) continue;
+    const int lz = e % bx.ez;
+    const int t = e / bx.ez;
+    const int ly = t % bx.ey;
+    const int lx = t / bx.ey;
+    const int gx = bx.x0 + lx, gy = bx.y0 + ly, gz = bx.z0 + lz;
+    const uint32_t cell = cell_id(d, wrap_any(gx, d.nx), wrap_any(gy, d.ny), wrap_any(gz, d.nz));
+    const double alpha = (double)cnt / s3;
+    const double asum = (double)csum[cell] / s3;
+    const double owner = alpha / fmax(asum, 1.0e-14);
+    const double nfx = -owner * (double)imb[cell];
+    const double nfy = -owner * (double)imb[N + cell];
+    const double nfz = -owner * (double)imb[2 * N + cell];
+    const double rx = ((double)gx + 0.5) - p.x;
+    const double ry = ((double)gy + 0.5) - p.y;
+    const double rz = ((double)gz + 0.5) - p.z;
+    acc[0] += nfx;
+    acc[1] += nfy;
+    acc[2] += nfz;
+    acc[3] += ry * nfz - rz * nfy;
+    acc[4] += rz * nfx - rx * nfz;
+    acc[5] += rx * nfy - ry * nfx;
+  }
+  block_sum<6>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) partials[6 * (size_t)blockIdx.x + k] = acc[k];
+  }
+}
+
+__global__ void k_loads_final(const int32_t* __restrict__ chunk_first, int np, const double* __restrict__ partials,
+                              double* __restrict__ loads) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = chunk_first[p]; c < chunk_first[p + 1]; ++c)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[k] += partials[6 * (size_t)c + k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) loads[6 * (size_t)p + k] = acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// K4: post-stream moments and FP64 readout sums (psc3d.py:243-253).
+// ---------------------------------------------------------------------------
+constexpr int kMomThreads = 256;
+constexpr int64_t kMomMaxBlocks = 2048;
+
+template <typename TS, bool DEV, bool PULL, bool HAVE_STATE, typename TI>
+__global__ __launch_bounds__(kMomThreads) void k_moments(const TS* __restrict__ st, const uint32_t* __restrict__ csum,
+                                                         const TI* __restrict__ imb, Dims d, double s3, double hfx,
+                                                         double hfy, double hfz, double* __restrict__ rho_u,
+                                                         double* __restrict__ partials) {
+  __shared__ double lds[PSC_NSUM * 16];
+  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  double acc[PSC_NSUM];
+#pragma unroll
+  for (int k = 0; k < PSC_NSUM; ++k) acc[k] = 0.0;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t cnt = csum[c];
+    const double asum = (double)cnt / s3;
+    const double solid = asum < 1.0 ? asum : 1.0;
+    const double fl = 1.0 - solid;
+    acc[0] += solid;
+    acc[1] += fl;
+    if (imb != nullptr && cnt != 0) {
+      acc[8] += (double)imb[c];
+      acc[9] += (double)imb[N + c];
+      acc[10] += (double)imb[2 * N + c];
+    }
+    if (HAVE_STATE) {
+      const int iz = (int)(c % d.nz);
+      const size_t t = c / d.nz;
+      const int iy = (int)(t % d.ny);
+      const int ix = (int)(t / d.ny);
+      double f[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const uint32_t src = PULL ? cell_id(d, wrap_step(ix, -cx_of(i), d.nx), wrap_step(iy, -cy_of(i), d.ny),
+                                            wrap_step(iz, -cz_of(i), d.nz))
+                                  : (uint32_t)c;
+        f[i] = (double)st[(size_t)i * N + src];
+      }
+      double rho;
+      if (DEV) {
+        // rho = 1 + sum g
+        double s = f[0];
+#pragma unroll
+        for (int i = 1; i < Q; ++i) s = s + f[i];
+        rho = 1.0 + s;
+      } else {
+        rho = f[0];
+#pragma unroll
+        for (int i = 1; i < Q; ++i) rho = rho + f[i];
+      }
+      const double ux = (moment<0>(f) + hfx) / rho;
+      const double uy = (moment<1>(f) + hfy) / rho;
+      const double uz = (moment<2>(f) + hfz) / rho;
+      if (rho_u != nullptr) {
+        rho_u[c] = rho;
+        rho_u[N + c] = ux;
+        rho_u[2 * N + c] = uy;
+        rho_u[3 * N + c] = uz;
+      }
+      const bool finite = isfinite(rho) && isfinite(ux) && isfinite(uy) && isfinite(uz);
+      acc[2] += ux;
+      acc[3] += uy;
+      acc[4] += uz;
+      acc[5] += fl * ux;
+      acc[6] += fl * uy;
+      acc[7] += fl * uz;
+      acc[11] += rho;
+      acc[12] += finite ? 0.0 : 1.0;
+    }
+  }
+  block_sum<PSC_NSUM>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < PSC_NSUM; ++k) partials[(size_t)PSC_NSUM * blockIdx.x + k] = acc[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partials, int nblk,
+                                                      double* __restrict__ out) {
+  __shared__ double lds[PSC_NSUM * 16];
+  double acc[PSC_NSUM];
+#pragma unroll
+  for (int k = 0; k < PSC_NSUM; ++k) acc[k] = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < PSC_NSUM; ++k) acc[k] += partials[(size_t)PSC_NSUM * b + k];
+  block_sum<PSC_NSUM>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < PSC_NSUM; ++k) out[k] = acc[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// f export / import and state init
+// ---------------------------------------------------------------------------
+template <typename TS, bool DEV, bool PULL>
+__global__ void k_export(const TS* __restrict__ st, Dims d, double* __restrict__ fout) {
+  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(c % d.nz);
+    const size_t t = c / d.nz;
+    const int iy = (int)(t % d.ny);
+    const int ix = (int)(t / d.ny);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const uint32_t src = PULL ? cell_id(d, wrap_step(ix, -cx_of(i), d.nx), wrap_step(iy, -cy_of(i), d.ny),
+                                          wrap_step(iz, -cz_of(i), d.nz))
+                                : (uint32_t)c;
+      const double v = (double)st[(size_t)i * N + src];
+      fout[(size_t)i * N + c] = DEV ? w_of(i) + v : v;
+    }
+  }
+}
+
+template <typename TS, bool DEV, bool PULL>
+__global__ void k_import(const double* __restrict__ fin, Dims d, TS* __restrict__ st) {
+  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(c % d.nz);
+    const size_t t = c / d.nz;
+    const int iy = (int)(t % d.ny);
+    const int ix = (int)(t / d.ny);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      // the state at y holds the value that export reads back at y + c_i
+      const uint32_t src = PULL ? cell_id(d, wrap_step(ix, cx_of(i), d.nx), wrap_step(iy, cy_of(i), d.ny),
+                                          wrap_step(iz, cz_of(i), d.nz))
+                                : (uint32_t)c;
+      const double v = fin[(size_t)i * N + src];
+      st[(size_t)i * N + c] = (TS)(DEV ? v - w_of(i) : v);
+    }
+  }
+}
+
+__global__ void k_init_plain(double* __restrict__ f, size_t N) {
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[(size_t)i * N + c] = w_of(i) * 1.0;
+  }
+}
+
+int grid_for(size_t n, int threads, int cap = 8192) {
+  size_t g = (n + threads - 1) / threads;
+  if (g > (size_t)cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace
+
+// ------------------------------- launchers ---------------------------------
+int launch_step_f64(const double* in, double* out, const uint32_t* csum, const double* usn, double* imb,
+                    int layout, int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz,
+                    int bz, int by, hipStream_t stream) {
+  const Dims d{nx, ny, nz};
+  const dim3 block(bz, by, 1);
+  const dim3 grid((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
+  const double s3 = (double)(s * s * s);
+  const double pref = 1.0 - 0.5 / tau;
+  const bool force = (fx != 0.0 || fy != 0.0 || fz != 0.0);
+  const bool moving = usn != nullptr;
+  const bool pull = layout == PSC_PULL;
+  const bool wimb = imb != nullptr;
+#define PSC_F64_CASE(P, M, F, I)                                                                      \
+  if (pull == P && moving == M && force == F && wimb == I) {                                        \
+    k_step_f64<P, M, F, I><<<grid, block, 0, stream>>>(in, out, csum, usn, imb, d, s3, tau, pref, fx, fy, fz); \
+    return check_launch("psc_step(f64)");                                                           \
+  }
+  PSC_F64_CASE(false, false, false, false)
+  PSC_F64_CASE(false, false, false, true)
+  PSC_F64_CASE(false, false, true, false)
+  PSC_F64_CASE(false, false, true, true)
+  PSC_F64_CASE(false, true, false, false)
+  PSC_F64_CASE(false, true, false, true)
+  PSC_F64_CASE(false, true, true, false)
+  PSC_F64_CASE(false, true, true, true)
+  PSC_F64_CASE(true, false, false, false)
+  PSC_F64_CASE(true, false, false, true)
+  PSC_F64_CASE(true, false, true, false)
+  PSC_F64_CASE(true, false, true, true)
+  PSC_F64_CASE(true, true, false, false)
+  PSC_F64_CASE(true, true, false, true)
+  PSC_F64_CASE(true, true, true, false)
+  PSC_F64_CASE(true, true, true, true)
+#undef PSC_F64_CASE
+  return set_error(PSC_EINVAL, "psc_step(f64): no kernel variant");
+}
+
+int launch_voxelize(const double* parts, const int32_t* boxes, const int32_t* chunks, int nchunks, int chunk_cells,
+                    int nx, int ny, int nz, int s, uint16_t* counts, uint32_t* csum, void* usn, int usp,
+                    hipStream_t stream) {
+  if (nchunks == 0) return PSC_OK;
+  const Dims d{nx, ny, nz};
+  const int threads = chunk_cells < 256 ? ((chunk_cells + 63) / 64) * 64 : 256;
+  if (usn == nullptr || usp == 0)
+    k_voxelize<0><<<nchunks, threads, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, counts, csum, nullptr);
+  else if (usp == 1)
+    k_voxelize<1><<<nchunks, threads, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, counts, csum, usn);
+  else
+    k_voxelize<2><<<nchunks, threads, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, counts, csum, usn);
+  return check_launch("psc_voxelize");
+}
+
+int launch_loads(const double* parts, const int32_t* boxes, const int32_t* chunks, const int32_t* chunk_first, int np,
+                 int nchunks, int chunk_cells, const uint16_t* counts, const uint32_t* csum, const void* imb, int imbp,
+                 int nx, int ny, int nz, int s, double* partials, double* loads, hipStream_t stream) {
+  const Dims d{nx, ny, nz};
+  const double s3 = (double)(s * s * s);
+  if (nchunks > 0) {
+    if (imbp == 2)
+      k_loads_partial<double><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, counts, csum,
+                                                           (const double*)imb, d, s3, partials);
+    else
+      k_loads_partial<float><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, counts, csum,
+                                                          (const float*)imb, d, s3, partials);
+    int rc = check_launch("psc_particle_loads(partial)");
+    if (rc) return rc;
+  }
+  if (np > 0) {
+    k_loads_final<<<(np + 127) / 128, 128, 0, stream>>>(chunk_first, np, partials, loads);
+    return check_launch("psc_particle_loads(final)");
+  }
+  return PSC_OK;
+}
+
+int64_t moments_blocks(int64_t n) {
+  int64_t b = (n + (int64_t)kMomThreads * 8 - 1) / ((int64_t)kMomThreads * 8);
+  if (b > kMomMaxBlocks) b = kMomMaxBlocks;
+  if (b < 1) b = 1;
+  return b;
+}
+
+int launch_moments(const void* st, int storage, int layout, const uint32_t* csum, const void* imb, int nx, int ny,
+                   int nz, int s, double fx, double fy, double fz, double* rho_u, double* partials, double* sums,
+                   hipStream_t stream) {
+  const Dims d{nx, ny, nz};
+  const size_t N = (size_t)nx * ny * nz;
+  const int nblk = (int)moments_blocks((int64_t)N);
+  const double s3 = (double)(s * s * s);
+  const double hx = 0.5 * fx, hy = 0.5 * fy, hz = 0.5 * fz;
+  const bool pull = layout == PSC_PULL;
+  if (storage == PSC_F32_DEV) {
+    const float* g = (const float*)st;
+    const float* im = (const float*)imb;
+    if (g == nullptr)
+      k_moments<float, true, false, false, float><<<nblk, kMomThreads, 0, stream>>>(g, csum, im, d, s3, hx, hy, hz,
+                                                                                    rho_u, partials);
+    else if (pull)
+      k_moments<float, true, true, true, float><<<nblk, kMomThreads, 0, stream>>>(g, csum, im, d, s3, hx, hy, hz,
+                                                                                  rho_u, partials);
+    else
+      k_moments<float, true, false, true, float><<<nblk, kMomThreads, 0, stream>>>(g, csum, im, d, s3, hx, hy, hz,
+                                                                                   rho_u, partials);
+  } else {
+    const double* f = (const double*)st;
+    const double* im = (const double*)imb;
+    if (f == nullptr)
+      k_moments<double, false, false, false, double><<<nblk, kMomThreads, 0, stream>>>(f, csum, im, d, s3, hx, hy,
+                                                                                       hz, rho_u, partials);
+    else if (pull)
+      k_moments<double, false, true, true, double><<<nblk, kMomThreads, 0, stream>>>(f, csum, im, d, s3, hx, hy, hz,
+                                                                                     rho_u, partials);
+    else
+      k_moments<double, false, false, true, double><<<nblk, kMomThreads, 0, stream>>>(f, csum, im, d, s3, hx, hy,
+                                                                                      hz, rho_u, partials);
+  }
+  int rc = check_launch("psc_moments");
+  if (rc) return rc;
+  k_sum_partials<<<1, 256, 0, stream>>>(partials, nblk, sums);
+  return check_launch("psc_moments(final)");
+}
+
+int launch_export(const void* st, int storage, int layout, int nx, int ny, int nz, double* f, hipStream_t stream) {
+  const Dims d{nx, ny, nz};
+  const size_t N = (size_t)nx * ny * nz;
+  const int g = grid_for(N, 256);
+  const bool pull = layout == PSC_PULL;
+  if (storage == PSC_F32_DEV) {
+    if (pull) k_export<float, true, true><<<g, 256, 0, stream>>>((const float*)st, d, f);
+    else k_export<float, true, false><<<g, 256, 0, stream>>>((const float*)st, d, f);
+  } else {
+    if (pull) k_export<double, false, true><<<g, 256, 0, stream>>>((const double*)st, d, f);
+    else k_export<double, false, false><<<g, 256, 0, stream>>>((const double*)st, d, f);
+  }
+  return check_launch("psc_export_f64");
+}
+
+int launch_import(const double* f, int storage, int layout, int nx, int ny, int nz, void* st, hipStream_t stream) {
+  const Dims d{nx, ny, nz};
+  const size_t N = (size_t)nx * ny * nz;
+  const int g = grid_for(N, 256);
+  const bool pull = layout == PSC_PULL;
+  if (storage == PSC_F32_DEV) {
+    if (pull) k_import<float, true, true><<<g, 256, 0, stream>>>(f, d, (float*)st);
+    else k_import<float, true, false><<<g, 256, 0, stream>>>(f, d, (float*)st);
+  } else {
+    if (pull) k_import<double, false, true><<<g, 256, 0, stream>>>(f, d, (double*)st);
+    else k_import<double, false, false><<<g, 256, 0, stream>>>(f, d, (double*)st);
+  }
+  return check_launch("psc_import_f64");
+}
+
+int launch_init_plain(double* f, size_t N, hipStream_t stream) {
+  k_init_plain<<<grid_for(N, 256), 256, 0, stream>>>(f, N);
+  return check_launch("psc_init_state");
+}
+
+}  // namespace psc

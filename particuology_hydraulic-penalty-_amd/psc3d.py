@@ -1,0 +1,609 @@
+"""Drop-in MI355X implementation of ``src/lbm_dem/psc3d.py``.
+
+Same public names as the reference module (``C, W, OPP, CS2, Particle3D,
+PscD3Q19, SparsePscD3Q19``), same constructor signatures, methods, attributes
+and ``diag`` keys (psc3d.py:17-407).  The lattice lives in HBM as torch-ROCm
+tensors; every step runs the hand-written gfx950 kernels of libpsc_hip.so
+(voxelize, fused collide+stream, particle loads, moments).  NumPy arrays are
+produced only when an attribute is read.
+
+Differences a caller can opt into (keywords after ``*``):
+
+* ``precision="fp64"`` (default) runs the FP64 plain step in the reference's
+  operation order: f, u, imb and the solid field equal the reference bit for
+  bit on cells owned by one particle.  ``precision="fp32"`` stores
+  g = f - w in FP32 (half the bytes, the fast path).
+* ``body_force=(fx, fy, fz)`` generalises ``body_force_z``.
+* ``periodic_geometry=True`` wraps sphere boxes around the periodic lattice
+  instead of clamping them (psc3d.py:73-78 clamps).
+* ``stream_layout="pull"|"push"`` selects the streaming form of the kernel;
+  both give identical results.
+* ``diag="lazy"|"eager"``: by default the returned diagnostics are computed
+  when first read, which must happen before the next step.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+import math
+
+import numpy as np
+
+from . import _lib
+from .geometry import build_tables, pack_particles
+
+# Lattice constants (psc3d.py:17-43): identical values and order.
+C = np.array(
+    [[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+     [1, 1, 0], [-1, -1, 0], [1, -1, 0], [-1, 1, 0], [1, 0, 1], [-1, 0, -1], [1, 0, -1],
+     [-1, 0, 1], [0, 1, 1], [0, -1, -1], [0, 1, -1], [0, -1, 1]],
+    dtype=int,
+)
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=float)
+OPP = np.array([0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17])
+CS2 = 1.0 / 3.0
+
+DIAG_KEYS = ("solid_volume_lu", "mean_uz", "momentum_residual", "max_force", "max_torque")
+
+
+@dataclass
+class Particle3D:
+    """Sphere in lattice units (psc3d.py:46-57).  Mutable: with dynamic
+    geometry, edits take effect at the next step."""
+
+    x: float
+    y: float
+    z: float
+    r: float
+    vx: float = 0.0
+    vy: float = 0.0
+    vz: float = 0.0
+    wx: float = 0.0
+    wy: float = 0.0
+    wz: float = 0.0
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class StaleDiagnostics(RuntimeError):
+    """A lazily evaluated diag dict was read after the solver advanced."""
+
+
+class LazyDiag(dict):
+    """The dict ``collide_stream_psc`` returns (keys of psc3d.py:247-253).
+
+    Values are computed on the GPU at the first read, which must come before
+    the solver's next step (otherwise StaleDiagnostics is raised)."""
+
+    def __init__(self, solver, step):
+        super().__init__()
+        self._solver = solver
+        self._step = step
+        self._ready = False
+
+    def _fill(self):
+        if self._ready:
+            return
+        if self._solver._step_count != self._step:
+            raise StaleDiagnostics(
+                "diagnostics of step %d read after step %d; read them before the next step or construct the "
+                "solver with diag='eager'" % (self._step, self._solver._step_count))
+        super().update(self._solver._diagnostics())
+        self._ready = True
+        self._solver = None
+
+    def __getitem__(self, k):
+        self._fill()
+        return super().__getitem__(k)
+
+    def get(self, k, default=None):
+        self._fill()
+        return super().get(k, default)
+
+    def __iter__(self):
+        self._fill()
+        return super().__iter__()
+
+    def __len__(self):
+        self._fill()
+        return super().__len__()
+
+    def __contains__(self, k):
+        self._fill()
+        return super().__contains__(k)
+
+    def keys(self):
+        self._fill()
+        return super().keys()
+
+    def values(self):
+        self._fill()
+        return super().values()
+
+    def items(self):
+        self._fill()
+        return super().items()
+
+    def copy(self):
+        self._fill()
+        return dict(super().items())
+
+    def __repr__(self):
+        self._fill()
+        return super().__repr__()
+
+    def __eq__(self, other):
+        self._fill()
+        return dict(super().items()) == other
+
+    __hash__ = None
+
+
+class PscD3Q19:
+    """D3Q19 PSC-IMB solver with node-summed particle loads (psc3d.py:117-281)."""
+
+    _LAYOUT_NAME = "dense"
+
+    def __init__(self, n, particles, tau: float = 0.72, subsamples: int = 4, body_force_z: float = 0.0,
+                 static_geometry: bool = False, *, body_force=None, precision: str = "fp64",
+                 stream_layout: str = "pull", periodic_geometry: bool = False, device=None,
+                 chunk_cells: int = 2048, block=(0, 0), diag: str = "lazy"):
+        torch = _torch()
+        self._lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise _lib.PscError("PscD3Q19 needs a ROCm GPU (gfx950); torch.cuda.is_available() is False")
+        if isinstance(n, (tuple, list)):
+            shape = tuple(int(v) for v in n)
+            if len(shape) != 3:
+                raise ValueError("n must be an int or a 3-tuple")
+        else:
+            shape = (int(n),) * 3
+        self._shape = shape
+        self.n = shape[0]
+        self._N = shape[0] * shape[1] * shape[2]
+        self._particles = particles
+        self.tau = float(tau)
+        if not self.tau > 0.5:
+            raise ValueError("tau must exceed 1/2")
+        self.subsamples = int(subsamples)
+        self._s = max(1, self.subsamples)
+        if body_force is not None:
+            self._force = tuple(float(v) for v in body_force)
+            if len(self._force) != 3:
+                raise ValueError("body_force must have 3 components")
+        else:
+            self._force = (0.0, 0.0, float(body_force_z))
+        self.static_geometry = bool(static_geometry)
+        self.periodic_geometry = bool(periodic_geometry)
+        if precision not in ("fp32", "fp64"):
+            raise ValueError("precision must be 'fp32' or 'fp64'")
+        self.precision = precision
+        self._storage = _lib.PSC_F32_DEV if precision == "fp32" else _lib.PSC_F64_PLAIN
+        if stream_layout not in ("pull", "push"):
+            raise ValueError("stream_layout must be 'pull' or 'push'")
+        self.stream_layout = stream_layout
+        self._layout = _lib.PSC_PULL if stream_layout == "pull" else _lib.PSC_PUSH
+        if diag not in ("lazy", "eager"):
+            raise ValueError("diag must be 'lazy' or 'eager'")
+        self._diag_mode = diag
+        self._chunk_cells = int(chunk_cells)
+        self._block = (int(block[0]), int(block[1]))
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self._fdt = torch.float32 if self._storage == _lib.PSC_F32_DEV else torch.float64
+        N = self._N
+        self._a = torch.empty(19 * N, dtype=self._fdt, device=self.device)
+        self._b = torch.empty(19 * N, dtype=self._fdt, device=self.device)
+        self._csum = torch.zeros(N, dtype=torch.int32, device=self.device)
+        self._imb = torch.zeros(3 * N, dtype=self._fdt, device=self.device)
+        self._us = None
+        self._counts = torch.zeros(1, dtype=torch.int16, device=self.device)
+        self._tables = None
+        self._dev_tables = None
+        self._loads = torch.zeros((0, 6), dtype=torch.float64, device=self.device)
+        self._partials = torch.zeros(6, dtype=torch.float64, device=self.device)
+        nblk = int(self._lib.psc_moments_blocks(N))
+        self._mom_partials = torch.zeros(nblk * _lib.PSC_NSUM, dtype=torch.float64, device=self.device)
+        self._sums = torch.zeros(_lib.PSC_NSUM, dtype=torch.float64, device=self.device)
+        self._rho_u = None
+        self._geometry_built = False
+        self._step_count = 0
+        self._cache = {}
+        _lib.call("psc_init_state", self._ptr(self._a), self._storage, N, self._stream())
+        self.geometry_workers = 1
+
+    # ------------------------------------------------------------------ utils
+    def _stream(self):
+        return _torch().cuda.current_stream(self.device).cuda_stream
+
+    @staticmethod
+    def _ptr(t):
+        return None if t is None else t.data_ptr()
+
+    def _np(self, t):
+        return t.detach().cpu().numpy()
+
+    # ------------------------------------------------------------- particles
+    @property
+    def particles(self):
+        return self._particles
+
+    @particles.setter
+    def particles(self, value):
+        self._particles = value
+        self._cache.pop("hydro_forces", None)
+        self._cache.pop("hydro_torques", None)
+
+    @property
+    def body_force_z(self):
+        return self._force[2]
+
+    @property
+    def body_force(self):
+        return self._force
+
+    # --------------------------------------------------------------- geometry
+    def rebuild_geometry(self) -> None:
+        """Voxelize every sphere (psc3d.py:151-179 / 328-345) on the GPU."""
+        torch = _torch()
+        table = pack_particles(self._particles)
+        tabs = build_tables(table, self._shape, self.periodic_geometry, self._chunk_cells)
+        dev = self.device
+        parts = torch.from_numpy(tabs.particles).to(dev)
+        boxes = torch.from_numpy(tabs.boxes).to(dev)
+        chunks = torch.from_numpy(tabs.chunks).to(dev)
+        cfirst = torch.from_numpy(tabs.chunk_first).to(dev)
+        if self._counts.numel() < max(1, tabs.n_entries):
+            self._counts = torch.empty(max(1, int(tabs.n_entries * 1.25)), dtype=torch.int16, device=dev)
+        if self._partials.numel() < 6 * max(1, tabs.chunks.shape[0]):
+            self._partials = torch.empty(6 * max(1, int(tabs.chunks.shape[0] * 1.25)), dtype=torch.float64,
+                                         device=dev)
+        P = table.shape[0]
+        if self._loads.shape[0] != P:
+            self._loads = torch.zeros((P, 6), dtype=torch.float64, device=dev)
+        stream = self._stream()
+        N = self._N
+        _lib.call("psc_clear", self._ptr(self._csum), 4 * N, stream)
+        usp = 0
+        if tabs.moving:
+            if self._us is None:
+                self._us = torch.empty(3 * N, dtype=self._fdt, device=dev)
+            _lib.call("psc_clear", self._ptr(self._us), 3 * N * self._us.element_size(), stream)
+            usp = 1 if self._storage == _lib.PSC_F32_DEV else 2
+        us_ptr = self._ptr(self._us) if tabs.moving else None
+        nx, ny, nz = self._shape
+        _lib.call("psc_voxelize", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), int(tabs.chunks.shape[0]),
+                  tabs.chunk_cells, nx, ny, nz, self._s, self._ptr(self._counts), self._ptr(self._csum), us_ptr,
+                  usp, stream)
+        self._tables = tabs
+        # keep the device tables alive until the kernels that read them have run
+        self._dev_tables = (parts, boxes, chunks, cfirst)
+        self._moving = tabs.moving
+        self._geometry_built = True
+        self._cache.clear()
+
+    # ------------------------------------------------------------------- step
+    def collide_stream_psc(self):
+        """One PSC collide + stream step (psc3d.py:208-253) on the GPU."""
+        self._advance()
+        d = LazyDiag(self, self._step_count)
+        if self._diag_mode == "eager":
+            d._fill()
+        return d
+
+    def run(self, steps: int) -> None:
+        """``steps`` calls of collide_stream_psc without building diag dicts."""
+        for _ in range(int(steps)):
+            self._advance()
+
+    def _advance(self):
+        if (not self.static_geometry) or (not self._geometry_built):
+            self.rebuild_geometry()
+        nx, ny, nz = self._shape
+        stream = self._stream()
+        fx, fy, fz = self._force
+        us_ptr = self._ptr(self._us) if self._moving else None
+        _lib.call("psc_step", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum), us_ptr,
+                  self._ptr(self._imb), self._storage, self._layout, nx, ny, nz, self._s, self.tau, fx, fy, fz,
+                  self._block[0], self._block[1], stream)
+        self._a, self._b = self._b, self._a
+        self._step_count += 1
+        self._cache.clear()
+        self.accumulate_particle_loads()
+
+    def accumulate_particle_loads(self) -> None:
+        """Node-summed force and torque per particle (psc3d.py:255-281 / 375-407)."""
+        tabs = self._tables
+        if tabs is None:
+            return
+        parts, boxes, chunks, cfirst = self._dev_tables
+        nx, ny, nz = self._shape
+        imbp = 1 if self._storage == _lib.PSC_F32_DEV else 2
+        _lib.call("psc_particle_loads", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), self._ptr(cfirst),
+                  int(tabs.particles.shape[0]), int(tabs.chunks.shape[0]), tabs.chunk_cells,
+                  self._ptr(self._counts), self._ptr(self._csum), self._ptr(self._imb), imbp, nx, ny, nz, self._s,
+                  self._ptr(self._partials), self._ptr(self._loads), self._stream())
+        self._cache.pop("hydro_forces", None)
+        self._cache.pop("hydro_torques", None)
+        self._cache.pop("loads", None)
+
+    # --------------------------------------------------------------- readouts
+    def _moments(self, fields: bool, with_state: bool = True):
+        torch = _torch()
+        nx, ny, nz = self._shape
+        if fields and self._rho_u is None:
+            self._rho_u = torch.empty(4 * self._N, dtype=torch.float64, device=self.device)
+        fx, fy, fz = self._force
+        _lib.call("psc_moments", self._ptr(self._a) if with_state else None, self._storage, self._layout,
+                  self._ptr(self._csum), self._ptr(self._imb), nx, ny, nz, self._s, fx, fy, fz,
+                  self._ptr(self._rho_u) if fields else None, self._ptr(self._mom_partials), self._ptr(self._sums),
+                  self._stream())
+        return self._np(self._sums)
+
+    def _fields(self):
+        if "fields" not in self._cache:
+            sums = self._moments(True)
+            self._cache["sums"] = sums
+            self._cache["fields"] = self._np(self._rho_u).reshape((4,) + self._shape)
+        return self._cache["fields"]
+
+    def readout_sums(self) -> dict:
+        """FP64 sums of the post-stream state (see psc_moments in psc_hip.h)."""
+        if "sums" not in self._cache:
+            self._cache["sums"] = self._moments(False)
+        s = self._cache["sums"]
+        return {
+            "solid": s[0], "fluid": s[1], "u": s[2:5].copy(), "fluid_u": s[5:8].copy(), "imb": s[8:11].copy(),
+            "rho": s[11], "nonfinite": int(s[12]),
+        }
+
+    def _geometry_sums(self):
+        if "gsums" not in self._cache:
+            self._cache["gsums"] = self._moments(False, with_state=False)
+        return self._cache["gsums"]
+
+    @property
+    def rho(self):
+        return self._fields()[0]
+
+    @property
+    def ux(self):
+        return self._fields()[1]
+
+    @property
+    def uy(self):
+        return self._fields()[2]
+
+    @property
+    def uz(self):
+        return self._fields()[3]
+
+    @property
+    def f(self):
+        """Post-stream distributions (19, n, n, n) float64 (psc3d.py:242)."""
+        if "f" not in self._cache:
+            torch = _torch()
+            out = torch.empty(19 * self._N, dtype=torch.float64, device=self.device)
+            nx, ny, nz = self._shape
+            _lib.call("psc_export_f64", self._ptr(self._a), self._storage, self._layout, nx, ny, nz,
+                      self._ptr(out), self._stream())
+            self._cache["f"] = self._np(out).reshape((19,) + self._shape)
+        return self._cache["f"]
+
+    @f.setter
+    def f(self, value):
+        torch = _torch()
+        arr = np.ascontiguousarray(np.asarray(value, dtype=np.float64).reshape(19 * self._N))
+        src = torch.from_numpy(arr).to(self.device)
+        nx, ny, nz = self._shape
+        _lib.call("psc_import_f64", self._ptr(src), self._storage, self._layout, nx, ny, nz, self._ptr(self._a),
+                  self._stream())
+        _torch().cuda.current_stream(self.device).synchronize()
+        self._cache.clear()
+
+    def _alpha_sum_np(self):
+        if "asum" not in self._cache:
+            cnt = self._np(self._csum).astype(np.float64)
+            self._cache["asum"] = (cnt / float(self._s ** 3)).reshape(self._shape)
+        return self._cache["asum"]
+
+    @property
+    def solid(self):
+        """clip(sum_p alpha_p, 0, 1) (psc3d.py:175 / 344)."""
+        if "solid" not in self._cache:
+            self._cache["solid"] = np.clip(self._alpha_sum_np(), 0.0, 1.0)
+        return self._cache["solid"]
+
+    def _imb_np(self):
+        if "imb" not in self._cache:
+            mask = self._np(self._csum).reshape(self._shape) != 0
+            imb = self._np(self._imb).astype(np.float64).reshape((3,) + self._shape)
+            self._cache["imb"] = np.where(mask[None], imb, 0.0)
+        return self._cache["imb"]
+
+    @property
+    def imb_mx(self):
+        return self._imb_np()[0]
+
+    @property
+    def imb_my(self):
+        return self._imb_np()[1]
+
+    @property
+    def imb_mz(self):
+        return self._imb_np()[2]
+
+    def _loads_np(self):
+        if "loads" not in self._cache:
+            self._cache["loads"] = self._np(self._loads).reshape(-1, 6)
+        return self._cache["loads"]
+
+    @property
+    def hydro_forces(self):
+        if "hydro_forces" not in self._cache:
+            self._cache["hydro_forces"] = self._loads_np()[:, 0:3].copy()
+        return self._cache["hydro_forces"]
+
+    @hydro_forces.setter
+    def hydro_forces(self, value):
+        self._cache["hydro_forces"] = np.asarray(value, dtype=float)
+
+    @property
+    def hydro_torques(self):
+        if "hydro_torques" not in self._cache:
+            self._cache["hydro_torques"] = self._loads_np()[:, 3:6].copy()
+        return self._cache["hydro_torques"]
+
+    @hydro_torques.setter
+    def hydro_torques(self, value):
+        self._cache["hydro_torques"] = np.asarray(value, dtype=float)
+
+    @property
+    def momentum_residual(self) -> float:
+        """|sum_p F_p + sum_x imb| (psc3d.py:280-281)."""
+        if "residual" not in self._cache:
+            loads = self._loads_np()
+            total = loads[:, 0:3].sum(axis=0) if len(loads) else np.zeros(3)
+            fluid = self._geometry_sums()[8:11]
+            self._cache["residual"] = float(np.linalg.norm(total + fluid))
+        return self._cache["residual"]
+
+    def solid_velocity(self):
+        """Per-cell solid velocity sum_p alpha_p (U + omega x r) / sum alpha
+        (psc3d.py:181-206 / 347-373), as three (n, n, n) arrays."""
+        if (not self.static_geometry) or (not self._geometry_built):
+            self.rebuild_geometry()
+        if not self._moving or self._us is None:
+            z = np.zeros(self._shape)
+            return z, z.copy(), z.copy()
+        asum = self._alpha_sum_np()
+        num = self._np(self._us).astype(np.float64).reshape((3,) + self._shape)
+        mask = asum > 1.0e-14
+        out = []
+        for a in range(3):
+            u = num[a].copy()
+            u[mask] /= asum[mask]
+            out.append(u)
+        return tuple(out)
+
+    def _diagnostics(self) -> dict:
+        sums = self.readout_sums()
+        loads = self._loads_np()
+        if len(loads):
+            mf = float(np.max(np.linalg.norm(loads[:, 0:3], axis=1)))
+            mt = float(np.max(np.linalg.norm(loads[:, 3:6], axis=1)))
+        else:
+            mf = mt = 0.0
+        total = loads[:, 0:3].sum(axis=0) if len(loads) else np.zeros(3)
+        residual = float(np.linalg.norm(total + sums["imb"]))
+        self._cache["residual"] = residual
+        return {
+            "solid_volume_lu": float(sums["solid"]),
+            "mean_uz": float(sums["u"][2] / self._N),
+            "momentum_residual": residual,
+            "max_force": mf,
+            "max_torque": mt,
+        }
+
+    # --------------------------------------------------------------- physics
+    def permeability(self, axis=None) -> float:
+        """k = nu sum((1 - solid) u_axis) / N / F_axis, nu = (tau - 1/2)/3
+        (run_psc_permeability_resolution.py:144-150; axis defaults to the body
+        force direction)."""
+        if axis is None:
+            axis = int(np.argmax(np.abs(self._force)))
+        F = self._force[axis]
+        sums = self.readout_sums()
+        nu = (self.tau - 0.5) / 3.0
+        return float(nu * (sums["fluid_u"][axis] / self._N) / max(F, 1.0e-300))
+
+    def fluid_statistics(self, axis: int = 2):
+        """(intrinsic, superficial, porosity) of u_axis
+        (run_precision_validation.py:142-147)."""
+        sums = self.readout_sums()
+        vol = float(sums["fluid"])
+        intrinsic = float(sums["fluid_u"][axis] / max(vol, 1.0e-30))
+        superficial = float(sums["fluid_u"][axis] / self._N)
+        return intrinsic, superficial, vol / float(self._N)
+
+    def state_dict(self) -> dict:
+        """Distributions and step count for checkpoint/resume."""
+        return {"f": self.f.copy(), "step": self._step_count, "precision": self.precision}
+
+    def load_state_dict(self, state: dict) -> None:
+        self.f = state["f"]
+        self._step_count = int(state.get("step", 0))
+
+    # dense-layout views (psc3d.py:141-142, 176-178); P * n^3 arrays
+    @property
+    def alpha_p(self):
+        if (not self._geometry_built):
+            self.rebuild_geometry()
+        ids, vals = self._sparse_lists()
+        out = np.zeros((len(ids),) + self._shape)
+        for p, (i, v) in enumerate(zip(ids, vals)):
+            out[p].ravel()[i] = v
+        return out
+
+    @property
+    def owner_weights(self):
+        ap = self.alpha_p
+        asum = self._alpha_sum_np()
+        out = np.zeros_like(ap)
+        mask = asum > 1.0e-14
+        out[:, mask] = ap[:, mask] / asum[mask]
+        return out
+
+    def _sparse_lists(self):
+        if "lists" not in self._cache:
+            tabs = self._tables
+            counts = self._np(self._counts)[: max(tabs.n_entries, 0)].view(np.uint16).astype(np.int64)
+            nx, ny, nz = self._shape
+            ids, vals = [], []
+            for p in range(tabs.particles.shape[0]):
+                x0, y0, z0, ex, ey, ez, off = (int(v) for v in tabs.boxes[p, :7])
+                c = counts[off: off + ex * ey * ez].reshape(ex, ey, ez)
+                sel = np.nonzero(c)
+                gx = (sel[0] + x0) % nx
+                gy = (sel[1] + y0) % ny
+                gz = (sel[2] + z0) % nz
+                ids.append(((gx * ny + gy) * nz + gz).astype(np.int64))
+                vals.append(c[sel].astype(np.float64) / float(self._s ** 3))
+            self._cache["lists"] = (ids, vals)
+        return self._cache["lists"]
+
+
+class SparsePscD3Q19(PscD3Q19):
+    """Sparse-geometry variant (psc3d.py:284-407): same step, per-particle
+    cell lists.  On the GPU both classes share the bounding-box count
+    layout; ``geometry_workers`` is accepted for compatibility and unused."""
+
+    _LAYOUT_NAME = "sparse"
+
+    def __init__(self, n, particles, tau: float = 0.72, subsamples: int = 3, body_force_z: float = 0.0,
+                 static_geometry: bool = False, geometry_workers: int = 1, **kw):
+        super().__init__(n, particles, tau=tau, subsamples=subsamples, body_force_z=body_force_z,
+                         static_geometry=static_geometry, **kw)
+        self.geometry_workers = max(1, int(geometry_workers))
+
+    @property
+    def node_ids(self):
+        if not self._geometry_built:
+            return []
+        return self._sparse_lists()[0]
+
+    @property
+    def alpha_vals(self):
+        if not self._geometry_built:
+            return []
+        return self._sparse_lists()[1]
+
+    @property
+    def alpha_sum_flat(self):
+        return self._alpha_sum_np().ravel()
+
+
+def sphere_volume(r: float) -> float:
+    return 4.0 / 3.0 * math.pi * r ** 3

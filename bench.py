@@ -1,0 +1,260 @@
+#!/usr/bin/env python3
+"""Benchmark of the D3Q19 PSC-IMB step on one MI355X.
+
+Workload: the fixed-array permeability case -- periodic 128^3 lattice, 60
+non-overlapping spheres of radius 12 placed by seeded random sequential
+addition, tau 0.8, body force 1e-6 along x, 4x4x4 sub-samples, static
+geometry.  A "step" is one collide_stream_psc: the fused collide+stream
+kernel plus the per-particle force/torque reduction, exactly what the
+reference performs per call.
+
+The timed path is the FP32 deviation storage (g = f - w).  The same
+trajectory is run with the FP64 plain kernel (reference operation order) to
+state the permeability error of FP32 against FP64 after the same number of
+steps, and to time the FP64 step.  The CPU baseline is the NumPy FP64
+oracle (a restatement of the reference) on the same lattice, a bounded number
+of steps on one host core.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
+"""
+
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PKG = "particuology_hydraulic-penalty-_amd"
+METRIC = "D3Q19 PSC MLUPS + % HBM roofline at 128³; FP32 vs FP64 at matched k error"
+HBM_PEAK_GBPS = 8000.0
+BYTES_F32_STATIC = 19 * 4 * 2 + 4   # g read + g written + solid count
+BYTES_F64_STATIC = 19 * 8 * 2 + 4
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4900)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--kernel-repeats", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--layout", default="pull", choices=["pull", "push"])
+    ap.add_argument("--quiet", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg, quiet=False):
+    if not quiet:
+        print(msg, file=sys.stderr, flush=True)
+
+
+def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch):
+    sol = psc3d.PscD3Q19(n, table, tau=0.8, subsamples=4, body_force=(1.0e-6, 0.0, 0.0), static_geometry=True,
+                         precision=precision, stream_layout=layout, device="cuda:0")
+    sol.rebuild_geometry()
+    sol.run(warmup)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    sol.run(steps)
+    k = sol.permeability(axis=0)   # FP64 flux reduction of the final state (part of the job)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1)
+    return sol, k, wall, gpu_ms
+
+
+def step_kernel_ms(sol, repeats, torch, lib):
+    """Average duration of the collide+stream kernel alone, HIP events on the
+    stream it is launched on."""
+    nx, ny, nz = sol._shape
+    fx, fy, fz = sol.body_force
+    stream = torch.cuda.current_stream()
+    h = stream.cuda_stream
+
+    def launch(a, b):
+        lib.call("psc_step", a.data_ptr(), b.data_ptr(), sol._csum.data_ptr(), None, sol._imb.data_ptr(),
+                 sol._storage, sol._layout, nx, ny, nz, sol._s, sol.tau, fx, fy, fz, sol._block[0], sol._block[1], h)
+
+    a, b = sol._a, sol._b
+    for _ in range(20):
+        launch(a, b)
+        a, b = b, a
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(repeats):
+        launch(a, b)
+        a, b = b, a
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / repeats
+
+
+def copy_bandwidth(torch, lib, nbytes=2 << 30, repeats=20):
+    n = nbytes // 4
+    src = torch.empty(n, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    dst = torch.empty_like(src)
+    h = torch.cuda.current_stream().cuda_stream
+    for _ in range(3):
+        lib.call("psc_copy_f32", src.data_ptr(), dst.data_ptr(), n, h)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(repeats):
+        lib.call("psc_copy_f32", src.data_ptr(), dst.data_ptr(), n, h)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / repeats
+    del src, dst
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(table, n, steps, quiet):
+    """NumPy FP64 oracle (restatement of the reference) timed on one core
+    for ``steps`` static-geometry steps of an n^3 lattice."""
+    from oracle import psc_oracle as O
+
+    parts = [O.Sphere(*row) for row in table]
+    s = O.OracleSolver(n, parts, tau=0.8, subsamples=4, body_force=(1.0e-6, 0.0, 0.0), static_geometry=True,
+                       layout="sparse")
+    s.rebuild_geometry()
+    s.collide_stream_psc()   # first step outside the timing (allocations settle)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.collide_stream_psc()
+    dt = time.perf_counter() - t0
+    log(f"cpu oracle: {steps} steps of {n}^3 in {dt:.2f}s", quiet)
+    return n ** 3 * steps / dt / 1e6, dt
+
+
+def read_traffic():
+    """HBM bytes per launch of the FP32 step from the committed rocprofv3
+    counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "k_step_f32_128_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
+                                   "(--gpus must be 1)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    import numpy as np
+    import torch
+
+    psc3d = importlib.import_module(PKG + ".psc3d")
+    workloads = importlib.import_module(PKG + ".workloads")
+    lib = importlib.import_module(PKG + "._lib")
+    lib.load()
+    n = args.n
+    table = workloads.fixed_array(seed=0, n=n)
+    total = args.warmup + args.steps
+    N = n ** 3
+
+    log(f"fp32 run: {args.warmup} warmup + {args.steps} timed steps at {n}^3", args.quiet)
+    sol32, k32, wall32, gpu_ms32 = timed_run(psc3d, table, n, "fp32", args.layout, args.steps, args.warmup, torch)
+    porosity = 1.0 - float(sol32.solid.mean())
+    kern_ms = step_kernel_ms(sol32, args.kernel_repeats, torch, lib)
+    mlups = N * args.steps / wall32 / 1e6
+    log(f"fp32: {mlups:.0f} MLUPS (wall), step kernel {kern_ms*1e3:.1f} us, k={k32:.9e}", args.quiet)
+
+    fp64 = None
+    if not args.no_fp64:
+        sol64, k64, wall64, gpu_ms64 = timed_run(psc3d, table, n, "fp64", args.layout, args.steps, args.warmup,
+                                                 torch)
+        kern64 = step_kernel_ms(sol64, max(50, args.kernel_repeats // 4), torch, lib)
+        mlups64 = N * args.steps / wall64 / 1e6
+        fp64 = {"MLUPS": mlups64, "ms_per_step": wall64 * 1e3 / args.steps, "k": k64,
+                "step_kernel_us": kern64 * 1e3,
+                "step_kernel_GBps": BYTES_F64_STATIC * N / (kern64 * 1e-3) / 1e9}
+        log(f"fp64: {mlups64:.0f} MLUPS, step kernel {kern64*1e3:.1f} us, k={k64:.9e}", args.quiet)
+        del sol64
+    copy_gbps = copy_bandwidth(torch, lib)
+    log(f"copy kernel: {copy_gbps:.0f} GB/s", args.quiet)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        # bounded sample: same sphere size and volume fraction on a cpu_n^3 box
+        cn = args.cpu_n
+        count = max(1, int(round(table.shape[0] * (cn / n) ** 3)))
+        ctable = table if cn == n else workloads.fixed_array(seed=0, n=cn, count=count)
+        cpu_mlups, cpu_dt = cpu_baseline(ctable, cn, args.cpu_steps, args.quiet)
+        cpu = {"value": cpu_mlups, "unit": "MLUPS", "cores": 1, "kind": "port",
+               "sample": f"NumPy FP64 oracle (restatement of psc3d.py), {args.cpu_steps} static-geometry steps of a "
+                         f"{cn}^3 periodic box with {count} RSA spheres r=12 (same volume fraction) in {cpu_dt:.1f} s, "
+                         f"one thread; host {platform.processor() or platform.machine()}, "
+                         f"os.cpu_count()={os.cpu_count()}"}
+
+    achieved = BYTES_F32_STATIC * N / (kern_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": mlups,
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall32 * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"fixed-array permeability: periodic {n}^3, 60 RSA spheres r=12 (seed 0), tau 0.8, "
+                        "F=1e-6 along x, s=4, static geometry, FP32 deviation storage, "
+                        f"{args.layout} streaming; step = collide+stream + particle loads",
+            "lattice": [n, n, n],
+            "particles": int(table.shape[0]),
+            "porosity": porosity,
+            "total_steps": total,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": read_traffic(),
+            "kernel": "k_step_f32 (pull, static)",
+            "bytes_per_cell": BYTES_F32_STATIC,
+            "kernel_us": kern_ms * 1e3,
+        },
+        "cpu_baseline": cpu,
+        "k_fp32": k32,
+        "fp64": fp64,
+        "k_rel_err_fp32_vs_fp64": (abs(k32 - fp64["k"]) / abs(fp64["k"])) if fp64 else None,
+        "fp32_over_fp64_MLUPS": (mlups / fp64["MLUPS"]) if fp64 else None,
+        "copy_GBps": copy_gbps,
+        "gpu_ms_timed_region": gpu_ms32,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,62 @@
+"""Seeded synthetic geometries of the benchmark workloads.
+
+Every generator returns a (P, 10) float64 particle table (x, y, z, r, vx, vy,
+vz, wx, wy, wz; lattice units) and is deterministic for a given seed.
+
+* ``moving_sphere``   20^3, one r=2.5 sphere at a uniform random centre moving
+  at U=(1e-4, 0, 0) (the plain-method FP64 case; tau 0.8, s=2, 100 steps).
+* ``fixed_array``     128^3, 60 non-overlapping r=12 spheres by random
+  sequential addition (the fixed-array permeability case; tau 0.8,
+  F=1e-6 along x, s=4, 5000 steps).  60 such spheres fill 20.7 % of the box,
+  so the porosity is about 0.79 (the nominal "~0.45" of the case description
+  needs ~159 spheres, beyond the random-sequential-addition jamming limit).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def _table(centres, radii, vel=None, omega=None):
+    P = len(radii)
+    t = np.zeros((P, 10), dtype=np.float64)
+    t[:, 0:3] = centres
+    t[:, 3] = radii
+    if vel is not None:
+        t[:, 4:7] = vel
+    if omega is not None:
+        t[:, 7:10] = omega
+    return t
+
+
+def moving_sphere(seed: int = 0, n: int = 20, radius: float = 2.5, u=(1.0e-4, 0.0, 0.0)):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, float(n), size=3)
+    return _table(c[None, :], [radius], vel=np.asarray(u, dtype=np.float64)[None, :])
+
+
+def rsa_spheres(n: int, count: int, radius: float, seed: int = 0, margin: float = 1.0, gap: float = 0.0,
+                max_tries: int = 2_000_000):
+    """Random sequential addition of ``count`` equal spheres inside the box,
+    centres in [r + margin, n - r - margin]^3 (so clamped and periodic
+    voxelization agree), pairwise distance >= 2r + gap."""
+    rng = np.random.default_rng(seed)
+    lo, hi = radius + margin, n - radius - margin
+    if hi <= lo:
+        raise ValueError("box too small for the radius")
+    pts = np.zeros((0, 3))
+    tries = 0
+    dmin2 = (2.0 * radius + gap) ** 2
+    while len(pts) < count:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError(f"random sequential addition jammed at {len(pts)} of {count} spheres")
+        c = rng.uniform(lo, hi, size=3)
+        if len(pts) == 0 or np.min(np.sum((pts - c) ** 2, axis=1)) >= dmin2:
+            pts = np.vstack([pts, c])
+    return pts
+
+
+def fixed_array(seed: int = 0, n: int = 128, count: int = 60, radius: float = 12.0):
+    pts = rsa_spheres(n, count, radius, seed=seed)
+    return _table(pts, np.full(count, radius))

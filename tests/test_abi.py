@@ -1,0 +1,83 @@
+"""The C ABI library loads on the host and exports every declared entry
+point; argument checking works without touching a GPU (CPU)."""
+
+import ctypes as ct
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT, load_package
+
+HEADER = os.path.join(ROOT, "include", "psc_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*(psc_[a-z_0-9]+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return load_package("_lib")
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 12
+    handle = lib.load()
+    for name in names:
+        assert hasattr(handle, name), name
+    assert set(names) == set(lib.SIGNATURES), "ctypes signature table out of sync with the header"
+
+
+def test_lattice_constants_match_reference_order(lib):
+    from oracle import psc_oracle as O
+    c = np.zeros(57, dtype=np.int32)
+    w = np.zeros(19)
+    opp = np.zeros(19, dtype=np.int32)
+    lib.call("psc_lattice_constants", c.ctypes.data, w.ctypes.data, opp.ctypes.data)
+    assert np.array_equal(c.reshape(19, 3), O.C)
+    assert np.array_equal(w, O.W)
+    assert np.array_equal(opp, O.OPP)
+    psc3d = load_package("psc3d")
+    assert np.array_equal(psc3d.C, O.C) and np.array_equal(psc3d.W, O.W) and np.array_equal(psc3d.OPP, O.OPP)
+    assert psc3d.CS2 == O.CS2
+
+
+def test_argument_errors_are_reported(lib):
+    h = lib.load()
+    rc = h.psc_step(None, None, None, None, None, 1, 1, 8, 8, 8, 2, 0.8, 0.0, 0.0, 0.0, 0, 0, None)
+    assert rc == -1
+    assert b"null" in h.psc_last_error()
+    rc = h.psc_step(ct.c_void_p(16), ct.c_void_p(16), ct.c_void_p(64), None, None, 1, 1, 8, 8, 8, 2, 0.8, 0, 0, 0,
+                    0, 0, None)
+    assert rc == -1 and b"alias" in h.psc_last_error()
+    rc = h.psc_step(ct.c_void_p(16), ct.c_void_p(32), ct.c_void_p(64), None, None, 1, 1, 8, 8, 8, 2, 0.4, 0, 0, 0,
+                    0, 0, None)
+    assert rc == -1 and b"tau" in h.psc_last_error()
+    rc = h.psc_step(ct.c_void_p(16), ct.c_void_p(32), ct.c_void_p(64), None, None, 1, 1, 8, 8, 8, 2, 0.8, 0, 0, 0,
+                    48, 3, None)
+    assert rc == -2
+    rc = h.psc_voxelize(None, None, None, 0, 2048, 8, 8, 8, 41, None, None, None, 0, None)
+    assert rc == -1 and b"subsamples" in h.psc_last_error()
+    rc = h.psc_moments(None, 3, 0, None, None, 8, 8, 8, 2, 0, 0, 0, None, None, None, None)
+    assert rc == -1
+    rc = h.psc_clear(None, 0, None)
+    assert rc == 0
+    with pytest.raises(lib.PscError):
+        lib.call("psc_clear", None, 16, None)
+    assert h.psc_moments_blocks(128 ** 3) >= 1
+    assert h.psc_version() >= 10000
+
+
+def test_solver_refuses_to_run_without_gpu():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    psc3d = load_package("psc3d")
+    lib = load_package("_lib")
+    with pytest.raises(lib.PscError):
+        psc3d.PscD3Q19(8, [psc3d.Particle3D(4, 4, 4, 2)])

@@ -1,0 +1,241 @@
+"""Closed forms and invariants that pin the oracle independently of the
+reference (CPU).  Each check is chosen so that a plausible slip (dropped
+half-force, wrong opposite index, transposed axis, off-by-one sample offset)
+fails it."""
+
+from fractions import Fraction
+import math
+
+import numpy as np
+import pytest
+
+from oracle import psc_oracle as O
+
+
+# ---------------------------------------------------------------- a1 lattice
+def test_lattice_moments():
+    c = O.C.astype(float)
+    assert O.W.sum() == pytest.approx(1.0, abs=1e-15)
+    np.testing.assert_allclose(O.W @ c, 0.0, atol=1e-16)
+    np.testing.assert_allclose(np.einsum("i,ia,ib->ab", O.W, c, c), O.CS2 * np.eye(3), atol=1e-16)
+    # fourth moment isotropy of D3Q19: sum w c_a^2 c_b^2 = cs^4 (1 + 2 delta_ab)
+    m4 = np.einsum("i,ia,ib->ab", O.W, c * c, c * c)
+    np.testing.assert_allclose(m4, O.CS2 ** 2 * (np.ones((3, 3)) + 2 * np.eye(3)), atol=1e-15)
+    assert np.array_equal(O.OPP[O.OPP], np.arange(19))
+    assert np.array_equal(O.C[O.OPP], -O.C)
+    assert len({tuple(v) for v in O.C}) == 19
+
+
+# ------------------------------------------------------- a6/a7 eq and forcing
+def test_equilibrium_moments():
+    rng = np.random.default_rng(1)
+    rho = 1.0 + 0.01 * rng.standard_normal(50)
+    u = 0.02 * rng.standard_normal((3, 50))
+    feq = O.equilibrium(rho, *u)
+    c = O.C.astype(float)
+    np.testing.assert_allclose(feq.sum(0), rho, rtol=1e-14)
+    np.testing.assert_allclose(np.einsum("ia,in->an", c, feq), rho * u, rtol=1e-12, atol=1e-17)
+    pi = np.einsum("ia,ib,in->abn", c, c, feq)
+    expect = rho * (O.CS2 * np.eye(3)[:, :, None] + u[:, None, :] * u[None, :, :])
+    np.testing.assert_allclose(pi, expect, rtol=1e-12, atol=1e-16)
+
+
+def test_guo_moments():
+    rng = np.random.default_rng(2)
+    u = 0.02 * rng.standard_normal((3, 40))
+    F = (3e-6, -1e-6, 2e-6)
+    tau = 0.8
+    g = O.guo_force(*F, *u, tau)
+    c = O.C.astype(float)
+    np.testing.assert_allclose(g.sum(0), 0.0, atol=1e-20)
+    mom = np.einsum("ia,in->an", c, g)
+    np.testing.assert_allclose(mom, (1 - 0.5 / tau) * np.array(F)[:, None] * np.ones((1, 40)), rtol=1e-12)
+
+
+# ------------------------------------------------------------------ a8 step
+def test_rest_state_is_fixed_point():
+    s = O.OracleSolver(8, [], tau=0.72, subsamples=2, static_geometry=True)
+    f0 = s.f.copy()
+    for _ in range(3):
+        s.collide_stream_psc()
+    np.testing.assert_allclose(s.f, f0, rtol=0, atol=1e-16)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_uniform_body_force_momentum_double_half_shift(axis):
+    """No solids: every step adds F (1 + 1/(2 tau)) of momentum per cell --
+    the reference's two half-force shifts (u carries F/2, u_eff another F/2)
+    give F/tau through the BGK relaxation plus (1 - 1/(2 tau)) F from Guo."""
+    n, tau, t = 6, 0.8, 7
+    F = [0.0, 0.0, 0.0]
+    F[axis] = 1e-5
+    s = O.OracleSolver(n, [], tau=tau, subsamples=2, body_force=F, static_geometry=True)
+    for _ in range(t):
+        s.collide_stream_psc()
+    j = O.axis_moment(s.f, axis).sum()
+    assert j == pytest.approx(t * n ** 3 * F[axis] * (1 + 0.5 / tau), rel=1e-10)
+    # the stored velocity is (j + F/2)/rho: uniform field
+    u = [s.ux, s.uy, s.uz][axis]
+    rho = s.rho
+    np.testing.assert_allclose(u * rho, t * F[axis] * (1 + 0.5 / tau) + 0.5 * F[axis], rtol=1e-9)
+
+
+def test_streaming_is_a_permutation():
+    rng = np.random.default_rng(3)
+    f = rng.random((19, 5, 6, 7))
+    g = O.push_stream(f)
+    for i in range(19):
+        assert np.array_equal(np.sort(g[i].ravel()), np.sort(f[i].ravel()))
+        cx, cy, cz = O.C[i]
+        assert g[i][(1 + cx) % 5, (2 + cy) % 6, (3 + cz) % 7] == f[i][1, 2, 3]
+
+
+def test_fully_solid_cell_collision():
+    """alpha = 1: B = 1/(1+1e-14); the post-collision state is
+    f + (1-B) Omega_bgk + B (f_opp - f + feq_s - feq_f,opp) + Guo."""
+    n, tau = 8, 0.72
+    s = O.OracleSolver(n, [O.Sphere(4.0, 4.0, 4.0, 3.0)], tau=tau, subsamples=2, static_geometry=True,
+                       layout="dense")
+    rng = np.random.default_rng(4)
+    s.f = s.f + 1e-4 * rng.standard_normal(s.f.shape)
+    s.rebuild_geometry()
+    assert s.solid[4, 4, 4] == 1.0
+    f = s.f[:, 4, 4, 4].copy()
+    rho = f.sum()
+    u = O.C.T.astype(float) @ f / rho
+    feq_f = O.equilibrium(np.array(rho), *u)
+    feq_s = O.equilibrium(np.array(rho), 0.0, 0.0, 0.0)
+    B = 1.0 / (1.0 + 1e-14)
+    post = f + (1 - B) * (-(f - feq_f) / tau) + B * (f[O.OPP] - f + feq_s - feq_f[O.OPP])
+    s.collide_stream_psc()
+    # read the post-collision value back from the streamed destination
+    got = np.array([s.f[i][(4 + O.C[i, 0]) % n, (4 + O.C[i, 1]) % n, (4 + O.C[i, 2]) % n] for i in range(19)])
+    np.testing.assert_allclose(got, post, rtol=1e-13, atol=1e-19)
+
+
+def test_mirror_symmetry_of_moving_sphere():
+    """A centred sphere translating along x: no transverse force, no torque."""
+    n = 12
+    s = O.OracleSolver(n, [O.Sphere(6.0, 6.0, 6.0, 2.7, vx=1e-4)], tau=0.8, subsamples=3, layout="dense")
+    for _ in range(10):
+        s.collide_stream_psc()
+    F = s.hydro_forces[0]
+    T = s.hydro_torques[0]
+    assert F[0] < 0  # resists the motion
+    assert abs(F[1]) < 1e-12 * abs(F[0]) and abs(F[2]) < 1e-12 * abs(F[0])
+    assert np.max(np.abs(T)) < 1e-10 * abs(F[0])
+    assert s.momentum_residual < 1e-12 * abs(F[0])
+
+
+def test_axis_permutation_equivariance():
+    """Force and motion along x on a geometry equal the z case with x and z
+    swapped (pins the vector body-force generalisation)."""
+    n = 10
+    pz = O.Sphere(4.3, 5.1, 5.6, 2.2, vz=2e-4, wx=1e-5)
+    px = O.Sphere(5.6, 5.1, 4.3, 2.2, vx=2e-4, wz=-1e-5)  # omega is a pseudo-vector
+    sz = O.OracleSolver(n, [pz], tau=0.9, subsamples=2, body_force=(0, 0, 3e-6), layout="sparse")
+    sx = O.OracleSolver(n, [px], tau=0.9, subsamples=2, body_force=(3e-6, 0, 0), layout="sparse")
+    for _ in range(6):
+        sz.collide_stream_psc()
+        sx.collide_stream_psc()
+    np.testing.assert_allclose(sx.hydro_forces[0][::-1], sz.hydro_forces[0], rtol=1e-9, atol=1e-20)
+    # torque is a pseudo-vector: swapping x and z flips its sign
+    np.testing.assert_allclose(-sx.hydro_torques[0][::-1], sz.hydro_torques[0], rtol=1e-8, atol=1e-20)
+    np.testing.assert_allclose(np.transpose(sx.ux, (2, 1, 0)), sz.uz, rtol=1e-9, atol=1e-20)
+
+
+# ------------------------------------------------------------ a3 voxelization
+def _exact_count(ix, iy, iz, p, s):
+    """Exact rational count (ties resolved exactly)."""
+    r2 = Fraction(p.r) ** 2
+    cnt = 0
+    for a in range(s):
+        for b in range(s):
+            for c in range(s):
+                dx = ix + Fraction(2 * a + 1, 2 * s) - Fraction(p.x)
+                dy = iy + Fraction(2 * b + 1, 2 * s) - Fraction(p.y)
+                dz = iz + Fraction(2 * c + 1, 2 * s) - Fraction(p.z)
+                cnt += (dx * dx + dy * dy + dz * dz) <= r2
+    return cnt
+
+
+def test_voxel_counts_match_exact_arithmetic_away_from_ties():
+    rng = np.random.default_rng(6)
+    for trial in range(6):
+        p = O.Sphere(*rng.uniform(3.0, 7.0, 3), rng.uniform(0.8, 2.6))
+        s = int(rng.integers(1, 5))
+        ix, iy, iz, counts = O.sphere_counts(p, 11, s)
+        for a, x in enumerate(ix):
+            for b, y in enumerate(iy):
+                for c, z in enumerate(iz):
+                    assert counts[a, b, c] == _exact_count(int(x), int(y), int(z), p, s)
+
+
+def test_voxel_volume_converges_and_interior_is_full():
+    p = O.Sphere(10.37, 9.81, 10.12, 5.3)
+    vols = []
+    for s in (2, 4, 8):
+        ids, vals, _ = O.sparse_alpha(p, 21, s)
+        vols.append(vals.sum())
+    exact = 4 / 3 * math.pi * p.r ** 3
+    errs = [abs(v - exact) / exact for v in vols]
+    assert errs[-1] < 0.01 and errs[-1] < errs[0]
+    ix, iy, iz, counts = O.sphere_counts(p, 21, 3)
+    X, Y, Z = np.meshgrid(ix, iy, iz, indexing="ij")
+    far = np.sqrt((np.maximum(np.abs(X - p.x), np.abs(X + 1 - p.x))) ** 2
+                  + (np.maximum(np.abs(Y - p.y), np.abs(Y + 1 - p.y))) ** 2
+                  + (np.maximum(np.abs(Z - p.z), np.abs(Z + 1 - p.z))) ** 2)
+    assert np.all(counts[far < p.r] == 27)
+
+
+def test_clamped_box_truncates_spheres():
+    p = O.Sphere(0.4, 5.0, 5.0, 2.0)
+    lo = O.sphere_box(p, 10)
+    assert lo[0] == 0 and lo[1] == 4
+    ids, vals, _ = O.sparse_alpha(p, 10, 2)
+    # periodic boxes wrap the same sphere onto x = 7..9
+    ids_p, vals_p, pos = O.sparse_alpha(p, 10, 2, periodic=True)
+    assert vals_p.sum() > vals.sum()
+    assert np.any(pos[0] < 0)
+
+
+# ---------------------------------------------------- a5 solid velocity, a10
+def test_solid_velocity_rigid_body():
+    p = O.Sphere(6.2, 5.7, 6.4, 2.9, vx=1e-4, vy=-2e-4, vz=3e-4, wx=1e-5, wy=2e-5, wz=-3e-5)
+    s = O.OracleSolver(12, [p], subsamples=2, layout="sparse")
+    s.rebuild_geometry()
+    us = s.solid_velocity()
+    X, Y, Z = np.indices((12, 12, 12)) + 0.5
+    r = np.stack([X - p.x, Y - p.y, Z - p.z])
+    w = np.array([p.wx, p.wy, p.wz])[:, None, None, None]
+    v = np.array([p.vx, p.vy, p.vz])[:, None, None, None] + np.cross(w, r, axis=0)
+    mask = s.solid > 0
+    for a in range(3):
+        np.testing.assert_allclose(us[a][mask], v[a][mask], rtol=1e-12, atol=1e-18)
+        assert np.all(us[a][~mask] == 0.0)
+
+
+def test_load_closure_and_owner_split():
+    parts = [O.Sphere(5.0, 6.0, 6.0, 2.5, vx=1e-4), O.Sphere(8.5, 6.2, 5.9, 2.5, vy=-1e-4)]
+    s = O.OracleSolver(14, parts, tau=0.8, subsamples=3, body_force=(0, 0, 1e-6), layout="sparse")
+    for _ in range(4):
+        s.collide_stream_psc()
+    total = s.hydro_forces.sum(axis=0)
+    fluid = np.array([s.imb_mx.sum(), s.imb_my.sum(), s.imb_mz.sum()])
+    np.testing.assert_allclose(total, -fluid, rtol=1e-12, atol=1e-22)
+    assert s.momentum_residual < 1e-13 * np.max(np.abs(total))
+
+
+# ---------------------------------------------------------------- a9 readouts
+def test_published_permeability_reproduced_from_its_csv_columns():
+    """convergence/permeability/baseline_n128.csv of the reference: k_lu from
+    superficial_uz, tau 0.72 and Fz 2e-7 (values copied as data)."""
+    import json
+    import os
+    row = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "published_k.json")))
+    nu = (row["tau"] - 0.5) / 3.0
+    assert nu * row["superficial_uz"] / row["force_z"] == pytest.approx(row["permeability_lu"], rel=1e-15)
+
+
+def test_hasimoto_small_lambda():
+    assert O.hasimoto_factor(1e-6, 1.0) == pytest.approx(1.0 + 2.837297e-6, rel=1e-9)

@@ -148,6 +148,42 @@ int psc_export_f64(const void* d_state, int32_t storage, int32_t layout, int32_t
 int psc_import_f64(const double* d_f, int32_t storage, int32_t layout, int32_t nx, int32_t ny,
                    int32_t nz, void* d_state, void* stream);
 
+/* Everything one static-geometry step needs: the K2 arguments of psc_step
+ * and, when nparticles > 0, the K3 arguments of psc_particle_loads.  Plain
+ * pointers and sizes; the state is ping-ponged between state_a and state_b. */
+typedef struct PscStepPlan {
+  void* state_a;
+  void* state_b;
+  const uint32_t* count_sum;
+  const void* us_num;
+  void* imb;
+  int32_t storage, layout, nx, ny, nz, subsamples;
+  double tau, fx, fy, fz;
+  int32_t block_z, block_y;
+  const double* particles;
+  const int32_t* boxes;
+  const int32_t* chunks;
+  const int32_t* chunk_first;
+  int32_t nparticles, nchunks, chunk_cells;
+  const uint16_t* counts;
+  double* partials;
+  double* loads;
+} PscStepPlan;
+
+/* nsteps repetitions of collide_stream_psc with static geometry
+ * (psc3d.py:208-253 in a loop, as the drivers run it, e.g.
+ * run_psc_permeability_resolution.py:137-138): K2 from the current buffer
+ * into the other one, then K3.  The step starts from state_a when *parity is
+ * 0 (state_b when 1); on return *parity names the buffer holding the state.
+ * use_graph = 1 captures `graph_steps` steps (even, >= 2) once per plan into
+ * a hipGraph and replays it; the remainder runs as direct launches.  Graphs
+ * are cached per plan contents until psc_graph_cache_clear(). */
+int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t use_graph, int32_t graph_steps,
+                  int32_t* parity, void* stream);
+
+/* Destroy every cached hipGraph of psc_run_steps. */
+int psc_graph_cache_clear(void);
+
 /* Plain device copy of n floats, the bandwidth reference for the roofline
  * and for scaling HBM byte counters (no counterpart in the reference). */
 int psc_copy_f32(const float* d_src, float* d_dst, int64_t n, void* stream);

@@ -39,8 +39,24 @@ SIGNATURES = {
     "psc_moments_blocks": ([_I64], ct.c_int64),
     "psc_export_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_import_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
+    "psc_run_steps": ([_P, _I32, _I32, _I32, _P, _P], ct.c_int),
+    "psc_graph_cache_clear": ([], ct.c_int),
     "psc_copy_f32": ([_P, _P, _I64, _P], ct.c_int),
 }
+
+
+class PscStepPlan(ct.Structure):
+    """Mirror of struct PscStepPlan (include/psc_hip.h)."""
+
+    _fields_ = [
+        ("state_a", _P), ("state_b", _P), ("count_sum", _P), ("us_num", _P), ("imb", _P),
+        ("storage", _I32), ("layout", _I32), ("nx", _I32), ("ny", _I32), ("nz", _I32), ("subsamples", _I32),
+        ("tau", _D), ("fx", _D), ("fy", _D), ("fz", _D),
+        ("block_z", _I32), ("block_y", _I32),
+        ("particles", _P), ("boxes", _P), ("chunks", _P), ("chunk_first", _P),
+        ("nparticles", _I32), ("nchunks", _I32), ("chunk_cells", _I32),
+        ("counts", _P), ("partials", _P), ("loads", _P),
+    ]
 
 
 class PscError(RuntimeError):

@@ -5,7 +5,10 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <cstring>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "psc_common.h"
 #include "../../include/psc_hip.h"
@@ -239,3 +242,164 @@ int psc_copy_f32(const float* d_src, float* d_dst, int64_t n, void* stream) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------
+// psc_run_steps: native step loop with hipGraph replay
+// ------------------------------------------------------------------------
+namespace psc {
+namespace {
+
+struct GraphEntry {
+  PscStepPlan plan;
+  int steps;
+  int start_parity;
+  hipGraphExec_t exec;
+};
+std::mutex g_graph_mu;
+std::vector<GraphEntry> g_graphs;
+constexpr size_t kMaxGraphs = 64;
+
+// Copy of a plan with every padding byte zero, so plans compare with memcmp.
+PscStepPlan normalized(const PscStepPlan& p) {
+  PscStepPlan k;
+  std::memset(&k, 0, sizeof(k));
+  k.state_a = p.state_a;
+  k.state_b = p.state_b;
+  k.count_sum = p.count_sum;
+  k.us_num = p.us_num;
+  k.imb = p.imb;
+  k.storage = p.storage;
+  k.layout = p.layout;
+  k.nx = p.nx;
+  k.ny = p.ny;
+  k.nz = p.nz;
+  k.subsamples = p.subsamples;
+  k.tau = p.tau;
+  k.fx = p.fx;
+  k.fy = p.fy;
+  k.fz = p.fz;
+  k.block_z = p.block_z;
+  k.block_y = p.block_y;
+  k.particles = p.particles;
+  k.boxes = p.boxes;
+  k.chunks = p.chunks;
+  k.chunk_first = p.chunk_first;
+  k.nparticles = p.nparticles;
+  k.nchunks = p.nchunks;
+  k.chunk_cells = p.chunk_cells;
+  k.counts = p.counts;
+  k.partials = p.partials;
+  k.loads = p.loads;
+  return k;
+}
+
+int one_step(const PscStepPlan& p, int parity, hipStream_t s) {
+  void* in = parity ? p.state_b : p.state_a;
+  void* out = parity ? p.state_a : p.state_b;
+  int rc = psc_step(in, out, p.count_sum, p.us_num, p.imb, p.storage, p.layout, p.nx, p.ny, p.nz, p.subsamples,
+                    p.tau, p.fx, p.fy, p.fz, p.block_z, p.block_y, s);
+  if (rc) return rc;
+  if (p.nparticles > 0) {
+    rc = psc_particle_loads(p.particles, p.boxes, p.chunks, p.chunk_first, p.nparticles, p.nchunks, p.chunk_cells,
+                            p.counts, p.count_sum, p.imb, p.storage == PSC_F32_DEV ? 1 : 2, p.nx, p.ny, p.nz,
+                            p.subsamples, p.partials, p.loads, s);
+  }
+  return rc;
+}
+
+// Library-owned stream per device for graph capture: the caller's stream may
+// be the legacy default stream, which cannot be captured.  Captured work does
+// not execute, so capturing on a side stream orders nothing; the graph is
+// launched on the caller's stream.
+std::vector<hipStream_t> g_capture_streams;
+
+int capture_stream(hipStream_t* out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error((int)e, "psc_run_steps: hipGetDevice: %s", hipGetErrorString(e));
+  if ((int)g_capture_streams.size() <= dev) g_capture_streams.resize(dev + 1, nullptr);
+  if (g_capture_streams[dev] == nullptr) {
+    e = hipStreamCreateWithFlags(&g_capture_streams[dev], hipStreamNonBlocking);
+    if (e != hipSuccess) return set_error((int)e, "psc_run_steps: capture stream: %s", hipGetErrorString(e));
+  }
+  *out = g_capture_streams[dev];
+  return PSC_OK;
+}
+
+int graph_for(const PscStepPlan& p, int steps, int parity, hipGraphExec_t* out) {
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  for (const GraphEntry& e : g_graphs) {
+    if (e.steps == steps && e.start_parity == parity && std::memcmp(&e.plan, &p, sizeof(p)) == 0) {
+      *out = e.exec;
+      return PSC_OK;
+    }
+  }
+  hipStream_t s = nullptr;
+  const int rcs = capture_stream(&s);
+  if (rcs) return rcs;
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return set_error((int)e, "psc_run_steps: begin capture: %s", hipGetErrorString(e));
+  int rc = PSC_OK;
+  int par = parity;
+  for (int k = 0; k < steps && rc == PSC_OK; ++k) {
+    rc = one_step(p, par, s);
+    par ^= 1;
+  }
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(s, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return set_error((int)e, "psc_run_steps: end capture: %s", hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return set_error((int)e, "psc_run_steps: instantiate: %s", hipGetErrorString(e));
+  if (g_graphs.size() >= kMaxGraphs) {
+    (void)hipGraphExecDestroy(g_graphs.front().exec);
+    g_graphs.erase(g_graphs.begin());
+  }
+  g_graphs.push_back(GraphEntry{p, steps, parity, exec});
+  *out = exec;
+  return PSC_OK;
+}
+
+}  // namespace
+}  // namespace psc
+
+extern "C" int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t use_graph, int32_t graph_steps,
+                             int32_t* parity, void* stream) {
+  if (plan == nullptr || parity == nullptr || nsteps < 0) return set_error(PSC_EINVAL, "psc_run_steps: arguments");
+  if (*parity != 0 && *parity != 1) return set_error(PSC_EINVAL, "psc_run_steps: parity %d", *parity);
+  if (use_graph && (graph_steps < 2 || (graph_steps & 1)))
+    return set_error(PSC_EINVAL, "psc_run_steps: graph_steps %d must be even and >= 2", graph_steps);
+  hipStream_t s = (hipStream_t)stream;
+  const PscStepPlan key = normalized(*plan);
+  int par = *parity;
+  int done = 0;
+  if (use_graph && nsteps >= graph_steps) {
+    hipGraphExec_t exec = nullptr;
+    int rc = graph_for(key, graph_steps, par, &exec);
+    if (rc) return rc;
+    while (nsteps - done >= graph_steps) {
+      const hipError_t e = hipGraphLaunch(exec, s);
+      if (e != hipSuccess) return set_error((int)e, "psc_run_steps: graph launch: %s", hipGetErrorString(e));
+      done += graph_steps;  // even: parity unchanged
+    }
+  }
+  for (; done < nsteps; ++done) {
+    const int rc = one_step(key, par, s);
+    if (rc) return rc;
+    par ^= 1;
+  }
+  *parity = par;
+  return PSC_OK;
+}
+
+extern "C" int psc_graph_cache_clear(void) {
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  for (GraphEntry& e : g_graphs) (void)hipGraphExecDestroy(e.exec);
+  g_graphs.clear();
+  return PSC_OK;
+}

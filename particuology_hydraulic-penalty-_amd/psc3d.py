@@ -293,10 +293,51 @@ class PscD3Q19:
             d._fill()
         return d
 
-    def run(self, steps: int) -> None:
-        """``steps`` calls of collide_stream_psc without building diag dicts."""
-        for _ in range(int(steps)):
-            self._advance()
+    def run(self, steps: int, graph: bool = True, graph_steps: int = 16) -> None:
+        """``steps`` calls of collide_stream_psc without building diag dicts.
+
+        With static geometry the loop runs natively (psc_run_steps): direct
+        launches, or a captured hipGraph of ``graph_steps`` steps replayed.
+        With dynamic geometry every step re-voxelizes from Python objects."""
+        steps = int(steps)
+        if steps <= 0:
+            return
+        if (not self.static_geometry) or (not self._geometry_built):
+            for _ in range(steps):
+                self._advance()
+            return
+        import ctypes as ct
+        plan = self._plan()
+        parity = ct.c_int32(0)
+        _lib.call("psc_run_steps", ct.byref(plan), steps, 1 if graph else 0, int(graph_steps), ct.byref(parity),
+                  self._stream())
+        if parity.value:
+            self._a, self._b = self._b, self._a
+        self._step_count += steps
+        self._cache.clear()
+
+    def _plan(self):
+        """PscStepPlan of the current buffers (state in self._a)."""
+        p = _lib.PscStepPlan()
+        nx, ny, nz = self._shape
+        p.state_a, p.state_b = self._ptr(self._a), self._ptr(self._b)
+        p.count_sum = self._ptr(self._csum)
+        p.us_num = self._ptr(self._us) if self._moving else None
+        p.imb = self._ptr(self._imb)
+        p.storage, p.layout = self._storage, self._layout
+        p.nx, p.ny, p.nz, p.subsamples = nx, ny, nz, self._s
+        p.tau = self.tau
+        p.fx, p.fy, p.fz = self._force
+        p.block_z, p.block_y = self._block
+        tabs = self._tables
+        if tabs is not None and tabs.particles.shape[0] > 0:
+            parts, boxes, chunks, cfirst = self._dev_tables
+            p.particles, p.boxes, p.chunks, p.chunk_first = (self._ptr(parts), self._ptr(boxes), self._ptr(chunks),
+                                                             self._ptr(cfirst))
+            p.nparticles, p.nchunks, p.chunk_cells = (int(tabs.particles.shape[0]), int(tabs.chunks.shape[0]),
+                                                      tabs.chunk_cells)
+            p.counts, p.partials, p.loads = self._ptr(self._counts), self._ptr(self._partials), self._ptr(self._loads)
+        return p
 
     def _advance(self):
         if (not self.static_geometry) or (not self._geometry_built):

@@ -133,3 +133,24 @@ def test_voxelizer_counts_random(psc3d):
             np.add.at(ref_cnt, ids, np.rint(vals * s_ ** 3))
         got = np.rint(sol.alpha_sum_flat * s_ ** 3)
         assert np.array_equal(got, ref_cnt)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision):
+    """psc_run_steps (direct launches and hipGraph replays, with eager reads in
+    between the replays) gives the same state as stepping one call at a time."""
+    g = G.load("perm24")
+    a, _ = _solver(psc3d, g, precision)
+    b, _ = _solver(psc3d, g, precision)
+    c, _ = _solver(psc3d, g, precision)
+    for chunk in (16, 16, 5, 32):
+        for _ in range(chunk):
+            a.collide_stream_psc()
+        b.run(chunk, graph=True, graph_steps=8)
+        c.run(chunk, graph=False)
+        fa = a.f
+        assert np.array_equal(fa, b.f)
+        assert np.array_equal(fa, c.f)
+        assert np.array_equal(a.hydro_forces, b.hydro_forces)
+        assert np.array_equal(a.hydro_torques, c.hydro_torques)
+    assert a._step_count == b._step_count == 69

@@ -64,6 +64,7 @@ def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch):
     sol = psc3d.PscD3Q19(n, table, tau=0.8, subsamples=4, body_force=(1.0e-6, 0.0, 0.0), static_geometry=True,
                          precision=precision, stream_layout=layout, device="cuda:0")
     sol.rebuild_geometry()
+    sol._tune()   # launch-shape autotuning happens before the warm-up
     sol.run(warmup)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
@@ -90,7 +91,8 @@ def step_kernel_ms(sol, repeats, torch, lib):
 
     def launch(a, b):
         lib.call("psc_step", a.data_ptr(), b.data_ptr(), sol._csum.data_ptr(), None, sol._imb.data_ptr(),
-                 sol._storage, sol._layout, nx, ny, nz, sol._s, sol.tau, fx, fy, fz, sol._block[0], sol._block[1], h)
+                 sol._storage, sol._layout, nx, ny, nz, sol._s, sol.tau, fx, fy, fz, sol._block[0], sol._block[1],
+                 sol._flags, h)
 
     a, b = sol._a, sol._b
     for _ in range(20):
@@ -242,6 +244,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": read_traffic(),
             "kernel": "k_step_f32 (pull, static)",
+            "launch": {"block": list(sol32._block), "flags": sol32._flags},
             "bytes_per_cell": BYTES_F32_STATIC,
             "kernel_us": kern_ms * 1e3,
         },

@@ -51,6 +51,11 @@ extern "C" {
 #define PSC_PUSH 0
 #define PSC_PULL 1
 
+/* psc_step flags */
+#define PSC_STEP_NONTEMPORAL 1      /* bit 0: non-temporal loads/stores (FP32)      */
+#define PSC_STEP_BPC_MASK 0xff00    /* bits 8-15: cap on resident workgroups per CU */
+#define PSC_STEP_CPT_MASK 0x30000   /* bits 16-17: cells per thread along y (1, 2)  */
+
 #define PSC_NSUM 13 /* number of FP64 sums produced by psc_moments           */
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -74,18 +79,25 @@ int psc_init_state(void* d_state, int32_t storage, int64_t ncell, void* stream);
  * Replaces _sparse_particle_alpha_task (psc3d.py:60-94) with the sum of
  * SparsePscD3Q19.rebuild_geometry (psc3d.py:328-345) / the dense
  * PscD3Q19.rebuild_geometry (psc3d.py:151-179), and the numerator of
- * solid_velocity (psc3d.py:181-206, :347-373).
- *   d_counts[entry]   = number of samples inside (uint16) for every box entry
+ * solid_velocity (psc3d.py:181-206, :347-373).  One workgroup per chunk:
+ *   entries of chunk c (the cells of its box range with count > 0, in box
+ *   order) are written at [c*chunk_cells, c*chunk_cells + d_chunk_nnz[c]):
+ *     d_ent_cell  wrapped cell id (uint32)
+ *     d_ent_loc   box-local coordinates lx | ly << 10 | lz << 20 (box
+ *                 extents must not exceed 1024 cells)
+ *     d_ent_cnt   number of sub-samples inside (uint16; alpha = cnt/s^3)
  *   d_count_sum[cell] += count              (must be cleared before the call)
  *   d_us_num[a*N+cell] += (count/s^3)*(U + omega x r)_a   if d_us_num != NULL
  *                     (FP32 if us_precision == 1, FP64 if 2; must be cleared)
  * Box cells are wrapped into the lattice, so clamped boxes (the reference,
  * psc3d.py:73-78) and unclamped periodic boxes use the same call; lever
- * arms use the unwrapped cell position. */
+ * arms use the unwrapped cell position.  Deterministic apart from the
+ * order of the FP atomics into d_us_num on cells shared by two spheres. */
 int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
                  int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz,
-                 int32_t subsamples, uint16_t* d_counts, uint32_t* d_count_sum,
-                 void* d_us_num, int32_t us_precision, void* stream);
+                 int32_t subsamples, uint32_t* d_ent_cell, uint32_t* d_ent_loc, uint16_t* d_ent_cnt,
+                 int32_t* d_chunk_nnz, uint32_t* d_count_sum, void* d_us_num, int32_t us_precision,
+                 void* stream);
 
 /* K2 -- fused PSC collision + streaming, one lattice step.
  * Replaces PscD3Q19.collide_stream_psc (psc3d.py:208-246) with equilibrium
@@ -96,26 +108,33 @@ int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_
  * reference.  d_imb (3 planes, optional) receives sum_i (B Omega_s)_i c_i
  * (psc3d.py:234-236) in every cell with count_sum > 0; other cells are not
  * written.  d_in and d_out must not alias.  block_z/block_y select the
- * workgroup shape (0 = default); storage/layout as above.  Precision of
- * d_in/d_out/d_us_num/d_imb follows `storage`. */
+ * workgroup shape (0 = default); flags (FP32 storage): PSC_STEP_NONTEMPORAL
+ * streams the distributions with non-temporal loads/stores, bits 8-15 cap
+ * the resident workgroups per CU (through an LDS reservation, 0 = no cap),
+ * bits 16-17 set the cells per thread (1 or 2);
+ * storage/layout as above.  Precision of d_in/d_out/d_us_num/d_imb follows
+ * `storage`. */
 int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const void* d_us_num,
              void* d_imb, int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz,
              int32_t subsamples, double tau, double fx, double fy, double fz, int32_t block_z,
-             int32_t block_y, void* stream);
+             int32_t block_y, int32_t flags, void* stream);
 
 /* K3 -- node-summed hydrodynamic force and torque per particle.
  * Replaces accumulate_particle_loads (psc3d.py:255-281 dense, :375-407
- * sparse): owner = (count/s^3)/max(count_sum/s^3, 1e-14), node force
+ * sparse): owner = alpha_p / sum alpha = cnt / count_sum, node force
  * -owner*imb, torque about the centre with lever arm (cell + 1/2) - centre.
- * Two launches: per-chunk FP64 partial sums (d_partials, nchunks*6 doubles)
- * then a fixed-order per-particle sum over d_chunk_first[p]..[p+1] into
- * d_loads (P*6 doubles: Fx Fy Fz Tx Ty Tz).  Deterministic.
+ * Pass 1 sums the entries psc_voxelize wrote for each chunk in FP64 into
+ * d_partials (nchunks*6 doubles); pass 2 adds the partials of chunks
+ * d_chunk_first[p] .. d_chunk_first[p+1]-1 into d_loads (P*6 doubles: Fx Fy
+ * Fz Tx Ty Tz).  Fixed thread assignment and reduction trees: deterministic.
  * imb_precision: 1 = FP32 imb, 2 = FP64 imb. */
 int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
                        const int32_t* d_chunk_first, int32_t nparticles, int32_t nchunks,
-                       int32_t chunk_cells, const uint16_t* d_counts, const uint32_t* d_count_sum,
-                       const void* d_imb, int32_t imb_precision, int32_t nx, int32_t ny, int32_t nz,
-                       int32_t subsamples, double* d_partials, double* d_loads, void* stream);
+                       int32_t chunk_cells, const uint32_t* d_ent_cell, const uint32_t* d_ent_loc,
+                       const uint16_t* d_ent_cnt, const int32_t* d_chunk_nnz,
+                       const uint32_t* d_count_sum, const void* d_imb, int32_t imb_precision,
+                       int32_t nx, int32_t ny, int32_t nz, double* d_partials, double* d_loads,
+                       void* stream);
 
 /* K4 -- post-stream moments and FP64 readout sums.
  * Replaces the tail of collide_stream_psc (psc3d.py:243-253) and the
@@ -157,16 +176,21 @@ typedef struct PscStepPlan {
   const uint32_t* count_sum;
   const void* us_num;
   void* imb;
+  void* imb_alt;        /* optional second imb buffer (enables pipelining)   */
   int32_t storage, layout, nx, ny, nz, subsamples;
   double tau, fx, fy, fz;
-  int32_t block_z, block_y;
+  int32_t block_z, block_y, flags;
   const double* particles;
   const int32_t* boxes;
   const int32_t* chunks;
   const int32_t* chunk_first;
   int32_t nparticles, nchunks, chunk_cells;
-  const uint16_t* counts;
+  const uint32_t* ent_cell;
+  const uint32_t* ent_loc;
+  const uint16_t* ent_cnt;
+  const int32_t* chunk_nnz;
   double* partials;
+  double* partials_alt; /* optional second partials buffer (pipelining)      */
   double* loads;
 } PscStepPlan;
 
@@ -175,6 +199,12 @@ typedef struct PscStepPlan {
  * run_psc_permeability_resolution.py:137-138): K2 from the current buffer
  * into the other one, then K3.  The step starts from state_a when *parity is
  * 0 (state_b when 1); on return *parity names the buffer holding the state.
+ * A step writes imb, or imb_alt when it is set and the step's input has
+ * parity 1 (then the last step's imb is the one of parity 1 - *parity on
+ * return).  With particles and imb_alt/partials_alt set, the load reduction
+ * of step k runs on a library-owned side stream concurrently with step k+1
+ * (fork/join through events, captured into the graph as parallel branches);
+ * every step's loads are computed and d_loads ends with the last step's.
  * use_graph = 1 captures `graph_steps` steps (even, >= 2) once per plan into
  * a hipGraph and replays it; the remainder runs as direct launches.  Graphs
  * are cached per plan contents until psc_graph_cache_clear(). */

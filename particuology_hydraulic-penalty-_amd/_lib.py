@@ -17,6 +17,7 @@ PSC_F32_DEV = 1
 PSC_F64_PLAIN = 2
 PSC_PUSH = 0
 PSC_PULL = 1
+PSC_STEP_NONTEMPORAL = 1
 PSC_NSUM = 13
 
 # Names and argument types of every exported entry point, in header order.
@@ -30,11 +31,11 @@ SIGNATURES = {
     "psc_lattice_constants": ([_P, _P, _P], ct.c_int),
     "psc_clear": ([_P, _I64, _P], ct.c_int),
     "psc_init_state": ([_P, _I32, _I64, _P], ct.c_int),
-    "psc_voxelize": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I32, _P], ct.c_int),
-    "psc_step": ([_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _P],
+    "psc_voxelize": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, _P], ct.c_int),
+    "psc_step": ([_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _I32, _P],
                  ct.c_int),
-    "psc_particle_loads": ([_P, _P, _P, _P, _I32, _I32, _I32, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P,
-                            _P], ct.c_int),
+    "psc_particle_loads": ([_P, _P, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P,
+                            _P, _P], ct.c_int),
     "psc_moments": ([_P, _I32, _I32, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _P, _P, _P, _P], ct.c_int),
     "psc_moments_blocks": ([_I64], ct.c_int64),
     "psc_export_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
@@ -49,13 +50,14 @@ class PscStepPlan(ct.Structure):
     """Mirror of struct PscStepPlan (include/psc_hip.h)."""
 
     _fields_ = [
-        ("state_a", _P), ("state_b", _P), ("count_sum", _P), ("us_num", _P), ("imb", _P),
+        ("state_a", _P), ("state_b", _P), ("count_sum", _P), ("us_num", _P), ("imb", _P), ("imb_alt", _P),
         ("storage", _I32), ("layout", _I32), ("nx", _I32), ("ny", _I32), ("nz", _I32), ("subsamples", _I32),
         ("tau", _D), ("fx", _D), ("fy", _D), ("fz", _D),
-        ("block_z", _I32), ("block_y", _I32),
+        ("block_z", _I32), ("block_y", _I32), ("flags", _I32),
         ("particles", _P), ("boxes", _P), ("chunks", _P), ("chunk_first", _P),
         ("nparticles", _I32), ("nchunks", _I32), ("chunk_cells", _I32),
-        ("counts", _P), ("partials", _P), ("loads", _P),
+        ("ent_cell", _P), ("ent_loc", _P), ("ent_cnt", _P), ("chunk_nnz", _P), ("partials", _P),
+        ("partials_alt", _P), ("loads", _P),
     ]
 
 

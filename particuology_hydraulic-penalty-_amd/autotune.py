@@ -1,0 +1,132 @@
+"""Launch-configuration autotuner for the collide+stream kernel (psc_step).
+
+The FP32 step is HBM-bound; how close it gets to the roofline depends on the
+workgroup shape, on non-temporal (streaming) loads/stores and on how many
+workgroups are resident per CU (fewer resident waves keep fewer DRAM pages
+open at once).  The best point moves with the lattice shape, so it is
+measured, not guessed: every candidate is timed with HIP events on the
+solver's own buffers (state A -> scratch B; the state is not modified) and
+the fastest is kept.  Results are cached per (device, lattice, storage,
+stream layout, moving) in memory and in a JSON file
+($PSC_HIP_TUNE_CACHE, default ~/.cache/psc_hip/step_tuning.json).
+
+All candidates compute bit-identical results; only the launch differs.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import threading
+
+from . import _lib
+
+_LOCK = threading.Lock()
+_MEM: dict = {}
+
+# (block_z, block_y, nontemporal, workgroups-per-CU cap; 0 = none)
+SHAPES = [(64, 4), (64, 2), (64, 1), (128, 2), (128, 1), (256, 1)]
+NT = (0, 1)
+BPC = (0, 2, 3, 4, 6, 8)
+
+
+def candidates(nz: int):
+    out = []
+    for bz, by in SHAPES:
+        if bz > 64 and nz <= bz // 2:
+            continue  # more than half the threads of a row would idle
+        for nt in NT:
+            for bpc in BPC:
+                out.append((bz, by, nt, bpc))
+    return out
+
+
+def _cache_path():
+    return os.environ.get("PSC_HIP_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "psc_hip", "step_tuning.json"))
+
+
+def _key(solver):
+    import torch
+    name = torch.cuda.get_device_properties(solver.device).name
+    arch = getattr(torch.cuda.get_device_properties(solver.device), "gcnArchName", "")
+    return "|".join(str(v) for v in (name, arch, solver._shape, solver._storage, solver._layout,
+                                      bool(getattr(solver, "_moving", False))))
+
+
+def _load_file():
+    try:
+        with open(_cache_path()) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_file(key, cfg):
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        data = _load_file()
+        data[key] = cfg
+        tmp = path + ".tmp%d" % os.getpid()
+        with open(tmp, "w") as fh:
+            json.dump(data, fh, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # the cache is an optimisation only
+
+
+def flags_of(nt: int, bpc: int, cpt: int = 1) -> int:
+    return (_lib.PSC_STEP_NONTEMPORAL if nt else 0) | ((int(bpc) & 0xFF) << 8) | ((int(cpt) & 3) << 16)
+
+
+def time_config(solver, bz, by, flags, repeats):
+    """Average psc_step duration (ms) of one launch configuration."""
+    import torch
+    nx, ny, nz = solver._shape
+    fx, fy, fz = solver.body_force
+    stream = torch.cuda.current_stream(solver.device)
+    h = stream.cuda_stream
+    us = solver._ptr(solver._us) if getattr(solver, "_moving", False) else None
+    args = (solver._ptr(solver._a), solver._ptr(solver._b), solver._ptr(solver._csum), us, solver._ptr(solver._imb),
+            solver._storage, solver._layout, nx, ny, nz, solver._s, solver.tau, fx, fy, fz, bz, by, flags, h)
+    lib = _lib.load()
+    rc = lib.psc_step(*args)
+    _lib.check(rc, "psc_step")
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(repeats):
+        lib.psc_step(*args)
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / repeats
+
+
+def tune(solver, budget_ms: float = 400.0, use_cache: bool = True):
+    """Return (block_z, block_y, flags) of the fastest candidate for solver."""
+    key = _key(solver)
+    with _LOCK:
+        if use_cache:
+            if key in _MEM:
+                return _MEM[key]
+            disk = _load_file().get(key)
+            if disk:
+                cfg = (int(disk[0]), int(disk[1]), int(disk[2]))
+                _MEM[key] = cfg
+                return cfg
+        nz = solver._shape[2]
+        cands = candidates(nz)
+        # one probe launch sizes the repeat count to the time budget
+        probe = time_config(solver, 64, 4, 0, 1)
+        reps = max(2, min(50, int(budget_ms / max(probe, 1e-3) / len(cands))))
+        best = None
+        for bz, by, nt, bpc in cands:
+            flags = flags_of(nt, bpc)
+            ms = time_config(solver, bz, by, flags, reps)
+            if best is None or ms < best[0]:
+                best = (ms, bz, by, flags)
+        cfg = (best[1], best[2], best[3])
+        _MEM[key] = cfg
+        _store_file(key, list(cfg) + [best[0]])
+        return cfg

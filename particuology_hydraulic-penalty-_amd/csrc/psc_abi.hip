@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "psc_common.h"
+#include "psc_loads.h"
 #include "../../include/psc_hip.h"
 
 namespace psc {
@@ -39,14 +40,18 @@ int check_launch(const char* what) {
 int launch_step_f64(const double*, double*, const uint32_t*, const double*, double*, int, int, int, int, int, double,
                     double, double, double, int, int, hipStream_t);
 int launch_step_f32(const float*, float*, const uint32_t*, const float*, float*, int, int, int, int, int, double,
-                    double, double, double, int, int, hipStream_t);
-int launch_voxelize(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, uint16_t*, uint32_t*,
-                    void*, int, hipStream_t);
-int launch_loads(const double*, const int32_t*, const int32_t*, const int32_t*, int, int, int, const uint16_t*,
-                 const uint32_t*, const void*, int, int, int, int, int, double*, double*, hipStream_t);
+                    double, double, double, int, int, int, hipStream_t);
+int launch_voxelize(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, uint32_t*, uint32_t*,
+                    uint16_t*, int32_t*, uint32_t*, void*, int, hipStream_t);
+int launch_loads(const double*, const int32_t*, const int32_t*, const int32_t*, int, int, int, const uint32_t*,
+                 const uint32_t*, const uint16_t*, const int32_t*, const uint32_t*, const void*, int, int, int, int,
+                 double*, double*, hipStream_t);
 int launch_moments(const void*, int, int, const uint32_t*, const void*, int, int, int, int, double, double, double,
                    double*, double*, double*, hipStream_t);
 int64_t moments_blocks(int64_t);
+int launch_loads_chunks(const LoadsArgs&, const void*, int, int, int, int, double*, hipStream_t);
+int launch_loads_final(const int32_t*, int, const double*, double*, hipStream_t);
+
 int launch_export(const void*, int, int, int, int, int, double*, hipStream_t);
 int launch_import(const double*, int, int, int, int, int, void*, hipStream_t);
 int launch_init_plain(double*, size_t, hipStream_t);
@@ -133,24 +138,27 @@ int psc_init_state(void* d_state, int32_t storage, int64_t ncell, void* stream) 
 }
 
 int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks, int32_t nchunks,
-                 int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, uint16_t* d_counts,
-                 uint32_t* d_count_sum, void* d_us_num, int32_t us_precision, void* stream) {
+                 int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, uint32_t* d_ent_cell,
+                 uint32_t* d_ent_loc, uint16_t* d_ent_cnt, int32_t* d_chunk_nnz, uint32_t* d_count_sum,
+                 void* d_us_num, int32_t us_precision, void* stream) {
   if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_voxelize: bad lattice %dx%dx%d", nx, ny, nz);
   if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
     return set_error(PSC_EINVAL, "psc_voxelize: subsamples %d outside [1, 40]", subsamples);
   if (nchunks < 0 || chunk_cells < 1) return set_error(PSC_EINVAL, "psc_voxelize: bad chunking");
-  if (nchunks > 0 && (d_particles == nullptr || d_boxes == nullptr || d_chunks == nullptr || d_counts == nullptr ||
+  if (nchunks > 0 && (d_particles == nullptr || d_boxes == nullptr || d_chunks == nullptr || d_ent_cell == nullptr ||
+                      d_ent_loc == nullptr || d_ent_cnt == nullptr || d_chunk_nnz == nullptr ||
                       d_count_sum == nullptr))
     return set_error(PSC_EINVAL, "psc_voxelize: null pointer");
   if (d_us_num != nullptr && us_precision != 1 && us_precision != 2)
     return set_error(PSC_EINVAL, "psc_voxelize: us_precision %d", us_precision);
-  return launch_voxelize(d_particles, d_boxes, d_chunks, nchunks, chunk_cells, nx, ny, nz, subsamples, d_counts,
-                         d_count_sum, d_us_num, d_us_num ? us_precision : 0, (hipStream_t)stream);
+  return launch_voxelize(d_particles, d_boxes, d_chunks, nchunks, chunk_cells, nx, ny, nz, subsamples, d_ent_cell,
+                         d_ent_loc, d_ent_cnt, d_chunk_nnz, d_count_sum, d_us_num, d_us_num ? us_precision : 0,
+                         (hipStream_t)stream);
 }
 
 int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const void* d_us_num, void* d_imb,
              int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double tau,
-             double fx, double fy, double fz, int32_t block_z, int32_t block_y, void* stream) {
+             double fx, double fy, double fz, int32_t block_z, int32_t block_y, int32_t flags, void* stream) {
   if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_step: bad lattice %dx%dx%d", nx, ny, nz);
   if (d_in == nullptr || d_out == nullptr || d_count_sum == nullptr) return set_error(PSC_EINVAL, "psc_step: null");
   if (d_in == d_out) return set_error(PSC_EINVAL, "psc_step: input and output alias");
@@ -161,12 +169,15 @@ int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const v
   int bz = block_z, by = block_y;
   if (bz <= 0) bz = nz >= 64 ? 64 : 32;
   if (by <= 0) by = 256 / bz > 0 ? 256 / bz : 1;
+  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK)) != 0)
+    return set_error(PSC_EINVAL, "psc_step: unknown flags 0x%x", flags);
+  if (((flags & PSC_STEP_CPT_MASK) >> 16) > 2) return set_error(PSC_EINVAL, "psc_step: cells per thread > 2");
   if (bz * by > 256 || bz * by < 1 || ((bz * by) % 64) != 0)
     return set_error(PSC_ELAUNCH, "psc_step: block %dx%d must hold a multiple of 64 threads, at most 256", bz, by);
   hipStream_t s = (hipStream_t)stream;
   if (storage == PSC_F32_DEV)
     return launch_step_f32((const float*)d_in, (float*)d_out, d_count_sum, (const float*)d_us_num, (float*)d_imb,
-                           layout, nx, ny, nz, subsamples, tau, fx, fy, fz, bz, by, s);
+                           layout, nx, ny, nz, subsamples, tau, fx, fy, fz, bz, by, flags, s);
   if (storage == PSC_F64_PLAIN)
     return launch_step_f64((const double*)d_in, (double*)d_out, d_count_sum, (const double*)d_us_num,
                            (double*)d_imb, layout, nx, ny, nz, subsamples, tau, fx, fy, fz, bz, by, s);
@@ -175,22 +186,22 @@ int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const v
 
 int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
                        const int32_t* d_chunk_first, int32_t nparticles, int32_t nchunks, int32_t chunk_cells,
-                       const uint16_t* d_counts, const uint32_t* d_count_sum, const void* d_imb,
-                       int32_t imb_precision, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples,
-                       double* d_partials, double* d_loads, void* stream) {
+                       const uint32_t* d_ent_cell, const uint32_t* d_ent_loc, const uint16_t* d_ent_cnt,
+                       const int32_t* d_chunk_nnz, const uint32_t* d_count_sum, const void* d_imb,
+                       int32_t imb_precision, int32_t nx, int32_t ny, int32_t nz, double* d_partials,
+                       double* d_loads, void* stream) {
   if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_particle_loads: bad lattice");
   if (nparticles < 0 || nchunks < 0 || chunk_cells < 1) return set_error(PSC_EINVAL, "psc_particle_loads: sizes");
   if (imb_precision != 1 && imb_precision != 2) return set_error(PSC_EINVAL, "psc_particle_loads: imb precision");
-  if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
-    return set_error(PSC_EINVAL, "psc_particle_loads: subsamples %d", subsamples);
-  if (nparticles > 0 && (d_chunk_first == nullptr || d_loads == nullptr))
+  if (nparticles > 0 && (d_chunk_first == nullptr || d_loads == nullptr || d_partials == nullptr))
     return set_error(PSC_EINVAL, "psc_particle_loads: null output");
-  if (nchunks > 0 && (d_particles == nullptr || d_boxes == nullptr || d_chunks == nullptr || d_counts == nullptr ||
-                      d_count_sum == nullptr || d_imb == nullptr || d_partials == nullptr))
+  if (nchunks > 0 && (d_particles == nullptr || d_boxes == nullptr || d_chunks == nullptr || d_ent_cell == nullptr ||
+                      d_ent_loc == nullptr || d_ent_cnt == nullptr || d_chunk_nnz == nullptr ||
+                      d_count_sum == nullptr || d_imb == nullptr))
     return set_error(PSC_EINVAL, "psc_particle_loads: null input");
-  return launch_loads(d_particles, d_boxes, d_chunks, d_chunk_first, nparticles, nchunks, chunk_cells, d_counts,
-                      d_count_sum, d_imb, imb_precision, nx, ny, nz, subsamples, d_partials, d_loads,
-                      (hipStream_t)stream);
+  return launch_loads(d_particles, d_boxes, d_chunks, d_chunk_first, nparticles, nchunks, chunk_cells, d_ent_cell,
+                      d_ent_loc, d_ent_cnt, d_chunk_nnz, d_count_sum, d_imb, imb_precision, nx, ny, nz, d_partials,
+                      d_loads, (hipStream_t)stream);
 }
 
 int64_t psc_moments_blocks(int64_t ncell) { return moments_blocks(ncell); }
@@ -268,6 +279,7 @@ PscStepPlan normalized(const PscStepPlan& p) {
   k.count_sum = p.count_sum;
   k.us_num = p.us_num;
   k.imb = p.imb;
+  k.imb_alt = p.imb_alt;
   k.storage = p.storage;
   k.layout = p.layout;
   k.nx = p.nx;
@@ -280,6 +292,7 @@ PscStepPlan normalized(const PscStepPlan& p) {
   k.fz = p.fz;
   k.block_z = p.block_z;
   k.block_y = p.block_y;
+  k.flags = p.flags;
   k.particles = p.particles;
   k.boxes = p.boxes;
   k.chunks = p.chunks;
@@ -287,8 +300,12 @@ PscStepPlan normalized(const PscStepPlan& p) {
   k.nparticles = p.nparticles;
   k.nchunks = p.nchunks;
   k.chunk_cells = p.chunk_cells;
-  k.counts = p.counts;
+  k.ent_cell = p.ent_cell;
+  k.ent_loc = p.ent_loc;
+  k.ent_cnt = p.ent_cnt;
+  k.chunk_nnz = p.chunk_nnz;
   k.partials = p.partials;
+  k.partials_alt = p.partials_alt;
   k.loads = p.loads;
   return k;
 }
@@ -296,13 +313,16 @@ PscStepPlan normalized(const PscStepPlan& p) {
 int one_step(const PscStepPlan& p, int parity, hipStream_t s) {
   void* in = parity ? p.state_b : p.state_a;
   void* out = parity ? p.state_a : p.state_b;
-  int rc = psc_step(in, out, p.count_sum, p.us_num, p.imb, p.storage, p.layout, p.nx, p.ny, p.nz, p.subsamples,
-                    p.tau, p.fx, p.fy, p.fz, p.block_z, p.block_y, s);
+  // with a second imb buffer, the input parity selects it (as in the pipeline)
+  void* imb = (parity && p.imb_alt != nullptr) ? p.imb_alt : p.imb;
+  double* partials = (parity && p.partials_alt != nullptr) ? p.partials_alt : p.partials;
+  int rc = psc_step(in, out, p.count_sum, p.us_num, imb, p.storage, p.layout, p.nx, p.ny, p.nz, p.subsamples,
+                    p.tau, p.fx, p.fy, p.fz, p.block_z, p.block_y, p.flags, s);
   if (rc) return rc;
   if (p.nparticles > 0) {
     rc = psc_particle_loads(p.particles, p.boxes, p.chunks, p.chunk_first, p.nparticles, p.nchunks, p.chunk_cells,
-                            p.counts, p.count_sum, p.imb, p.storage == PSC_F32_DEV ? 1 : 2, p.nx, p.ny, p.nz,
-                            p.subsamples, p.partials, p.loads, s);
+                            p.ent_cell, p.ent_loc, p.ent_cnt, p.chunk_nnz, p.count_sum, imb,
+                            p.storage == PSC_F32_DEV ? 1 : 2, p.nx, p.ny, p.nz, partials, p.loads, s);
   }
   return rc;
 }
@@ -326,6 +346,88 @@ int capture_stream(hipStream_t* out) {
   return PSC_OK;
 }
 
+// Side stream and events of the overlapped loop (per device, library-owned,
+// default priority).
+struct SideRes {
+  hipStream_t side = nullptr;
+  hipEvent_t ev_step = nullptr;
+  hipEvent_t ev_loads[2] = {nullptr, nullptr};
+};
+std::vector<SideRes> g_side;
+
+int side_resources(SideRes** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error((int)e, "psc_run_steps: hipGetDevice: %s", hipGetErrorString(e));
+  if ((int)g_side.size() <= dev) g_side.resize(dev + 1);
+  SideRes& r = g_side[dev];
+  if (r.side == nullptr) {
+    e = hipStreamCreateWithFlags(&r.side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r.ev_step, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r.ev_loads[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r.ev_loads[1], hipEventDisableTiming);
+    if (e != hipSuccess) return set_error((int)e, "psc_run_steps: side stream: %s", hipGetErrorString(e));
+  }
+  *out = &r;
+  return PSC_OK;
+}
+
+#define PSC_HIP_TRY(x)                                                                          \
+  do {                                                                                          \
+    const hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) return set_error((int)e_, "psc_run_steps: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+// n steps from parity `par`.  With imb_alt and partials_alt set and
+// particles present, the particle-load reduction of step k runs on a side
+// stream while step k+1 streams the lattice on `s` (step k+2 waits for it,
+// since it rewrites the imb buffer step k's loads read); the side stream is
+// joined back into `s` at the end, so on return every step's loads have been
+// computed and d_loads holds the last step's.  Otherwise steps and loads run
+// in order on `s`.
+int run_sequence(const PscStepPlan& p, int n, int* par, hipStream_t s) {
+  const bool overlap = p.nparticles > 0 && p.imb != nullptr && p.imb_alt != nullptr && p.partials != nullptr &&
+                       p.partials_alt != nullptr;
+  if (!overlap) {
+    for (int k = 0; k < n; ++k) {
+      const int rc = one_step(p, *par, s);
+      if (rc) return rc;
+      *par ^= 1;
+    }
+    return PSC_OK;
+  }
+  if (n == 0) return PSC_OK;
+  SideRes* r = nullptr;
+  int rc = side_resources(&r);
+  if (rc) return rc;
+  const LoadsArgs la{p.particles, p.boxes, p.chunks, p.chunk_first, p.nparticles, p.nchunks, p.chunk_cells,
+                     p.ent_cell, p.ent_loc, p.ent_cnt, p.chunk_nnz, p.count_sum};
+  void* imbs[2] = {p.imb, p.imb_alt};
+  double* prt[2] = {p.partials, p.partials_alt};
+  const int imbp = p.storage == PSC_F32_DEV ? 1 : 2;
+  // the side stream starts after the work already queued on s
+  PSC_HIP_TRY(hipEventRecord(r->ev_step, s));
+  PSC_HIP_TRY(hipStreamWaitEvent(r->side, r->ev_step, 0));
+  for (int k = 0; k < n; ++k) {
+    const int pk = *par;
+    if (k >= 2) PSC_HIP_TRY(hipStreamWaitEvent(s, r->ev_loads[k & 1], 0));
+    rc = psc_step(pk ? p.state_b : p.state_a, pk ? p.state_a : p.state_b, p.count_sum, p.us_num, imbs[pk],
+                  p.storage, p.layout, p.nx, p.ny, p.nz, p.subsamples, p.tau, p.fx, p.fy, p.fz, p.block_z,
+                  p.block_y, p.flags, s);
+    if (rc) return rc;
+    PSC_HIP_TRY(hipEventRecord(r->ev_step, s));
+    PSC_HIP_TRY(hipStreamWaitEvent(r->side, r->ev_step, 0));
+    rc = launch_loads_chunks(la, imbs[pk], imbp, p.nx, p.ny, p.nz, prt[pk], r->side);
+    if (rc) return rc;
+    rc = launch_loads_final(p.chunk_first, p.nparticles, prt[pk], p.loads, r->side);
+    if (rc) return rc;
+    PSC_HIP_TRY(hipEventRecord(r->ev_loads[k & 1], r->side));
+    *par ^= 1;
+  }
+  PSC_HIP_TRY(hipStreamWaitEvent(s, r->ev_loads[(n - 1) & 1], 0));
+  return PSC_OK;
+}
+
 int graph_for(const PscStepPlan& p, int steps, int parity, hipGraphExec_t* out) {
   std::lock_guard<std::mutex> lock(g_graph_mu);
   for (const GraphEntry& e : g_graphs) {
@@ -339,12 +441,8 @@ int graph_for(const PscStepPlan& p, int steps, int parity, hipGraphExec_t* out) 
   if (rcs) return rcs;
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return set_error((int)e, "psc_run_steps: begin capture: %s", hipGetErrorString(e));
-  int rc = PSC_OK;
   int par = parity;
-  for (int k = 0; k < steps && rc == PSC_OK; ++k) {
-    rc = one_step(p, par, s);
-    par ^= 1;
-  }
+  const int rc = run_sequence(p, steps, &par, s);
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(s, &graph);
   if (rc) {
@@ -388,10 +486,9 @@ extern "C" int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t us
       done += graph_steps;  // even: parity unchanged
     }
   }
-  for (; done < nsteps; ++done) {
-    const int rc = one_step(key, par, s);
+  if (done < nsteps) {
+    const int rc = run_sequence(key, nsteps - done, &par, s);
     if (rc) return rc;
-    par ^= 1;
   }
   *parity = par;
   return PSC_OK;

@@ -5,6 +5,7 @@
 // then reproduces src/lbm_dem/psc3d.py bit for bit on cells owned by one
 // particle.
 #include "psc_common.h"
+#include "psc_loads.h"
 #include "../../include/psc_hip.h"
 
 namespace psc {
@@ -296,11 +297,39 @@ __device__ __forceinline__ Box load_box(const int32_t* __restrict__ b, int p) {
   return Box{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
 }
 
+// Ordered compaction of the non-empty cells of one workgroup round: returns
+// the position of this thread's entry among the round's entries and writes
+// the round total to *total (uniform).  Wave ballots + an LDS sum over the
+// (at most 4) waves keep entry order = box order, so the output is
+// deterministic.
+__device__ __forceinline__ int round_position(bool flag, int* lds_waves, int* total) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(flag);
+  const int before = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) lds_waves[wave] = __popcll(m);
+  __syncthreads();
+  int off = 0, tot = 0;
+  const int nw = blockDim.x >> 6;
+  for (int w = 0; w < nw; ++w) {
+    const int c = lds_waves[w];
+    off += (w < wave) ? c : 0;
+    tot += c;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + before;
+}
+
+
 template <int USP>
 __global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ parts, const int32_t* __restrict__ boxes,
                                                   const int32_t* __restrict__ chunks, int chunk_cells, Dims d,
-                                                  int s, uint16_t* __restrict__ counts,
-                                                  uint32_t* __restrict__ csum, void* __restrict__ usn) {
+                                                  int s, uint32_t* __restrict__ ent_cell,
+                                                  uint32_t* __restrict__ ent_loc, uint16_t* __restrict__ ent_cnt,
+                                                  int32_t* __restrict__ chunk_nnz, uint32_t* __restrict__ csum,
+                                                  void* __restrict__ usn) {
+  __shared__ int lds_waves[4];
   const int pid = chunks[2 * blockIdx.x];
   const int e0 = chunks[2 * blockIdx.x + 1];
   const Box bx = load_box(boxes, pid);
@@ -310,102 +339,74 @@ __global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ par
   const double r2 = p.r * p.r;
   const double s3 = (double)(s * s * s);
   const size_t N = (size_t)d.nx * d.ny * d.nz;
-  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
-    const int lz = e % bx.ez;
-    const int t = e / bx.ez;
-    const int ly = t % bx.ey;
-    const int lx = t / bx.ey;
-    const int gx = bx.x0 + lx, gy = bx.y0 + ly, gz = bx.z0 + lz;
-    const int cnt = sample_count(gx, gy, gz, p, r2, s);
-    counts[(size_t)bx.off + e] = (uint16_t)cnt;
-    if (cnt == 0) continue;
-    const uint32_t cell = cell_id(d, wrap_any(gx, d.nx), wrap_any(gy, d.ny), wrap_any(gz, d.nz));
-    atomicAdd(csum + cell, (uint32_t)cnt);
-    if (USP != 0) {
-      const double a = (double)cnt / s3;
-      const double rx = ((double)gx + 0.5) - p.x;
-      const double ry = ((double)gy + 0.5) - p.y;
-      const double rz = ((double)gz + 0.5) - p.z;
-      const double vx = (p.vx + p.wy * rz) - p.wz * ry;
-      const double vy = (p.vy + p.wz * rx) - p.wx * rz;
-      const double vz = (p.vz + p.wx * ry) - p.wy * rx;
-      if (USP == 2) {
-        double* u = (double*)usn;
-        unsafeAtomicAdd(u + cell, a * vx);
-        unsafeAtomicAdd(u + N + cell, a * vy);
-        unsafeAtomicAdd(u + 2 * N + cell, a * vz);
-      } else {
-        float* u = (float*)usn;
-        unsafeAtomicAdd(u + cell, (float)(a * vx));
-        unsafeAtomicAdd(u + N + cell, (float)(a * vy));
-        unsafeAtomicAdd(u + 2 * N + cell, (float)(a * vz));
+  const size_t out0 = (size_t)blockIdx.x * chunk_cells;
+  int written = 0;
+  for (int r = e0; r < e1; r += blockDim.x) {
+    const int e = r + (int)threadIdx.x;
+    int cnt = 0, lx = 0, ly = 0, lz = 0;
+    uint32_t cell = 0;
+    if (e < e1) {
+      lz = e % bx.ez;
+      const int t = e / bx.ez;
+      ly = t % bx.ey;
+      lx = t / bx.ey;
+      const int gx = bx.x0 + lx, gy = bx.y0 + ly, gz = bx.z0 + lz;
+      cnt = sample_count(gx, gy, gz, p, r2, s);
+      if (cnt) {
+        cell = cell_id(d, wrap_any(gx, d.nx), wrap_any(gy, d.ny), wrap_any(gz, d.nz));
+        atomicAdd(csum + cell, (uint32_t)cnt);
+        if (USP != 0) {
+          const double a = (double)cnt / s3;
+          const double rx = ((double)gx + 0.5) - p.x;
+          const double ry = ((double)gy + 0.5) - p.y;
+          const double rz = ((double)gz + 0.5) - p.z;
+          const double vx = (p.vx + p.wy * rz) - p.wz * ry;
+          const double vy = (p.vy + p.wz * rx) - p.wx * rz;
+          const double vz = (p.vz + p.wx * ry) - p.wy * rx;
+          if (USP == 2) {
+            double* u = (double*)usn;
+            unsafeAtomicAdd(u + cell, a * vx);
+            unsafeAtomicAdd(u + N + cell, a * vy);
+            unsafeAtomicAdd(u + 2 * N + cell, a * vz);
+          } else {
+            float* u = (float*)usn;
+            unsafeAtomicAdd(u + cell, (float)(a * vx));
+            unsafeAtomicAdd(u + N + cell, (float)(a * vy));
+            unsafeAtomicAdd(u + 2 * N + cell, (float)(a * vz));
+          }
+        }
       }
     }
+    int tot = 0;
+    const int pos = round_position(cnt != 0, lds_waves, &tot);
+    if (cnt) {
+      const size_t o = out0 + written + pos;
+      ent_cell[o] = cell;
+      ent_loc[o] = (uint32_t)lx | ((uint32_t)ly << kLocBits) | ((uint32_t)lz << (2 * kLocBits));
+      ent_cnt[o] = (uint16_t)cnt;
+    }
+    written += tot;
   }
+  if (threadIdx.x == 0) chunk_nnz[blockIdx.x] = written;
 }
 
 // ---------------------------------------------------------------------------
 // K3: node-summed force/torque per particle (psc3d.py:255-281, 375-407).
+// Pass 1: one workgroup per chunk sums its compacted entries in FP64.  Pass
+// 2: one wavefront per particle adds its chunk partials.  Fixed thread
+// assignment and reduction trees: deterministic.  Bodies in psc_loads.h.
 // ---------------------------------------------------------------------------
 template <typename TI>
-__global__ __launch_bounds__(256) void k_loads_partial(const double* __restrict__ parts,
-                                                       const int32_t* __restrict__ boxes,
-                                                       const int32_t* __restrict__ chunks, int chunk_cells,
-                                                       const uint16_t* __restrict__ counts,
-                                                       const uint32_t* __restrict__ csum,
-                                                       const TI* __restrict__ imb, Dims d, double s3,
-                                                       double* __restrict__ partials) {
-  __shared__ double lds[6 * 16];
-  const int pid = chunks[2 * blockIdx.x];
-  const int e0 = chunks[2 * blockIdx.x + 1];
-  const Box bx = load_box(boxes, pid);
-  const Particle p = load_particle(parts, pid);
-  const int vol = bx.ex * bx.ey * bx.ez;
-  const int e1 = min(e0 + chunk_cells, vol);
-  const size_t N = (size_t)d.nx * d.ny * d.nz;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
-    const int cnt = counts[(size_t)bx.off + e];
-    if (cnt == 0) continue;
-    const int lz = e % bx.ez;
-    const int t = e / bx.ez;
-    const int ly = t % bx.ey;
-    const int lx = t / bx.ey;
-    const int gx = bx.x0 + lx, gy = bx.y0 + ly, gz = bx.z0 + lz;
-    const uint32_t cell = cell_id(d, wrap_any(gx, d.nx), wrap_any(gy, d.ny), wrap_any(gz, d.nz));
-    const double alpha = (double)cnt / s3;
-    const double asum = (double)csum[cell] / s3;
-    const double owner = alpha / fmax(asum, 1.0e-14);
-    const double nfx = -owner * (double)imb[cell];
-    const double nfy = -owner * (double)imb[N + cell];
-    const double nfz = -owner * (double)imb[2 * N + cell];
-    const double rx = ((double)gx + 0.5) - p.x;
-    const double ry = ((double)gy + 0.5) - p.y;
-    const double rz = ((double)gz + 0.5) - p.z;
-    acc[0] += nfx;
-    acc[1] += nfy;
-    acc[2] += nfz;
-    acc[3] += ry * nfz - rz * nfy;
-    acc[4] += rz * nfx - rx * nfz;
-    acc[5] += rx * nfy - ry * nfx;
-  }
-  block_sum<6>(acc, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) partials[6 * (size_t)blockIdx.x + k] = acc[k];
-  }
+__global__ __launch_bounds__(256) void k_loads_chunk(LoadsArgs a, const TI* __restrict__ imb, Dims d,
+                                                     double* __restrict__ partials) {
+  __shared__ double lds[6 * 4];
+  loads_chunk_body<TI>(a, imb, d.nx, d.ny, d.nz, blockIdx.x, threadIdx.x, blockDim.x, partials, lds);
 }
 
-__global__ void k_loads_final(const int32_t* __restrict__ chunk_first, int np, const double* __restrict__ partials,
-                              double* __restrict__ loads) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int c = chunk_first[p]; c < chunk_first[p + 1]; ++c)
-#pragma unroll
-    for (int k = 0; k < 6; ++k) acc[k] += partials[6 * (size_t)c + k];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) loads[6 * (size_t)p + k] = acc[k];
+__global__ __launch_bounds__(64) void k_loads_final(const int32_t* __restrict__ chunk_first,
+                                                    const double* __restrict__ partials,
+                                                    double* __restrict__ loads) {
+  loads_final_body(chunk_first, blockIdx.x, threadIdx.x, partials, loads);
 }
 
 // ---------------------------------------------------------------------------
@@ -601,40 +602,55 @@ int launch_step_f64(const double* in, double* out, const uint32_t* csum, const d
 }
 
 int launch_voxelize(const double* parts, const int32_t* boxes, const int32_t* chunks, int nchunks, int chunk_cells,
-                    int nx, int ny, int nz, int s, uint16_t* counts, uint32_t* csum, void* usn, int usp,
-                    hipStream_t stream) {
+                    int nx, int ny, int nz, int s, uint32_t* ent_cell, uint32_t* ent_loc, uint16_t* ent_cnt,
+                    int32_t* chunk_nnz, uint32_t* csum, void* usn, int usp, hipStream_t stream) {
   if (nchunks == 0) return PSC_OK;
   const Dims d{nx, ny, nz};
-  const int threads = chunk_cells < 256 ? ((chunk_cells + 63) / 64) * 64 : 256;
   if (usn == nullptr || usp == 0)
-    k_voxelize<0><<<nchunks, threads, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, counts, csum, nullptr);
+    k_voxelize<0><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
+                                               chunk_nnz, csum, nullptr);
   else if (usp == 1)
-    k_voxelize<1><<<nchunks, threads, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, counts, csum, usn);
+    k_voxelize<1><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
+                                               chunk_nnz, csum, usn);
   else
-    k_voxelize<2><<<nchunks, threads, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, counts, csum, usn);
+    k_voxelize<2><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
+                                               chunk_nnz, csum, usn);
   return check_launch("psc_voxelize");
 }
 
 int launch_loads(const double* parts, const int32_t* boxes, const int32_t* chunks, const int32_t* chunk_first, int np,
-                 int nchunks, int chunk_cells, const uint16_t* counts, const uint32_t* csum, const void* imb, int imbp,
-                 int nx, int ny, int nz, int s, double* partials, double* loads, hipStream_t stream) {
+                 int nchunks, int chunk_cells, const uint32_t* ent_cell, const uint32_t* ent_loc,
+                 const uint16_t* ent_cnt, const int32_t* chunk_nnz, const uint32_t* csum, const void* imb, int imbp,
+                 int nx, int ny, int nz, double* partials, double* loads, hipStream_t stream) {
+  if (np == 0) return PSC_OK;
   const Dims d{nx, ny, nz};
-  const double s3 = (double)(s * s * s);
+  const LoadsArgs la{parts, boxes, chunks, chunk_first, np, nchunks, chunk_cells, ent_cell, ent_loc, ent_cnt,
+                     chunk_nnz, csum};
   if (nchunks > 0) {
     if (imbp == 2)
-      k_loads_partial<double><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, counts, csum,
-                                                           (const double*)imb, d, s3, partials);
+      k_loads_chunk<double><<<nchunks, 256, 0, stream>>>(la, (const double*)imb, d, partials);
     else
-      k_loads_partial<float><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, counts, csum,
-                                                          (const float*)imb, d, s3, partials);
-    int rc = check_launch("psc_particle_loads(partial)");
+      k_loads_chunk<float><<<nchunks, 256, 0, stream>>>(la, (const float*)imb, d, partials);
+    const int rc = check_launch("psc_particle_loads(chunks)");
     if (rc) return rc;
   }
-  if (np > 0) {
-    k_loads_final<<<(np + 127) / 128, 128, 0, stream>>>(chunk_first, np, partials, loads);
-    return check_launch("psc_particle_loads(final)");
-  }
-  return PSC_OK;
+  k_loads_final<<<np, 64, 0, stream>>>(chunk_first, partials, loads);
+  return check_launch("psc_particle_loads(final)");
+}
+
+int launch_loads_chunks(const LoadsArgs& la, const void* imb, int imbp, int nx, int ny, int nz, double* partials,
+                        hipStream_t stream) {
+  if (la.nchunks == 0) return PSC_OK;
+  const Dims d{nx, ny, nz};
+  if (imbp == 2) k_loads_chunk<double><<<la.nchunks, 256, 0, stream>>>(la, (const double*)imb, d, partials);
+  else k_loads_chunk<float><<<la.nchunks, 256, 0, stream>>>(la, (const float*)imb, d, partials);
+  return check_launch("psc_particle_loads(chunks)");
+}
+
+int launch_loads_final(const int32_t* chunk_first, int np, const double* partials, double* loads, hipStream_t stream) {
+  if (np == 0) return PSC_OK;
+  k_loads_final<<<np, 64, 0, stream>>>(chunk_first, partials, loads);
+  return check_launch("psc_particle_loads(final)");
 }
 
 int64_t moments_blocks(int64_t n) {

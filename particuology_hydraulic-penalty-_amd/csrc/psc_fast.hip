@@ -14,11 +14,29 @@
 //   imb = sum_i B Omega_s_i c_i
 // Because sum_i w_i = 1 and sum_i w_i c_i = 0, the w_i of f cancel in every
 // difference above and only the deviations are ever stored or summed.
+//
+// Momentum-exact FP32 coefficients.  The linear equilibrium term 3 w_i rho
+// (c_i.u) is formed as k_i (c_i.(j + F)) with k = kAxis on the 6 axis links
+// and kDiag on the 12 diagonals, chosen so that 2 kAxis + 8 kDiag = 1 exactly
+// in FP32 (kAxis = 0x3e2aaaaa ~ 1/6, kDiag = (1 - 2 kAxis)/8 ~ 1/12; both
+// are exact and the sum is exact).  With the FP32-rounded 3*w_i instead,
+// sum_i 3 w_i c_ix^2 = 6 fl(1/18) != 1 and every BGK relaxation would inject
+// a systematic ~1e-8 j of momentum per step; over thousands of steps that
+// biases the permeability by ~1e-5.  Forming the term from j + F rather than
+// rho * ((j + F)/rho) also removes the rho*(1/rho) rounding from it.  The
+// remaining (quadratic and mass) terms are even in c_i and carry no momentum
+// in any arithmetic, because a direction and its opposite compute them
+// identically.  The Guo term uses the same k_i for its linear part.
 #include "psc_common.h"
 #include "../../include/psc_hip.h"
 
 namespace psc {
 namespace {
+
+__device__ __forceinline__ float k_lin(int i) {
+  // 0x3e2aaaaa and (1 - 2*0x3e2aaaaa)/8, see the header comment
+  return i == 0 ? 0.0f : (i <= 6 ? 0.16666665673255920410f : 0.08333333581686019897f);
+}
 
 struct Dims {
   int nx, ny, nz;
@@ -28,33 +46,48 @@ __device__ __forceinline__ uint32_t cell_id(const Dims d, int x, int y, int z) {
   return ((uint32_t)x * (uint32_t)d.ny + (uint32_t)y) * (uint32_t)d.nz + (uint32_t)z;
 }
 
-template <bool PULL, bool MOVING, bool IMB>
-__global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin, float* __restrict__ gout,
-                                                  const uint32_t* __restrict__ csum,
-                                                  const float* __restrict__ usn, float* __restrict__ imb, Dims d,
-                                                  float inv_s3, float tau, float inv_tau, float pref, float Fx,
-                                                  float Fy, float Fz) {
-  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
-  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
-  const int ix = blockIdx.z;
-  if (iz >= d.nz || iy >= d.ny) return;
-  const size_t N = (size_t)d.nx * d.ny * d.nz;
+// Load / store of one FP32 value at byte offset `off` of a direction plane.
+// The plane base is uniform (held in SGPRs) and the offset is a 32-bit VGPR,
+// so the compiler emits the saddr form of global_load/store with no 64-bit
+// address arithmetic per access.  NT selects non-temporal (streaming) hints.
+template <bool NT>
+__device__ __forceinline__ float ld(const char* __restrict__ base, uint32_t off) {
+  const float* p = (const float*)(base + off);
+  return NT ? __builtin_nontemporal_load(p) : *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st(char* __restrict__ base, uint32_t off, float v) {
+  float* p = (float*)(base + off);
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+struct StepConsts {
+  float inv_s3, tau, itau, omt, pref, Fx, Fy, Fz;
+};
+
+template <bool PULL, bool MOVING, bool IMB, bool NT>
+__device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* __restrict__ gout,
+                                          const uint32_t* __restrict__ csum, const float* __restrict__ usn,
+                                          float* __restrict__ imb, const Dims d, const StepConsts& k, int ix, int iy,
+                                          int iz) {
+  const size_t plane = (size_t)d.nx * d.ny * d.nz * sizeof(float);  // bytes per direction plane
   const int xm = wrap_step(ix, -1, d.nx), xp = wrap_step(ix, 1, d.nx);
   const int ym = wrap_step(iy, -1, d.ny), yp = wrap_step(iy, 1, d.ny);
   const int zm = wrap_step(iz, -1, d.nz), zp = wrap_step(iz, 1, d.nz);
-  const uint32_t cell = cell_id(d, ix, iy, iz);
-
-  // neighbour coordinate for a displacement s in {-1,0,1}
   auto X = [&](int s) { return s < 0 ? xm : (s > 0 ? xp : ix); };
   auto Y = [&](int s) { return s < 0 ? ym : (s > 0 ? yp : iy); };
   auto Z = [&](int s) { return s < 0 ? zm : (s > 0 ? zp : iz); };
+  // byte offset of the cell displaced by (sx, sy, sz)
+  auto off = [&](int sx, int sy, int sz) -> uint32_t { return cell_id(d, X(sx), Y(sy), Z(sz)) << 2; };
+  const uint32_t here = cell_id(d, ix, iy, iz) << 2;
+  const char* gi = (const char*)gin;
+  char* go = (char*)gout;
 
   float g[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const uint32_t src = PULL ? cell_id(d, X(-cx_of(i)), Y(-cy_of(i)), Z(-cz_of(i))) : cell;
-    g[i] = gin[(size_t)i * N + src];
-  }
+  for (int i = 0; i < Q; ++i)
+    g[i] = ld<NT>(gi + (size_t)i * plane, PULL ? off(-cx_of(i), -cy_of(i), -cz_of(i)) : here);
 
   const float drho = ((g[0] + (g[1] + g[2])) + ((g[3] + g[4]) + (g[5] + g[6]))) +
                      (((g[7] + g[8]) + (g[9] + g[10])) + ((g[11] + g[12]) + (g[13] + g[14]))) +
@@ -63,102 +96,184 @@ __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin,
   const float jy = (g[3] - g[4]) + ((g[7] - g[8]) - (g[9] - g[10])) + ((g[15] - g[16]) + (g[17] - g[18]));
   const float jz = (g[5] - g[6]) + ((g[11] - g[12]) - (g[13] - g[14])) + ((g[15] - g[16]) - (g[17] - g[18]));
   const float rho = 1.0f + drho;
-  const float ir = 1.0f / rho;
-  const float ux = (jx + Fx) * ir;
-  const float uy = (jy + Fy) * ir;
-  const float uz = (jz + Fz) * ir;
+  // hardware reciprocal (1 ulp): 1/rho only scales the quadratic terms, the
+  // momentum-carrying linear terms are formed from j + F directly
+  const float ir = __builtin_amdgcn_rcpf(rho);
+  const float jFx = jx + k.Fx, jFy = jy + k.Fy, jFz = jz + k.Fz;  // rho * u_eff
+  const float ux = jFx * ir, uy = jFy * ir, uz = jFz * ir;
   const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
-  const float uF3 = 3.0f * (ux * Fx + uy * Fy + uz * Fz);
+  const float uF3 = 3.0f * (ux * k.Fx + uy * k.Fy + uz * k.Fz);
 
-  float geq[Q];
-  float guo[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const float w = (float)w_of(i);
-    const float cu = (float)cx_of(i) * ux + (float)cy_of(i) * uy + (float)cz_of(i) * uz;
-    const float cf = (float)cx_of(i) * Fx + (float)cy_of(i) * Fy + (float)cz_of(i) * Fz;
-    geq[i] = w * drho + (w * rho) * (cu * (3.0f + 4.5f * cu) - usq15);
-    guo[i] = (pref * w) * ((3.0f * cf - uF3) + 9.0f * cu * cf);
-  }
+  // per weight class: w*drho, w*rho, pref*w
+  auto wdr = [&](int i) { return (float)w_of(i) * drho; };
+  auto wrh = [&](int i) { return (float)w_of(i) * rho; };
+  auto pw = [&](int i) { return k.pref * (float)w_of(i); };
 
-  const uint32_t cnt = csum[cell];
+  const uint32_t cnt = csum[cell_id(d, ix, iy, iz)];
+  // destination offset of direction i
+  auto dst = [&](int i) -> uint32_t { return PULL ? here : off(cx_of(i), cy_of(i), cz_of(i)); };
+
+  // Post-collision values; stored by one store per direction after the
+  // branch, so every store keeps the uniform-base + 32-bit-offset form.
   float gn[Q];
   if (cnt == 0) {
+    // fluid cell: g' = (1 - 1/tau) g + geq/tau + Guo, pairwise even/odd parts
+    {
+      const float geq0 = wdr(0) - wrh(0) * usq15;
+      const float gu0 = -pw(0) * uF3;
+      gn[0] = fmaf(g[0], k.omt, fmaf(geq0, k.itau, gu0));
+    }
 #pragma unroll
-    for (int i = 0; i < Q; ++i) gn[i] = g[i] + (geq[i] - g[i]) * inv_tau + guo[i];
+    for (int i = 1; i < Q; i += 2) {
+      const int o = i + 1;
+      const float cu = (float)cx_of(i) * ux + (float)cy_of(i) * uy + (float)cz_of(i) * uz;
+      const float cj = (float)cx_of(i) * jFx + (float)cy_of(i) * jFy + (float)cz_of(i) * jFz;
+      const float cf = (float)cx_of(i) * k.Fx + (float)cy_of(i) * k.Fy + (float)cz_of(i) * k.Fz;
+      const float S = fmaf(wrh(i), fmaf(4.5f * cu, cu, -usq15), wdr(i));  // even part of geq
+      const float A = k_lin(i) * cj;                                      // odd part of geq
+      const float Gs = pw(i) * fmaf(9.0f * cu, cf, -uF3);                 // even part of Guo
+      const float Ga = k.pref * (k_lin(i) * cf);                          // odd part of Guo
+      const float P = fmaf(S, k.itau, Gs);
+      const float M = fmaf(A, k.itau, Ga);
+      gn[i] = fmaf(g[i], k.omt, P + M);
+      gn[o] = fmaf(g[o], k.omt, P - M);
+    }
   } else {
-    const float a = fminf((float)cnt * inv_s3, 1.0f);
-    const float B = a / ((a + tau * (1.0f - a)) + 1.0e-14f);
-    const float omb = (1.0f - B) * inv_tau;
-    float geqs[Q];
+    const float a = fminf((float)cnt * k.inv_s3, 1.0f);
+    const float B = a / ((a + k.tau * (1.0f - a)) + 1.0e-14f);
+    const float omb = (1.0f - B) * k.itau;
+    float sx = 0.f, sy = 0.f, sz = 0.f, ssq15 = 0.f;
     if (MOVING) {
-      const float asum = (float)cnt * inv_s3;
-      const float ia = 1.0f / asum;
-      const float sx = usn[cell] * ia, sy = usn[N + cell] * ia, sz = usn[2 * N + cell] * ia;
-      const float ssq15 = 1.5f * (sx * sx + sy * sy + sz * sz);
-#pragma unroll
-      for (int i = 0; i < Q; ++i) {
-        const float w = (float)w_of(i);
-        const float cu = (float)cx_of(i) * sx + (float)cy_of(i) * sy + (float)cz_of(i) * sz;
-        geqs[i] = w * drho + (w * rho) * (cu * (3.0f + 4.5f * cu) - ssq15);
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < Q; ++i) geqs[i] = (float)w_of(i) * drho;
+      const uint32_t c = cell_id(d, ix, iy, iz);
+      const size_t N = (size_t)d.nx * d.ny * d.nz;
+      const float ia = 1.0f / ((float)cnt * k.inv_s3);
+      sx = usn[c] * ia;
+      sy = usn[N + c] * ia;
+      sz = usn[2 * N + c] * ia;
+      ssq15 = 1.5f * (sx * sx + sy * sy + sz * sz);
     }
     float mx = 0.f, my = 0.f, mz = 0.f;
+    {
+      const float geq0 = wdr(0) - wrh(0) * usq15;
+      const float geqs0 = MOVING ? wdr(0) - wrh(0) * ssq15 : wdr(0);
+      const float gu0 = -pw(0) * uF3;
+      const float os0 = B * (geqs0 - geq0);
+      gn[0] = g[0] + (geq0 - g[0]) * omb + os0 + gu0;
+    }
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const int o = opp_of(i);
-      const float os = B * ((g[o] - g[i]) + (geqs[i] - geq[o]));
-      gn[i] = g[i] + (geq[i] - g[i]) * omb + os + guo[i];
-      mx += (float)cx_of(i) * os;
-      my += (float)cy_of(i) * os;
-      mz += (float)cz_of(i) * os;
+    for (int i = 1; i < Q; i += 2) {
+      const int o = i + 1;
+      const float cu = (float)cx_of(i) * ux + (float)cy_of(i) * uy + (float)cz_of(i) * uz;
+      const float cj = (float)cx_of(i) * jFx + (float)cy_of(i) * jFy + (float)cz_of(i) * jFz;
+      const float cf = (float)cx_of(i) * k.Fx + (float)cy_of(i) * k.Fy + (float)cz_of(i) * k.Fz;
+      const float S = fmaf(wrh(i), fmaf(4.5f * cu, cu, -usq15), wdr(i));
+      const float A = k_lin(i) * cj;
+      const float Gs = pw(i) * fmaf(9.0f * cu, cf, -uF3);
+      const float Ga = k.pref * (k_lin(i) * cf);
+      float Ss = wdr(i), As = 0.0f;  // solid equilibrium, even and odd parts
+      if (MOVING) {
+        const float cs = (float)cx_of(i) * sx + (float)cy_of(i) * sy + (float)cz_of(i) * sz;
+        Ss = fmaf(wrh(i), fmaf(4.5f * cs, cs, -ssq15), wdr(i));
+        As = k_lin(i) * (rho * cs);
+      }
+      const float geq_i = S + A, geq_o = S - A;
+      const float os_i = B * ((g[o] - g[i]) + ((Ss + As) - geq_o));
+      const float os_o = B * ((g[i] - g[o]) + ((Ss - As) - geq_i));
+      gn[i] = g[i] + (geq_i - g[i]) * omb + os_i + (Gs + Ga);
+      gn[o] = g[o] + (geq_o - g[o]) * omb + os_o + (Gs - Ga);
+      const float dm = os_i - os_o;  // c_i os_i + c_o os_o
+      if (cx_of(i)) mx += (float)cx_of(i) * dm;
+      if (cy_of(i)) my += (float)cy_of(i) * dm;
+      if (cz_of(i)) mz += (float)cz_of(i) * dm;
     }
     if (IMB) {
-      imb[cell] = mx;
-      imb[N + cell] = my;
-      imb[2 * N + cell] = mz;
+      const uint32_t c = cell_id(d, ix, iy, iz);
+      const size_t N = (size_t)d.nx * d.ny * d.nz;
+      imb[c] = mx;
+      imb[N + c] = my;
+      imb[2 * N + c] = mz;
     }
   }
-
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const uint32_t dst = PULL ? cell : cell_id(d, X(cx_of(i)), Y(cy_of(i)), Z(cz_of(i)));
-    gout[(size_t)i * N + dst] = gn[i];
+  for (int i = 0; i < Q; ++i) st<NT>(go + (size_t)i * plane, dst(i), gn[i]);
+}
+
+// Grid: (z tiles, y tiles, x).  A thread updates CPT cells at consecutive y.
+// The dynamic LDS allocation is unused storage that caps the number of
+// resident workgroups per CU (an occupancy limit chosen by the tuner); it
+// leaves kLdsHeadroom bytes free so the small load-reduction workgroups of
+// the previous step, running on another stream, can share the CU.
+template <bool PULL, bool MOVING, bool IMB, bool NT, int CPT>
+__global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin, float* __restrict__ gout,
+                                                  const uint32_t* __restrict__ csum,
+                                                  const float* __restrict__ usn, float* __restrict__ imb, Dims d,
+                                                  StepConsts k) {
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy0 = (blockIdx.y * blockDim.y + threadIdx.y) * CPT;
+  const int ix = blockIdx.z;
+  if (iz >= d.nz) return;
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int iy = iy0 + c;
+    if (iy < d.ny) step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, usn, imb, d, k, ix, iy, iz);
   }
 }
 
 }  // namespace
 
+constexpr size_t kLdsPerCU = 163840;
+constexpr size_t kLdsHeadroom = 8192;
+
 int launch_step_f32(const float* in, float* out, const uint32_t* csum, const float* usn, float* imb, int layout,
                     int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz, int bz, int by,
-                    hipStream_t stream) {
+                    int nt, hipStream_t stream) {
   const Dims d{nx, ny, nz};
+  const int cpt = (nt >> 16) & 3 ? (nt >> 16) & 3 : 1;
+  const int bpc = (nt >> 8) & 0xff;
+  size_t lds = 0;
+  if (bpc > 0) {
+    lds = (kLdsPerCU - kLdsHeadroom) / (size_t)bpc;
+    if (lds > 65536) lds = 65536;
+  }
   const dim3 block(bz, by, 1);
-  const dim3 grid((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
-  const float inv_s3 = (float)(1.0 / (double)(s * s * s));
-  const float ftau = (float)tau;
-  const float inv_tau = (float)(1.0 / tau);
-  const float pref = (float)(1.0 - 0.5 / tau);
+  const int rows = (ny + cpt - 1) / cpt;
+  const dim3 grid((nz + bz - 1) / bz, (rows + by - 1) / by, nx);
+  if (cpt != 1 && cpt != 2) return set_error(PSC_EINVAL, "psc_step(f32): cells per thread %d", cpt);
+  if ((size_t)nx * ny * nz * sizeof(float) > 0xffffffffull)
+    return set_error(PSC_EINVAL, "psc_step(f32): a direction plane must stay below 4 GiB (N < 2^30 cells)");
+  StepConsts k;
+  k.inv_s3 = (float)(1.0 / (double)(s * s * s));
+  k.tau = (float)tau;
+  k.itau = (float)(1.0 / tau);
+  k.omt = (float)(1.0 - 1.0 / tau);
+  k.pref = (float)(1.0 - 0.5 / tau);
+  k.Fx = (float)fx;
+  k.Fy = (float)fy;
+  k.Fz = (float)fz;
   const bool pull = layout == PSC_PULL;
   const bool moving = usn != nullptr;
   const bool wimb = imb != nullptr;
-#define PSC_F32_CASE(P, M, I)                                                                                   \
-  if (pull == P && moving == M && wimb == I) {                                                                 \
-    k_step_f32<P, M, I><<<grid, block, 0, stream>>>(in, out, csum, usn, imb, d, inv_s3, ftau, inv_tau, pref,     \
-                                                    (float)fx, (float)fy, (float)fz);                          \
-    return check_launch("psc_step(f32)");                                                                      \
+  const bool ntb = (nt & 1) != 0;
+#define PSC_F32_CASE(P, M, I, T)                                                                  \
+  if (pull == P && moving == M && wimb == I && ntb == T) {                                       \
+    if (cpt == 2)                                                                                \
+      k_step_f32<P, M, I, T, 2><<<grid, block, lds, stream>>>(in, out, csum, usn, imb, d, k);    \
+    else                                                                                         \
+      k_step_f32<P, M, I, T, 1><<<grid, block, lds, stream>>>(in, out, csum, usn, imb, d, k);    \
+    return check_launch("psc_step(f32)");                                                        \
   }
-  PSC_F32_CASE(false, false, false)
-  PSC_F32_CASE(false, false, true)
-  PSC_F32_CASE(false, true, false)
-  PSC_F32_CASE(false, true, true)
-  PSC_F32_CASE(true, false, false)
-  PSC_F32_CASE(true, false, true)
-  PSC_F32_CASE(true, true, false)
-  PSC_F32_CASE(true, true, true)
+#define PSC_F32_CASES(T)                 \
+  PSC_F32_CASE(false, false, false, T)   \
+  PSC_F32_CASE(false, false, true, T)    \
+  PSC_F32_CASE(false, true, false, T)    \
+  PSC_F32_CASE(false, true, true, T)     \
+  PSC_F32_CASE(true, false, false, T)    \
+  PSC_F32_CASE(true, false, true, T)     \
+  PSC_F32_CASE(true, true, false, T)     \
+  PSC_F32_CASE(true, true, true, T)
+  PSC_F32_CASES(false)
+  PSC_F32_CASES(true)
+#undef PSC_F32_CASES
 #undef PSC_F32_CASE
   return set_error(PSC_EINVAL, "psc_step(f32): no kernel variant");
 }

@@ -20,6 +20,7 @@ import numpy as np
 
 FIELDS = ("x", "y", "z", "r", "vx", "vy", "vz", "wx", "wy", "wz")
 MAX_ENTRIES = 2 ** 31 - 1
+MAX_BOX_EXTENT = 1024  # box-local coordinates are packed in 10 bits per axis
 
 
 def pack_particles(particles) -> np.ndarray:
@@ -66,11 +67,14 @@ def build_tables(table: np.ndarray, shape, periodic: bool = False, chunk_cells: 
         lo[:, a] = l0
         hi[:, a] = np.maximum(h0, l0)
     ext = hi - lo
+    if P and int(ext.max()) > MAX_BOX_EXTENT:
+        raise ValueError(f"sphere box extent {int(ext.max())} exceeds {MAX_BOX_EXTENT} cells")
     vols = ext[:, 0] * ext[:, 1] * ext[:, 2]
     offs = np.zeros(P + 1, dtype=np.int64)
     np.cumsum(vols, out=offs[1:])
     total = int(offs[-1])
-    if total > MAX_ENTRIES:
+    nch_total = int(((vols + chunk_cells - 1) // chunk_cells).sum()) * chunk_cells
+    if total > MAX_ENTRIES or nch_total > MAX_ENTRIES:
         raise ValueError(f"{total} box entries exceed the int32 entry space")
     boxes = np.zeros((P, 8), dtype=np.int32)
     boxes[:, 0:3] = lo

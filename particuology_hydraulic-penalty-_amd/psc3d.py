@@ -150,7 +150,7 @@ class PscD3Q19:
     def __init__(self, n, particles, tau: float = 0.72, subsamples: int = 4, body_force_z: float = 0.0,
                  static_geometry: bool = False, *, body_force=None, precision: str = "fp64",
                  stream_layout: str = "pull", periodic_geometry: bool = False, device=None,
-                 chunk_cells: int = 2048, block=(0, 0), diag: str = "lazy"):
+                 chunk_cells: int = 2048, block="auto", nontemporal: bool = False, diag: str = "lazy"):
         torch = _torch()
         self._lib = _lib.load()
         if not torch.cuda.is_available():
@@ -190,20 +190,30 @@ class PscD3Q19:
             raise ValueError("diag must be 'lazy' or 'eager'")
         self._diag_mode = diag
         self._chunk_cells = int(chunk_cells)
-        self._block = (int(block[0]), int(block[1]))
+        if self._chunk_cells < 64 or self._chunk_cells & (self._chunk_cells - 1):
+            raise ValueError("chunk_cells must be a power of two >= 64")
+        # launch configuration of psc_step: "auto" = measured by the autotuner
+        # at the first step (FP32 storage; FP64 uses the default shape)
+        self._autotune = isinstance(block, str) and block == "auto"
+        self._block = (0, 0) if self._autotune else (int(block[0]), int(block[1]))
+        self._flags = _lib.PSC_STEP_NONTEMPORAL if nontemporal else 0
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self._fdt = torch.float32 if self._storage == _lib.PSC_F32_DEV else torch.float64
         N = self._N
         self._a = torch.empty(19 * N, dtype=self._fdt, device=self.device)
         self._b = torch.empty(19 * N, dtype=self._fdt, device=self.device)
         self._csum = torch.zeros(N, dtype=torch.int32, device=self.device)
-        self._imb = torch.zeros(3 * N, dtype=self._fdt, device=self.device)
+        # imb of the last step; a second buffer is added for the pipelined loop
+        self._imb_bufs = [torch.zeros(3 * N, dtype=self._fdt, device=self.device), None]
+        self._imb = self._imb_bufs[0]
+        self._partials_alt = None
         self._us = None
-        self._counts = torch.zeros(1, dtype=torch.int16, device=self.device)
+        self._ent = None          # (ent_cell u32, ent_loc u32, ent_cnt u16) as int32/int32/int16 tensors
+        self._chunk_nnz = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._partials = torch.zeros(6, dtype=torch.float64, device=self.device)
         self._tables = None
         self._dev_tables = None
         self._loads = torch.zeros((0, 6), dtype=torch.float64, device=self.device)
-        self._partials = torch.zeros(6, dtype=torch.float64, device=self.device)
         nblk = int(self._lib.psc_moments_blocks(N))
         self._mom_partials = torch.zeros(nblk * _lib.PSC_NSUM, dtype=torch.float64, device=self.device)
         self._sums = torch.zeros(_lib.PSC_NSUM, dtype=torch.float64, device=self.device)
@@ -255,11 +265,17 @@ class PscD3Q19:
         boxes = torch.from_numpy(tabs.boxes).to(dev)
         chunks = torch.from_numpy(tabs.chunks).to(dev)
         cfirst = torch.from_numpy(tabs.chunk_first).to(dev)
-        if self._counts.numel() < max(1, tabs.n_entries):
-            self._counts = torch.empty(max(1, int(tabs.n_entries * 1.25)), dtype=torch.int16, device=dev)
-        if self._partials.numel() < 6 * max(1, tabs.chunks.shape[0]):
-            self._partials = torch.empty(6 * max(1, int(tabs.chunks.shape[0] * 1.25)), dtype=torch.float64,
-                                         device=dev)
+        C = int(tabs.chunks.shape[0])
+        cap = max(1, C * tabs.chunk_cells)
+        if self._ent is None or self._ent[0].numel() < cap:
+            cap = int(cap * 1.25) + 1
+            self._ent = (torch.empty(cap, dtype=torch.int32, device=dev),
+                         torch.empty(cap, dtype=torch.int32, device=dev),
+                         torch.empty(cap, dtype=torch.int16, device=dev))
+        if self._chunk_nnz.numel() < max(1, C):
+            self._chunk_nnz = torch.empty(max(1, int(C * 1.25)), dtype=torch.int32, device=dev)
+        if self._partials.numel() < 6 * max(1, C):
+            self._partials = torch.empty(6 * max(1, int(C * 1.25)), dtype=torch.float64, device=dev)
         P = table.shape[0]
         if self._loads.shape[0] != P:
             self._loads = torch.zeros((P, 6), dtype=torch.float64, device=dev)
@@ -274,9 +290,10 @@ class PscD3Q19:
             usp = 1 if self._storage == _lib.PSC_F32_DEV else 2
         us_ptr = self._ptr(self._us) if tabs.moving else None
         nx, ny, nz = self._shape
-        _lib.call("psc_voxelize", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), int(tabs.chunks.shape[0]),
-                  tabs.chunk_cells, nx, ny, nz, self._s, self._ptr(self._counts), self._ptr(self._csum), us_ptr,
-                  usp, stream)
+        ec, el, en = self._ent
+        _lib.call("psc_voxelize", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), C, tabs.chunk_cells,
+                  nx, ny, nz, self._s, self._ptr(ec), self._ptr(el), self._ptr(en), self._ptr(self._chunk_nnz),
+                  self._ptr(self._csum), us_ptr, usp, stream)
         self._tables = tabs
         # keep the device tables alive until the kernels that read them have run
         self._dev_tables = (parts, boxes, chunks, cfirst)
@@ -306,29 +323,46 @@ class PscD3Q19:
             for _ in range(steps):
                 self._advance()
             return
+        if self._autotune:
+            self._tune()
         import ctypes as ct
-        plan = self._plan()
+        pipelined = self._tables is not None and self._tables.particles.shape[0] > 0
+        if pipelined:
+            self._ensure_pipeline_buffers()
+        plan = self._plan(pipelined)
         parity = ct.c_int32(0)
         _lib.call("psc_run_steps", ct.byref(plan), steps, 1 if graph else 0, int(graph_steps), ct.byref(parity),
                   self._stream())
         if parity.value:
             self._a, self._b = self._b, self._a
+        # the last step read parity 1 - p and wrote that imb buffer
+        self._imb = self._imb_bufs[1 - parity.value] if pipelined else self._imb_bufs[0]
         self._step_count += steps
         self._cache.clear()
 
-    def _plan(self):
+    def _ensure_pipeline_buffers(self):
+        torch = _torch()
+        if self._imb_bufs[1] is None:
+            self._imb_bufs[1] = torch.zeros(3 * self._N, dtype=self._fdt, device=self.device)
+        need = self._partials.numel()
+        if self._partials_alt is None or self._partials_alt.numel() < need:
+            self._partials_alt = torch.empty(need, dtype=torch.float64, device=self.device)
+
+    def _plan(self, pipelined: bool = False):
         """PscStepPlan of the current buffers (state in self._a)."""
         p = _lib.PscStepPlan()
         nx, ny, nz = self._shape
         p.state_a, p.state_b = self._ptr(self._a), self._ptr(self._b)
         p.count_sum = self._ptr(self._csum)
         p.us_num = self._ptr(self._us) if self._moving else None
-        p.imb = self._ptr(self._imb)
+        p.imb = self._ptr(self._imb_bufs[0])
+        p.imb_alt = self._ptr(self._imb_bufs[1]) if pipelined else None
         p.storage, p.layout = self._storage, self._layout
         p.nx, p.ny, p.nz, p.subsamples = nx, ny, nz, self._s
         p.tau = self.tau
         p.fx, p.fy, p.fz = self._force
         p.block_z, p.block_y = self._block
+        p.flags = self._flags
         tabs = self._tables
         if tabs is not None and tabs.particles.shape[0] > 0:
             parts, boxes, chunks, cfirst = self._dev_tables
@@ -336,19 +370,35 @@ class PscD3Q19:
                                                              self._ptr(cfirst))
             p.nparticles, p.nchunks, p.chunk_cells = (int(tabs.particles.shape[0]), int(tabs.chunks.shape[0]),
                                                       tabs.chunk_cells)
-            p.counts, p.partials, p.loads = self._ptr(self._counts), self._ptr(self._partials), self._ptr(self._loads)
+            ec, el, en = self._ent
+            p.ent_cell, p.ent_loc, p.ent_cnt = self._ptr(ec), self._ptr(el), self._ptr(en)
+            p.chunk_nnz, p.partials = self._ptr(self._chunk_nnz), self._ptr(self._partials)
+            p.partials_alt = self._ptr(self._partials_alt) if pipelined else None
+            p.loads = self._ptr(self._loads)
         return p
+
+    def _tune(self):
+        """Pick the psc_step launch configuration once (autotune.py)."""
+        if self._autotune and self._storage == _lib.PSC_F32_DEV:
+            from . import autotune
+            bz, by, flags = autotune.tune(self)
+            self._block = (bz, by)
+            self._flags = flags
+        self._autotune = False
 
     def _advance(self):
         if (not self.static_geometry) or (not self._geometry_built):
             self.rebuild_geometry()
+        if self._autotune:
+            self._tune()
         nx, ny, nz = self._shape
         stream = self._stream()
         fx, fy, fz = self._force
         us_ptr = self._ptr(self._us) if self._moving else None
+        self._imb = self._imb_bufs[0]
         _lib.call("psc_step", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum), us_ptr,
                   self._ptr(self._imb), self._storage, self._layout, nx, ny, nz, self._s, self.tau, fx, fy, fz,
-                  self._block[0], self._block[1], stream)
+                  self._block[0], self._block[1], self._flags, stream)
         self._a, self._b = self._b, self._a
         self._step_count += 1
         self._cache.clear()
@@ -362,10 +412,12 @@ class PscD3Q19:
         parts, boxes, chunks, cfirst = self._dev_tables
         nx, ny, nz = self._shape
         imbp = 1 if self._storage == _lib.PSC_F32_DEV else 2
+        ec, el, en = self._ent
         _lib.call("psc_particle_loads", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), self._ptr(cfirst),
-                  int(tabs.particles.shape[0]), int(tabs.chunks.shape[0]), tabs.chunk_cells,
-                  self._ptr(self._counts), self._ptr(self._csum), self._ptr(self._imb), imbp, nx, ny, nz, self._s,
-                  self._ptr(self._partials), self._ptr(self._loads), self._stream())
+                  int(tabs.particles.shape[0]), int(tabs.chunks.shape[0]), tabs.chunk_cells, self._ptr(ec),
+                  self._ptr(el), self._ptr(en), self._ptr(self._chunk_nnz), self._ptr(self._csum),
+                  self._ptr(self._imb), imbp, nx, ny, nz, self._ptr(self._partials), self._ptr(self._loads),
+                  self._stream())
         self._cache.pop("hydro_forces", None)
         self._cache.pop("hydro_torques", None)
         self._cache.pop("loads", None)
@@ -598,20 +650,23 @@ class PscD3Q19:
         return out
 
     def _sparse_lists(self):
+        """Per-particle (ids, alpha) of cells with alpha > 0, box order
+        (the reference's node_ids / alpha_vals, psc3d.py:339-343)."""
         if "lists" not in self._cache:
             tabs = self._tables
-            counts = self._np(self._counts)[: max(tabs.n_entries, 0)].view(np.uint16).astype(np.int64)
-            nx, ny, nz = self._shape
+            C = int(tabs.chunks.shape[0])
+            CK = tabs.chunk_cells
+            nnz = self._np(self._chunk_nnz)[:C].astype(np.int64)
+            cells = self._np(self._ent[0][: max(1, C * CK)]).view(np.uint32)
+            cnts = self._np(self._ent[2][: max(1, C * CK)]).view(np.uint16)
             ids, vals = [], []
+            s3 = float(self._s ** 3)
             for p in range(tabs.particles.shape[0]):
-                x0, y0, z0, ex, ey, ez, off = (int(v) for v in tabs.boxes[p, :7])
-                c = counts[off: off + ex * ey * ez].reshape(ex, ey, ez)
-                sel = np.nonzero(c)
-                gx = (sel[0] + x0) % nx
-                gy = (sel[1] + y0) % ny
-                gz = (sel[2] + z0) % nz
-                ids.append(((gx * ny + gy) * nz + gz).astype(np.int64))
-                vals.append(c[sel].astype(np.float64) / float(self._s ** 3))
+                c0, c1 = int(tabs.chunk_first[p]), int(tabs.chunk_first[p + 1])
+                sel = [np.arange(c * CK, c * CK + nnz[c]) for c in range(c0, c1)]
+                idx = np.concatenate(sel) if sel else np.zeros(0, dtype=np.int64)
+                ids.append(cells[idx].astype(np.int64))
+                vals.append(cnts[idx].astype(np.float64) / s3)
             self._cache["lists"] = (ids, vals)
         return self._cache["lists"]
 

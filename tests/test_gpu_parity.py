@@ -12,6 +12,7 @@ import pytest
 
 from oracle import psc_oracle as O
 from tests import _golden as G
+from tests.conftest import load_package
 
 pytestmark = pytest.mark.gpu
 
@@ -135,14 +136,17 @@ def test_voxelizer_counts_random(psc3d):
         assert np.array_equal(got, ref_cnt)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision):
+@pytest.mark.parametrize("precision,block", [("fp64", (64, 4)), ("fp32", (64, 4)), ("fp32", (64, 2)),
+                                             ("fp32", (64, 1))])
+def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, block):
     """psc_run_steps (direct launches and hipGraph replays, with eager reads in
-    between the replays) gives the same state as stepping one call at a time."""
+    between the replays; pipelined load reduction for FP32 with 64/128/256-
+    thread workgroups) gives the same state and loads as stepping one call at
+    a time."""
     g = G.load("perm24")
-    a, _ = _solver(psc3d, g, precision)
-    b, _ = _solver(psc3d, g, precision)
-    c, _ = _solver(psc3d, g, precision)
+    a, _ = _solver(psc3d, g, precision, block=block)
+    b, _ = _solver(psc3d, g, precision, block=block)
+    c, _ = _solver(psc3d, g, precision, block=block)
     for chunk in (16, 16, 5, 32):
         for _ in range(chunk):
             a.collide_stream_psc()
@@ -152,5 +156,30 @@ def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision):
         assert np.array_equal(fa, b.f)
         assert np.array_equal(fa, c.f)
         assert np.array_equal(a.hydro_forces, b.hydro_forces)
+        assert np.array_equal(a.hydro_torques, b.hydro_torques)
+        assert np.array_equal(a.hydro_forces, c.hydro_forces)
         assert np.array_equal(a.hydro_torques, c.hydro_torques)
+        assert np.array_equal(np.stack([a.imb_mx, a.imb_my, a.imb_mz]), np.stack([b.imb_mx, b.imb_my, b.imb_mz]))
     assert a._step_count == b._step_count == 69
+
+
+def test_every_launch_configuration_gives_identical_results(psc3d):
+    """The autotuner may pick any candidate: all must compute the same bits."""
+    import torch
+    at = load_package("autotune")
+    lib = load_package("_lib")
+    g = G.load("perm24")
+    s, _ = _solver(psc3d, g, "fp32", block=(64, 4))
+    s.run(7)
+    ref = None
+    nx, ny, nz = s._shape
+    h = torch.cuda.current_stream().cuda_stream
+    for bz, by, nt, bpc in at.candidates(nz) + [(32, 8, 1, 3), (16, 16, 0, 0)]:
+        for cpt in (1, 2):
+            lib.call("psc_step", s._a.data_ptr(), s._b.data_ptr(), s._csum.data_ptr(), None, s._imb.data_ptr(),
+                     1, 1, nx, ny, nz, s._s, s.tau, 0.0, 0.0, float(g["force"][2]), bz, by,
+                     at.flags_of(nt, bpc, cpt), h)
+            out = s._b.cpu().numpy().copy()
+            if ref is None:
+                ref = out
+            assert np.array_equal(out, ref), (bz, by, nt, bpc, cpt)

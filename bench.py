@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--layout", default="pull", choices=["pull", "push"])
+    ap.add_argument("--launch", default=None,
+                    help="fixed psc_step launch 'block_z,block_y,flags' instead of autotuning (for profiling)")
+    ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -60,9 +63,15 @@ def log(msg, quiet=False):
         print(msg, file=sys.stderr, flush=True)
 
 
-def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch):
+def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=None):
+    kw = {}
+    if launch is not None:
+        bz, by, flags = launch
+        kw = dict(block=(bz, by))
     sol = psc3d.PscD3Q19(n, table, tau=0.8, subsamples=4, body_force=(1.0e-6, 0.0, 0.0), static_geometry=True,
-                         precision=precision, stream_layout=layout, device="cuda:0")
+                         precision=precision, stream_layout=layout, device="cuda:0", **kw)
+    if launch is not None:
+        sol._flags = launch[2]
     sol.rebuild_geometry()
     sol._tune()   # launch-shape autotuning happens before the warm-up
     sol.run(warmup)
@@ -180,7 +189,9 @@ def main():
     N = n ** 3
 
     log(f"fp32 run: {args.warmup} warmup + {args.steps} timed steps at {n}^3", args.quiet)
-    sol32, k32, wall32, gpu_ms32 = timed_run(psc3d, table, n, "fp32", args.layout, args.steps, args.warmup, torch)
+    launch = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
+    sol32, k32, wall32, gpu_ms32 = timed_run(psc3d, table, n, "fp32", args.layout, args.steps, args.warmup, torch,
+                                             launch)
     porosity = 1.0 - float(sol32.solid.mean())
     kern_ms = step_kernel_ms(sol32, args.kernel_repeats, torch, lib)
     mlups = N * args.steps / wall32 / 1e6
@@ -197,8 +208,10 @@ def main():
                 "step_kernel_GBps": BYTES_F64_STATIC * N / (kern64 * 1e-3) / 1e9}
         log(f"fp64: {mlups64:.0f} MLUPS, step kernel {kern64*1e3:.1f} us, k={k64:.9e}", args.quiet)
         del sol64
-    copy_gbps = copy_bandwidth(torch, lib)
-    log(f"copy kernel: {copy_gbps:.0f} GB/s", args.quiet)
+    copy_gbps = None
+    if not args.no_copy:
+        copy_gbps = copy_bandwidth(torch, lib)
+        log(f"copy kernel: {copy_gbps:.0f} GB/s", args.quiet)
 
     cpu = None
     if not args.no_cpu_baseline:

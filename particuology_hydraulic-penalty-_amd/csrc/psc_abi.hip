@@ -68,9 +68,20 @@ __global__ void k_clear1(unsigned char* __restrict__ p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 
-__global__ void k_copy4(const float4* __restrict__ s, float4* __restrict__ d, size_t n4) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
-    d[i] = s[i];
+// Streaming copy: 4 independent 16-byte non-temporal loads in flight per
+// thread, then the 4 stores; grid-stride over 64-byte groups.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy4(const f32x4* __restrict__ s, f32x4* __restrict__ d, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    f32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(s + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(v[u], d + i + u * stride);
+  }
+  for (; i < n4; i += stride) d[i] = s[i];
 }
 
 __global__ void k_copy1(const float* __restrict__ s, float* __restrict__ d, size_t n) {
@@ -246,7 +257,7 @@ int psc_copy_f32(const float* d_src, float* d_dst, int64_t n, void* stream) {
   if (n == 0) return PSC_OK;
   hipStream_t s = (hipStream_t)stream;
   if ((((uintptr_t)d_src | (uintptr_t)d_dst) & 15u) == 0 && (n % 4) == 0)
-    k_copy4<<<grid_cap((size_t)n / 4, 256), 256, 0, s>>>((const float4*)d_src, (float4*)d_dst, (size_t)n / 4);
+    k_copy4<<<grid_cap((size_t)n / 16, 256), 256, 0, s>>>((const f32x4*)d_src, (f32x4*)d_dst, (size_t)n / 4);
   else
     k_copy1<<<grid_cap((size_t)n, 256), 256, 0, s>>>(d_src, d_dst, (size_t)n);
   return check_launch("psc_copy_f32");

@@ -99,6 +99,14 @@ int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_
                  int32_t* d_chunk_nnz, uint32_t* d_count_sum, void* d_us_num, int32_t us_precision,
                  void* stream);
 
+/* Incremental re-voxelization support: zero d_count_sum (and d_us_num, when
+ * not NULL) at every cell listed in the entries of a previous psc_voxelize
+ * call (same tables).  Replaces a full-lattice clear when moving spheres are
+ * re-voxelized every step (psc3d.py:209-210 with static_geometry=False). */
+int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, int32_t nchunks,
+                      int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, uint32_t* d_count_sum,
+                      void* d_us_num, int32_t us_precision, void* stream);
+
 /* K2 -- fused PSC collision + streaming, one lattice step.
  * Replaces PscD3Q19.collide_stream_psc (psc3d.py:208-246) with equilibrium
  * (psc3d.py:97-103) and guo_force (psc3d.py:106-114) inlined.  The solid

@@ -32,6 +32,7 @@ SIGNATURES = {
     "psc_clear": ([_P, _I64, _P], ct.c_int),
     "psc_init_state": ([_P, _I32, _I64, _P], ct.c_int),
     "psc_voxelize": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, _P], ct.c_int),
+    "psc_clear_entries": ([_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _I32, _P], ct.c_int),
     "psc_step": ([_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _I32, _P],
                  ct.c_int),
     "psc_particle_loads": ([_P, _P, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P,

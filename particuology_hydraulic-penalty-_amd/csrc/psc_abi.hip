@@ -49,6 +49,7 @@ int launch_loads(const double*, const int32_t*, const int32_t*, const int32_t*, 
 int launch_moments(const void*, int, int, const uint32_t*, const void*, int, int, int, int, double, double, double,
                    double*, double*, double*, hipStream_t);
 int64_t moments_blocks(int64_t);
+int launch_clear_entries(const uint32_t*, const int32_t*, int, int, int, int, int, uint32_t*, void*, int, hipStream_t);
 int launch_loads_chunks(const LoadsArgs&, const void*, int, int, int, int, double*, hipStream_t);
 int launch_loads_final(const int32_t*, int, const double*, double*, hipStream_t);
 
@@ -165,6 +166,19 @@ int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_
   return launch_voxelize(d_particles, d_boxes, d_chunks, nchunks, chunk_cells, nx, ny, nz, subsamples, d_ent_cell,
                          d_ent_loc, d_ent_cnt, d_chunk_nnz, d_count_sum, d_us_num, d_us_num ? us_precision : 0,
                          (hipStream_t)stream);
+}
+
+int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, int32_t nchunks, int32_t chunk_cells,
+                      int32_t nx, int32_t ny, int32_t nz, uint32_t* d_count_sum, void* d_us_num,
+                      int32_t us_precision, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_clear_entries: bad lattice");
+  if (nchunks < 0 || chunk_cells < 1) return set_error(PSC_EINVAL, "psc_clear_entries: sizes");
+  if (nchunks > 0 && (d_ent_cell == nullptr || d_chunk_nnz == nullptr || d_count_sum == nullptr))
+    return set_error(PSC_EINVAL, "psc_clear_entries: null pointer");
+  if (d_us_num != nullptr && us_precision != 1 && us_precision != 2)
+    return set_error(PSC_EINVAL, "psc_clear_entries: us_precision %d", us_precision);
+  return launch_clear_entries(d_ent_cell, d_chunk_nnz, nchunks, chunk_cells, nx, ny, nz, d_count_sum, d_us_num,
+                              d_us_num ? us_precision : 0, (hipStream_t)stream);
 }
 
 int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const void* d_us_num, void* d_imb,

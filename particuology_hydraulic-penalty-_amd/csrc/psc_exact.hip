@@ -390,6 +390,34 @@ __global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ par
   if (threadIdx.x == 0) chunk_nnz[blockIdx.x] = written;
 }
 
+// Zero count_sum (and the solid-velocity numerator) at the cells listed in
+// a previous voxelization's entries: the incremental alternative to clearing
+// the whole lattice before re-voxelizing moving spheres.
+template <int USP>
+__global__ __launch_bounds__(256) void k_clear_entries(const uint32_t* __restrict__ ent_cell,
+                                                       const int32_t* __restrict__ chunk_nnz, int chunk_cells,
+                                                       size_t N, uint32_t* __restrict__ csum,
+                                                       void* __restrict__ usn) {
+  const int c = blockIdx.x;
+  const int nnz = chunk_nnz[c];
+  const size_t out0 = (size_t)c * chunk_cells;
+  for (int k = threadIdx.x; k < nnz; k += blockDim.x) {
+    const uint32_t cell = ent_cell[out0 + k];
+    csum[cell] = 0u;
+    if (USP == 1) {
+      float* u = (float*)usn;
+      u[cell] = 0.0f;
+      u[N + cell] = 0.0f;
+      u[2 * N + cell] = 0.0f;
+    } else if (USP == 2) {
+      double* u = (double*)usn;
+      u[cell] = 0.0;
+      u[N + cell] = 0.0;
+      u[2 * N + cell] = 0.0;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // K3: node-summed force/torque per particle (psc3d.py:255-281, 375-407).
 // Pass 1: one workgroup per chunk sums its compacted entries in FP64.  Pass
@@ -651,6 +679,19 @@ int launch_loads_final(const int32_t* chunk_first, int np, const double* partial
   if (np == 0) return PSC_OK;
   k_loads_final<<<np, 64, 0, stream>>>(chunk_first, partials, loads);
   return check_launch("psc_particle_loads(final)");
+}
+
+int launch_clear_entries(const uint32_t* ent_cell, const int32_t* chunk_nnz, int nchunks, int chunk_cells, int nx,
+                         int ny, int nz, uint32_t* csum, void* usn, int usp, hipStream_t stream) {
+  if (nchunks == 0) return PSC_OK;
+  const size_t N = (size_t)nx * ny * nz;
+  if (usn == nullptr || usp == 0)
+    k_clear_entries<0><<<nchunks, 256, 0, stream>>>(ent_cell, chunk_nnz, chunk_cells, N, csum, nullptr);
+  else if (usp == 1)
+    k_clear_entries<1><<<nchunks, 256, 0, stream>>>(ent_cell, chunk_nnz, chunk_cells, N, csum, usn);
+  else
+    k_clear_entries<2><<<nchunks, 256, 0, stream>>>(ent_cell, chunk_nnz, chunk_cells, N, csum, usn);
+  return check_launch("psc_clear_entries");
 }
 
 int64_t moments_blocks(int64_t n) {

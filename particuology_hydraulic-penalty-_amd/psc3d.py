@@ -209,6 +209,8 @@ class PscD3Q19:
         self._partials_alt = None
         self._us = None
         self._ent = None          # (ent_cell u32, ent_loc u32, ent_cnt u16) as int32/int32/int16 tensors
+        self._old_ent_cell = None
+        self._us_clean = False    # count_sum/us_num are zero outside the current entries
         self._chunk_nnz = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._partials = torch.zeros(6, dtype=torch.float64, device=self.device)
         self._tables = None
@@ -267,6 +269,8 @@ class PscD3Q19:
         cfirst = torch.from_numpy(tabs.chunk_first).to(dev)
         C = int(tabs.chunks.shape[0])
         cap = max(1, C * tabs.chunk_cells)
+        self._old_ent_cell = self._ent[0] if self._ent is not None else None
+        old_nnz = self._chunk_nnz
         if self._ent is None or self._ent[0].numel() < cap:
             cap = int(cap * 1.25) + 1
             self._ent = (torch.empty(cap, dtype=torch.int32, device=dev),
@@ -274,6 +278,8 @@ class PscD3Q19:
                          torch.empty(cap, dtype=torch.int16, device=dev))
         if self._chunk_nnz.numel() < max(1, C):
             self._chunk_nnz = torch.empty(max(1, int(C * 1.25)), dtype=torch.int32, device=dev)
+            # the clear below still needs the previous counts
+            self._chunk_nnz[: old_nnz.numel()].copy_(old_nnz)
         if self._partials.numel() < 6 * max(1, C):
             self._partials = torch.empty(6 * max(1, int(C * 1.25)), dtype=torch.float64, device=dev)
         P = table.shape[0]
@@ -281,13 +287,22 @@ class PscD3Q19:
             self._loads = torch.zeros((P, 6), dtype=torch.float64, device=dev)
         stream = self._stream()
         N = self._N
-        _lib.call("psc_clear", self._ptr(self._csum), 4 * N, stream)
-        usp = 0
-        if tabs.moving:
-            if self._us is None:
-                self._us = torch.empty(3 * N, dtype=self._fdt, device=dev)
+        usp = (1 if self._storage == _lib.PSC_F32_DEV else 2) if tabs.moving else 0
+        if tabs.moving and self._us is None:
+            self._us = torch.empty(3 * N, dtype=self._fdt, device=dev)
             _lib.call("psc_clear", self._ptr(self._us), 3 * N * self._us.element_size(), stream)
-            usp = 1 if self._storage == _lib.PSC_F32_DEV else 2
+        if self._tables is not None and self._ent is not None and self._us_clean:
+            # zero only the cells the previous geometry touched
+            old = self._tables
+            oc = self._old_ent_cell
+            _lib.call("psc_clear_entries", self._ptr(oc), self._ptr(self._chunk_nnz), int(old.chunks.shape[0]),
+                      old.chunk_cells, *self._shape, self._ptr(self._csum),
+                      self._ptr(self._us) if self._us is not None else None,
+                      (1 if self._storage == _lib.PSC_F32_DEV else 2) if self._us is not None else 0, stream)
+        else:
+            _lib.call("psc_clear", self._ptr(self._csum), 4 * N, stream)
+            if self._us is not None:
+                _lib.call("psc_clear", self._ptr(self._us), 3 * N * self._us.element_size(), stream)
         us_ptr = self._ptr(self._us) if tabs.moving else None
         nx, ny, nz = self._shape
         ec, el, en = self._ent
@@ -299,6 +314,8 @@ class PscD3Q19:
         self._dev_tables = (parts, boxes, chunks, cfirst)
         self._moving = tabs.moving
         self._geometry_built = True
+        self._us_clean = True
+        self._old_ent_cell = None
         self._cache.clear()
 
     # ------------------------------------------------------------------- step

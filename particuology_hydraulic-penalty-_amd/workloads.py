@@ -60,3 +60,27 @@ def rsa_spheres(n: int, count: int, radius: float, seed: int = 0, margin: float 
 def fixed_array(seed: int = 0, n: int = 128, count: int = 60, radius: float = 12.0):
     pts = rsa_spheres(n, count, radius, seed=seed)
     return _table(pts, np.full(count, radius))
+
+
+def moving_window(seed: int = 0, n: int = 512, count: int = 2000, rmin: float = 6.0, rmax: float = 18.0,
+                  umax: float = 1.0e-3):
+    """Large moving-particle window: ``count`` spheres with r ~ U[rmin, rmax]
+    and centres ~ U[0, n)^3 (overlaps allowed), translational speed
+    ~ U[0, umax] in a random direction and angular speed with |omega| r ~
+    U[0, umax] about a random axis.  Used with periodic geometry; callers
+    advance the centres by U every step (dt = 1)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, float(n), size=(count, 3))
+    r = rng.uniform(rmin, rmax, size=count)
+    d = rng.normal(size=(count, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    e = rng.normal(size=(count, 3))
+    e /= np.linalg.norm(e, axis=1, keepdims=True)
+    U = d * rng.uniform(0.0, umax, size=count)[:, None]
+    Om = e * (rng.uniform(0.0, umax, size=count) / r)[:, None]
+    return _table(c, r, vel=U, omega=Om)
+
+
+def advance_centres(table: np.ndarray, n: int, dt: float = 1.0) -> None:
+    """x += U dt for every sphere, wrapped into [0, n) (periodic window)."""
+    table[:, 0:3] = np.mod(table[:, 0:3] + dt * table[:, 4:7], float(n))

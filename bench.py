@@ -54,8 +54,9 @@ def parse():
     ap.add_argument("--launch", default=None,
                     help="fixed psc_step launch 'block_z,block_y,flags' instead of autotuning (for profiling)")
     ap.add_argument("--no-copy", action="store_true")
-    ap.add_argument("--workload", default="fixed_array", choices=["fixed_array", "moving"],
-                    help="fixed_array: 128^3 permeability (default); moving: 512^3 window, 2000 moving spheres")
+    ap.add_argument("--workload", default="fixed_array", choices=["fixed_array", "moving", "windows", "adjoint"],
+                    help="fixed_array: 128^3 permeability (default); moving: 512^3 window, 2000 moving spheres; "
+                         "windows: 64 batched pore windows; adjoint: 96^3 permeability gradient")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -240,6 +241,158 @@ def bench_moving(args, psc3d, workloads, lib, torch, np):
     print(json.dumps(out))
 
 
+def bench_windows(args, workloads, torch, np):
+    """Batched pore windows: 64 periodic windows, edges ~ U{48..160}, pebbles
+    r=16 near solid fraction 0.6 plus 0-400 throat fines (skeleton / passive /
+    preloaded), tau 0.8, F = 1e-6 along z, s=3, FP32, 3000 steps; one
+    persistent launch per step (+ loads), replayed through a hipGraph."""
+    batch = importlib.import_module(PKG + ".batch")
+    steps = args.steps if args.steps != 4900 else 3000
+    warm = min(args.warmup, 20)
+    log("generating 64 windows", args.quiet)
+    wins = workloads.pebble_windows(seed=0)
+    B = batch.BatchedPscD3Q19([(L, t) for L, t, _ in wins], tau=0.8, subsamples=3, body_force=(0.0, 0.0, 1.0e-6),
+                              device="cuda:0")
+    B.run(warm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.run(steps)
+    k = B.permeability()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # batched step kernel alone, HIP events on its stream
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(stream)
+    for _ in range(reps):
+        B._launch_step(B._a, B._b, B._flags, B._grid, stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    kern = e0.elapsed_time(e1) / reps
+    cells = int(B.nlive.sum())  # live cells only: row and plane padding is neither read nor written
+    achieved = BYTES_F32_STATIC * cells / (kern * 1e-3) / 1e9
+    por = B.porosity()
+    variants = [v for _, _, v in wins]
+    out = {
+        "metric": METRIC + " (batched-window configuration)",
+        "value": cells * steps / wall / 1e6, "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "64 periodic pore windows, L~U{48..160} (seed 0), pebbles r=16 near solid fraction "
+                               "0.6 + 0-400 throat fines (skeleton/passive/preloaded), tau 0.8, F=1e-6 z, s=3, "
+                               "FP32; one persistent launch + loads per step, hipGraph replay",
+                   "windows": len(wins), "total_cells": cells, "particles": int(B.P)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_step_f32_batched",
+                     "bytes_per_cell": BYTES_F32_STATIC, "kernel_us": kern * 1e3,
+                     "launch": {"flags": B._flags, "grid": B._grid}},
+        "permeability": {v: [float(x) for x, vv in zip(k, variants) if vv == v][:4] for v in ("skeleton", "passive",
+                                                                                            "preloaded")},
+        "porosity_range": [float(por.min()), float(por.max())],
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def bench_adjoint(args, workloads, torch, np):
+    """Permeability gradient: periodic 96^3 window, 40 pebbles r=14 + 300
+    fines r in [2, 4] (lognormal), smooth solid fraction, tau 0.8, F = 1e-6
+    along z, T = 2000 steps from rest, sqrt(T) checkpointing.  Times one FP32
+    gradient evaluation (forward + reverse sweep) after a warm-up one, the
+    reverse-step kernel alone, and checks the FP32 gradient against the FP64
+    adjoint and FP64 central differences (h = 1e-3 cells) on the four largest
+    components."""
+    adj = importlib.import_module(PKG + ".adjoint")
+    lib = importlib.import_module(PKG + "._lib")
+    T = args.steps if args.steps != 4900 else 2000
+    n = 96
+    F = (0.0, 0.0, 1.0e-6)
+    tab = workloads.adjoint_window(seed=0)
+    N = n ** 3
+    p32 = adj.PermeabilityAdjoint(n, tau=0.8, body_force=F, steps=T, precision="fp32", device="cuda:0")
+    log(f"adjoint: warm-up gradient ({T} steps, K={p32.K})", args.quiet)
+    p32.value_and_grad(tab)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    k32 = p32.forward(tab)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g32 = p32.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd, bwd = t1 - t0, t2 - t1
+    log(f"fp32: k={k32:.9e} forward {fwd*1e3:.1f} ms, reverse sweep {bwd*1e3:.1f} ms", args.quiet)
+    alpha = p32.geometry.alpha32
+    trans = float(((alpha > 0) & (alpha < 1)).float().mean())
+    # kernels alone, HIP events on the stream they run on
+    stream = torch.cuda.current_stream()
+    h = stream.cuda_stream
+    reps = 100
+
+    def timed(fn):
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    adj_ms = timed(lambda: p32._reverse(p32._seg[0], p32._lam[0], p32._lam[1], h))
+    fwd_ms = timed(lambda: p32._step(p32._seg[0], p32._seg[1], h))
+    bytes_adj = 3 * 76 + 4 + 16 * trans   # state gather, adjoint in/out, alpha, dk/dalpha update
+    ach = bytes_adj * N / (adj_ms * 1e-3) / 1e9
+    log(f"k_adjoint_step {adj_ms*1e3:.1f} us ({ach:.0f} GB/s), field step {fwd_ms*1e3:.1f} us", args.quiet)
+
+    log("fp64 reference gradient", args.quiet)
+    p64 = adj.PermeabilityAdjoint(n, tau=0.8, body_force=F, steps=T, precision="fp64", device="cuda:0")
+    t3 = time.perf_counter()
+    k64, g64 = p64.value_and_grad(tab)
+    t4 = time.perf_counter()
+    scale = float(np.max(np.abs(g64)))
+    fd_rows = []
+    for flat in np.argsort(-np.abs(g64).ravel())[:4]:
+        pi, j = divmod(int(flat), 4)
+        hh = 1.0e-3
+        tp, tm = tab.copy(), tab.copy()
+        tp[pi, j] += hh
+        tm[pi, j] -= hh
+        fd = (p64.forward(tp) - p64.forward(tm)) / (2 * hh)
+        fd_rows.append({"particle": pi, "param": "xyzr"[j], "fd_fp64": fd, "adjoint_fp64": float(g64[pi, j]),
+                        "adjoint_fp32": float(g32[pi, j]),
+                        "rel_err_fp32_vs_fd": abs(float(g32[pi, j]) - fd) / abs(fd),
+                        "rel_err_fp64_vs_fd": abs(float(g64[pi, j]) - fd) / abs(fd)})
+    out = {
+        "metric": METRIC + " (adjoint-gradient configuration)",
+        "value": N * T / (fwd + bwd) / 1e6, "unit": "MLUPS", "n_gpus": 1, "steps": T, "warmup": 1,
+        "ms_per_step": (fwd + bwd) * 1e3 / T, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "permeability gradient: periodic 96^3 window, 40 pebbles r=14 + 300 passive throat "
+                               "fines r~lognormal(3, 0.25) clipped to [2,4] (seed 0), smooth solid fraction, tau 0.8, "
+                               f"F=1e-6 z, {T} steps from rest, sqrt(T) checkpoints (K={p32.K}), FP32 forward and "
+                               "adjoint, FP64 reductions; value = N*T / (forward + reverse sweep) seconds",
+                   "particles": int(tab.shape[0]), "lattice": [n, n, n],
+                   "porosity": 1.0 - float(alpha.double().mean()), "transition_cell_fraction": trans},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_adjoint_step (FP32)",
+                     "bytes_per_cell": bytes_adj, "kernel_us": adj_ms * 1e3},
+        "forward_ms": fwd * 1e3, "reverse_sweep_ms": bwd * 1e3,
+        "forward_MLUPS": N * T / fwd / 1e6,
+        "field_step_kernel_us": fwd_ms * 1e3,
+        "k_fp32": k32, "k_fp64": k64, "k_rel_err": abs(k32 - k64) / abs(k64),
+        "grad_rel_err_fp32_vs_fp64": float(np.max(np.abs(g32 - g64)) / scale),
+        "fp64_gradient_s": t4 - t3,
+        "finite_differences": fd_rows,
+        "checkpoint_memory_GB": p32.memory_bytes() / 1e9,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -256,6 +409,10 @@ def main():
     lib.load()
     if args.workload == "moving":
         return bench_moving(args, psc3d, workloads, lib, torch, np)
+    if args.workload == "windows":
+        return bench_windows(args, workloads, torch, np)
+    if args.workload == "adjoint":
+        return bench_adjoint(args, workloads, torch, np)
     n = args.n
     table = workloads.fixed_array(seed=0, n=n)
     total = args.warmup + args.steps

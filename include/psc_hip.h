@@ -58,6 +58,12 @@ extern "C" {
 
 #define PSC_NSUM 13 /* number of FP64 sums produced by psc_moments           */
 
+/* subsamples value of psc_step / psc_moments / psc_run_steps selecting a
+ * given solid-fraction field: d_count_sum then points to alpha in [0, 1],
+ * one value per cell in the storage precision (float for PSC_F32_DEV,
+ * double for PSC_F64_PLAIN), e.g. the output of psc_smooth_alpha. */
+#define PSC_ALPHA_FIELD 0
+
 /* Library version (major*10000 + minor*100 + patch). */
 int psc_version(void);
 
@@ -82,7 +88,9 @@ int psc_init_state(void* d_state, int32_t storage, int64_t ncell, void* stream);
  * solid_velocity (psc3d.py:181-206, :347-373).  One workgroup per chunk:
  *   entries of chunk c (the cells of its box range with count > 0, in box
  *   order) are written at [c*chunk_cells, c*chunk_cells + d_chunk_nnz[c]):
- *     d_ent_cell  wrapped cell id (uint32)
+ *     d_ent_cell  wrapped cell id + cell_base (uint32; cell_base places a
+ *                 window's cells in arrays shared by batched windows, 0
+ *                 otherwise)
  *     d_ent_loc   box-local coordinates lx | ly << 10 | lz << 20 (box
  *                 extents must not exceed 1024 cells)
  *     d_ent_cnt   number of sub-samples inside (uint16; alpha = cnt/s^3)
@@ -91,13 +99,14 @@ int psc_init_state(void* d_state, int32_t storage, int64_t ncell, void* stream);
  *                     (FP32 if us_precision == 1, FP64 if 2; must be cleared)
  * Box cells are wrapped into the lattice, so clamped boxes (the reference,
  * psc3d.py:73-78) and unclamped periodic boxes use the same call; lever
- * arms use the unwrapped cell position.  Deterministic apart from the
+ * arms use the unwrapped cell position.  pitch_z (0 = nz) is the row pitch
+ * of count_sum/us_num when rows are padded (batched windows).  Deterministic apart from the
  * order of the FP atomics into d_us_num on cells shared by two spheres. */
 int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
                  int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz,
-                 int32_t subsamples, uint32_t* d_ent_cell, uint32_t* d_ent_loc, uint16_t* d_ent_cnt,
+                 int32_t pitch_z, int32_t subsamples, uint32_t* d_ent_cell, uint32_t* d_ent_loc, uint16_t* d_ent_cnt,
                  int32_t* d_chunk_nnz, uint32_t* d_count_sum, void* d_us_num, int32_t us_precision,
-                 void* stream);
+                 uint32_t cell_base, void* stream);
 
 /* Incremental re-voxelization support: zero d_count_sum (and d_us_num, when
  * not NULL) at every cell listed in the entries of a previous psc_voxelize
@@ -121,11 +130,41 @@ int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, in
  * the resident workgroups per CU (through an LDS reservation, 0 = no cap),
  * bits 16-17 set the cells per thread (1 or 2);
  * storage/layout as above.  Precision of d_in/d_out/d_us_num/d_imb follows
- * `storage`. */
+ * `storage`.  subsamples = PSC_ALPHA_FIELD reads the solid fraction from a
+ * field instead (no solid velocity: d_us_num must be NULL); a cell is fluid
+ * where the field is +0. */
 int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const void* d_us_num,
              void* d_imb, int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz,
              int32_t subsamples, double tau, double fx, double fy, double fz, int32_t block_z,
              int32_t block_y, int32_t flags, void* stream);
+
+/* One lattice window of a batch (device table entry, 64 bytes). */
+typedef struct PscWindow {
+  int64_t state_off;  /* element offset of the window's 19 planes in the state buffers */
+  int64_t cell_off;   /* cell offset in count_sum and imb (shared by all windows)      */
+  int32_t nx, ny, nz; /* window extents                                               */
+  int32_t pz;         /* z row pitch >= nz (rows padded for aligned access)           */
+  float inv_pz, inv_nypz; /* 1/pz and 1/(ny*pz) in FP32 (index decomposition)        */
+  float fx, fy, fz, pad2; /* body force of the window                                 */
+  int64_t ps;         /* cells per direction plane: nx*ny*pz rounded up to 64 (256 B) */
+} PscWindow;
+
+/* K5 -- batched FP32 deviation step over many independent periodic windows
+ * in one persistent launch (pull layout, stationary solids).  Window w's
+ * state is 19 planes of its N_w = nx*ny*pz cells (rows padded to the pitch
+ * pz; padding cells are never read or written), each plane ps cells long,
+ * at d_in + state_off; its
+ * count_sum/imb cells start at cell_off (imb has 3 planes of imb_stride
+ * cells).  d_items lists (window, first cell) pairs of spans of up to 256
+ * cells; each workgroup of 256 threads steps one span at a time, looping
+ * over the items with a stride of grid_blocks (0 = one workgroup per item).
+ * Each window's step is collide_stream_psc (psc3d.py:208-246), the same
+ * arithmetic as psc_step with PSC_F32_DEV.  N_w must stay below 2^22 cells.
+ * flags: PSC_STEP_NONTEMPORAL and the workgroups-per-CU cap of psc_step. */
+int psc_step_batched(const float* d_in, float* d_out, const uint32_t* d_count_sum, float* d_imb,
+                     int64_t imb_stride, const PscWindow* d_windows, const int32_t* d_items,
+                     int32_t nitems, int32_t subsamples, double tau, int32_t flags,
+                     int32_t grid_blocks, void* stream);
 
 /* K3 -- node-summed hydrodynamic force and torque per particle.
  * Replaces accumulate_particle_loads (psc3d.py:255-281 dense, :375-407
@@ -154,11 +193,16 @@ int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const 
  *   0 sum solid, 1 sum (1-solid), 2-4 sum u, 5-7 sum (1-solid) u,
  *   8-10 sum imb over cells with count_sum > 0, 11 sum rho,
  *   12 number of cells with a non-finite moment.
- * solid = min(count_sum/s^3, 1).  d_partials needs psc_moments_blocks(N)*
+ * solid = min(count_sum/s^3, 1), or the field value for subsamples =
+ * PSC_ALPHA_FIELD.  d_partials needs psc_moments_blocks(N)*
  * PSC_NSUM doubles.  Deterministic (fixed-order two-pass reduction).
- * d_imb may be NULL (sums 8-10 are then 0). */
+ * d_imb may be NULL (sums 8-10 are then 0).  pitch_z (0 = nz): row pitch of a
+ * padded lattice (padding cells are skipped; fields need pitch_z = 0); with
+ * pitch_z > 0 each direction plane holds nx*ny*pitch_z cells rounded up to a
+ * multiple of 64 (the batched-window layout). */
 int psc_moments(const void* d_state, int32_t storage, int32_t layout, const uint32_t* d_count_sum,
-                const void* d_imb, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double fx,
+                const void* d_imb, int32_t nx, int32_t ny, int32_t nz, int32_t pitch_z,
+                int32_t subsamples, double fx,
                 double fy, double fz, double* d_rho_u, double* d_partials, double* d_sums,
                 void* stream);
 
@@ -225,6 +269,65 @@ int psc_graph_cache_clear(void);
 /* Plain device copy of n floats, the bandwidth reference for the roofline
  * and for scaling HBM byte counters (no counterpart in the reference). */
 int psc_copy_f32(const float* d_src, float* d_dst, int64_t n, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Differentiation: permeability gradients with respect to the particles.
+ * No counterpart in the reference (its solid fraction is a piecewise-
+ * constant sub-sample count, psc3d.py:83-89, with zero derivative almost
+ * everywhere).  The smooth solid fraction below replaces it for gradient
+ * work; the adjoint entry points are the reverse mode of psc_step in the
+ * pull layout with a PSC_ALPHA_FIELD solid fraction and stationary solids.
+ * ---------------------------------------------------------------------- */
+
+/* K1s -- smooth solid fraction of a periodic lattice:
+ *   alpha(x) = 1 - prod_p (1 - S(r_p + 1/2 - |x - c_p|)),
+ * x the cell centre (ix + 1/2, ...), S the quintic smoothstep on [0, 1]
+ * (S = 0 below 0, 1 above 1).  Boxes/chunks as for psc_voxelize, built
+ * without clamping (periodic); each box must contain the support
+ * |x - c| < r + 1/2.  Per cell, d_nfull (uint32) counts the particles with
+ * S = 1 and d_logsum (int64) sums log(1 - S) of the others in fixed point
+ * (units of 2^-50, integer atomics: the result does not depend on the order
+ * of the particles and is bitwise reproducible); both are cleared here and
+ * kept for psc_smooth_alpha_grad.  alpha is written as FP64 (d_alpha64)
+ * and/or FP32 (d_alpha32); either may be NULL. */
+int psc_smooth_alpha(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
+                     int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz,
+                     uint32_t* d_nfull, int64_t* d_logsum, double* d_alpha64, float* d_alpha32,
+                     void* stream);
+
+/* Chain rule through psc_smooth_alpha: given dL/dalpha per cell (d_galpha,
+ * FP64) and the d_nfull/d_logsum of the same call, writes d_grad (P rows of
+ * 4 doubles: dL/dx, dL/dy, dL/dz, dL/dr).  Pass 1 writes one partial per
+ * chunk into d_partials (nchunks*4 doubles), pass 2 adds the chunks
+ * d_chunk_first[p] .. d_chunk_first[p+1]-1 of each particle in order
+ * (deterministic). */
+int psc_smooth_alpha_grad(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
+                          const int32_t* d_chunk_first, int32_t nchunks, int32_t chunk_cells,
+                          int32_t nparticles, int32_t nx, int32_t ny, int32_t nz,
+                          const uint32_t* d_nfull, const int64_t* d_logsum, const double* d_galpha,
+                          double* d_partials, double* d_grad, void* stream);
+
+/* Adjoint seed of the permeability along the body force after the last
+ * step, k = nu sum_x (1 - alpha) (e . u) / (N |F|), nu = (tau - 1/2)/3,
+ * e = F/|F|, u = (j + F/2)/rho of the post-stream state (the readout of
+ * run_psc_permeability_resolution.py:144-150 with a field alpha).  Writes
+ * d_lam = weight * dk/d(state) (19 planes, pull layout, storage precision)
+ * and sets d_galpha = weight * dk/dalpha (explicit part, FP64 per cell). */
+int psc_adjoint_seed(const void* d_state, const void* d_alpha, void* d_lam, double* d_galpha,
+                     int32_t storage, int32_t nx, int32_t ny, int32_t nz, double tau, double fx,
+                     double fy, double fz, double weight, void* stream);
+
+/* K6 -- one reverse step.  d_state is the forward state the step started
+ * from (pull layout), d_lam_in the adjoint of the state it produced; writes
+ * d_lam_out, the adjoint of d_state (transposed collision Jacobian, then
+ * the reversed streaming as a scatter), and adds dL/dalpha of this step to
+ * d_galpha in every cell with 0 < alpha < 1 (the only cells where a smooth
+ * solid fraction depends on the particles).  Storage precision as for
+ * psc_step (FP32: deviations, adjoint in FP32; FP64: plain, adjoint in
+ * FP64); d_alpha as for PSC_ALPHA_FIELD.  flags: PSC_STEP_NONTEMPORAL. */
+int psc_adjoint_step(const void* d_state, const void* d_lam_in, void* d_lam_out, const void* d_alpha,
+                     double* d_galpha, int32_t storage, int32_t nx, int32_t ny, int32_t nz,
+                     double tau, double fx, double fy, double fz, int32_t flags, void* stream);
 
 #ifdef __cplusplus
 }

@@ -20,6 +20,13 @@ generalisations keep the reference behaviour as their default:
   periodic lattice instead of being clamped (psc3d.py:73-78 clamps); lever arms
   then use the unwrapped cell position.  Default False = reference.
 
+One extension has no reference counterpart: ``smooth_alpha`` (a smooth solid
+fraction of sphere lists, the geometry of permeability gradients) and the
+``solid_field`` option of ``OracleSolver`` that steps a given solid-fraction
+field.  The step itself is unchanged; ``smooth_alpha`` is pinned by closed
+forms (interior / exterior cells, the union rule, the sphere volume) in
+``tests/test_adjoint_cpu.py``.
+
 Parity status per function (how each is pinned) is listed in the test
 modules ``tests/test_oracle_golden.py`` and ``tests/test_oracle_invariants.py``.
 """
@@ -157,6 +164,40 @@ def dense_alpha(particles, n: int, subsamples: int, periodic: bool = False) -> n
 # --------------------------------------------------------------------------
 # a6/a7: equilibrium and Guo forcing (psc3d.py:97-114), same operation order
 # --------------------------------------------------------------------------
+def smoothstep5(t):
+    """Quintic smoothstep 6t^5 - 15t^4 + 10t^3 on [0, 1], 0 below, 1 above."""
+    t = np.clip(t, 0.0, 1.0)
+    return t * t * t * (10.0 + t * (-15.0 + 6.0 * t))
+
+
+def smooth_alpha(table, shape):
+    """Smooth solid fraction of spheres on a periodic lattice (no reference
+    counterpart; the geometry of the permeability gradient):
+
+        alpha(x) = 1 - prod_p (1 - S(r_p + 1/2 - |x - c_p|)),
+
+    x the cell centre (i + 1/2) and |x - c| the minimum-image distance.  A cell
+    fully inside a sphere (S = 1) gets alpha = 1.  ``table`` rows start with
+    x, y, z, r (the particle-table layout)."""
+    nx, ny, nz = shape
+    X = [np.arange(m, dtype=np.float64) + 0.5 for m in (nx, ny, nz)]
+    logs = np.zeros(shape)
+    full = np.zeros(shape, dtype=bool)
+    for row in np.asarray(table, dtype=np.float64):
+        d2 = np.zeros(shape)
+        for a, (m, xa) in enumerate(zip((nx, ny, nz), X)):
+            dd = xa - row[a]
+            dd = dd - m * np.round(dd / m)
+            sh = [1, 1, 1]
+            sh[a] = m
+            d2 = d2 + (dd * dd).reshape(sh)
+        ap = smoothstep5(row[3] + 0.5 - np.sqrt(d2))
+        full |= ap >= 1.0
+        part = (ap > 0.0) & (ap < 1.0)
+        logs[part] += np.log1p(-ap[part])
+    return np.where(full, 1.0, np.where(logs == 0.0, 0.0, -np.expm1(logs)))
+
+
 def equilibrium(rho, ux, uy, uz):
     uu = ux * ux + uy * uy + uz * uz
     feq = np.empty((Q, *np.shape(rho)), dtype=np.float64)
@@ -226,7 +267,10 @@ class OracleSolver:
     """
 
     def __init__(self, n, particles, tau=0.72, subsamples=4, body_force=(0.0, 0.0, 0.0),
-                 static_geometry=False, layout="sparse", periodic_geometry=False):
+                 static_geometry=False, layout="sparse", periodic_geometry=False, solid_field=None):
+        if solid_field is not None and len(particles):
+            raise ValueError("a solid_field replaces the particle list")
+        self.solid_field = None if solid_field is None else np.asarray(solid_field, dtype=np.float64)
         self.n = int(n)
         self.particles = particles
         self.tau = float(tau)
@@ -258,7 +302,12 @@ class OracleSolver:
     # -- geometry (psc3d.py:151-179 dense, 328-345 sparse) -----------------
     def rebuild_geometry(self):
         n = self.n
-        if self.layout == "dense":
+        if self.solid_field is not None:
+            # a given solid fraction (no particles, no solid velocity, no loads)
+            self.alpha_sum = self.solid_field.copy()
+            self.alpha_p = np.zeros((0, n, n, n))
+            self.node_ids, self.alpha_vals, self.node_pos = [], [], []
+        elif self.layout == "dense":
             self.alpha_p = dense_alpha(self.particles, n, self.subsamples, self.periodic_geometry)
             self.alpha_sum = self.alpha_p.sum(axis=0)
         else:

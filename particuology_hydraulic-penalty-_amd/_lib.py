@@ -19,6 +19,7 @@ PSC_PUSH = 0
 PSC_PULL = 1
 PSC_STEP_NONTEMPORAL = 1
 PSC_NSUM = 13
+PSC_ALPHA_FIELD = 0
 
 # Names and argument types of every exported entry point, in header order.
 _P = ct.c_void_p
@@ -31,19 +32,26 @@ SIGNATURES = {
     "psc_lattice_constants": ([_P, _P, _P], ct.c_int),
     "psc_clear": ([_P, _I64, _P], ct.c_int),
     "psc_init_state": ([_P, _I32, _I64, _P], ct.c_int),
-    "psc_voxelize": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, _P], ct.c_int),
+    "psc_voxelize": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, ct.c_uint32,
+                      _P], ct.c_int),
+    "psc_step_batched": ([_P, _P, _P, _P, _I64, _P, _P, _I32, _I32, _D, _I32, _I32, _P], ct.c_int),
     "psc_clear_entries": ([_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _I32, _P], ct.c_int),
     "psc_step": ([_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _I32, _P],
                  ct.c_int),
     "psc_particle_loads": ([_P, _P, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P,
                             _P, _P], ct.c_int),
-    "psc_moments": ([_P, _I32, _I32, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _P, _P, _P, _P], ct.c_int),
+    "psc_moments": ([_P, _I32, _I32, _P, _P, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _P, _P, _P, _P], ct.c_int),
     "psc_moments_blocks": ([_I64], ct.c_int64),
     "psc_export_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_import_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_run_steps": ([_P, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_graph_cache_clear": ([], ct.c_int),
     "psc_copy_f32": ([_P, _P, _I64, _P], ct.c_int),
+    "psc_smooth_alpha": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P], ct.c_int),
+    "psc_smooth_alpha_grad": ([_P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P],
+                              ct.c_int),
+    "psc_adjoint_seed": ([_P, _P, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _D, _P], ct.c_int),
+    "psc_adjoint_step": ([_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _P], ct.c_int),
 }
 
 
@@ -60,6 +68,14 @@ class PscStepPlan(ct.Structure):
         ("ent_cell", _P), ("ent_loc", _P), ("ent_cnt", _P), ("chunk_nnz", _P), ("partials", _P),
         ("partials_alt", _P), ("loads", _P),
     ]
+
+
+class PscWindow(ct.Structure):
+    """Mirror of struct PscWindow (include/psc_hip.h)."""
+
+    _fields_ = [("state_off", _I64), ("cell_off", _I64), ("nx", _I32), ("ny", _I32), ("nz", _I32), ("pz", _I32),
+                ("inv_pz", ct.c_float), ("inv_nypz", ct.c_float), ("fx", ct.c_float), ("fy", ct.c_float),
+                ("fz", ct.c_float), ("pad2", ct.c_float), ("ps", _I64)]
 
 
 class PscError(RuntimeError):

@@ -1,0 +1,272 @@
+"""Many independent pore windows stepped together on one GPU.
+
+The reference runs one window per process (``SparsePscD3Q19`` per case, sharded
+by manifest row, run_synced_campaign.py:59-73).  Here a batch of periodic
+windows of uneven sizes shares one set of device buffers:
+
+* state: the 19 FP32 deviation planes of each window, window after window;
+* count_sum / imb: one array over the concatenated cells of all windows;
+* particles, boxes, chunks and voxel entries: concatenated, with global
+  particle ids, so one launch of the load kernels covers every window.
+
+Each step is ONE persistent launch of ``psc_step_batched`` (work items are
+256-cell spans of a window), followed by the particle-load reduction; ``run``
+captures two steps into a hipGraph (torch.cuda.CUDAGraph around the C-ABI
+launches) and replays it.  The launch configuration (non-temporal access,
+occupancy cap, persistent grid size) is autotuned on first use.
+Every window is the reference's ``collide_stream_psc`` (psc3d.py:208-253) with
+periodic geometry; per-window permeability follows
+run_psc_permeability_resolution.py:144-150.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib
+from .geometry import build_tables, pack_particles
+
+SPAN = 256
+MAX_WINDOW_CELLS = 1 << 22
+ROW_ALIGN = 4  # z rows padded to a multiple of 4 cells (16-byte aligned rows)
+PLANE_ALIGN = 64  # direction planes padded to a multiple of 64 cells (256-byte aligned planes)
+
+
+class BatchedPscD3Q19:
+    def __init__(self, windows, tau: float = 0.8, subsamples: int = 3, body_force=(0.0, 0.0, 1.0e-6),
+                 device=None, chunk_cells: int = 2048, launch="auto"):
+        import torch
+        self._lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise _lib.PscError("BatchedPscD3Q19 needs a ROCm GPU (gfx950)")
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.tau = float(tau)
+        self.subsamples = int(subsamples)
+        self.body_force = tuple(float(v) for v in body_force)
+        self._s = max(1, self.subsamples)
+        self.shapes = []
+        tables = []
+        for shape, parts in windows:
+            shp = (int(shape),) * 3 if np.isscalar(shape) else tuple(int(v) for v in shape)
+            if shp[0] * shp[1] * shp[2] > MAX_WINDOW_CELLS:
+                raise ValueError(f"window {shp} exceeds {MAX_WINDOW_CELLS} cells")
+            self.shapes.append(shp)
+            tables.append(pack_particles(parts))
+        self.nwin = len(self.shapes)
+        self.pitch = [-(-c // ROW_ALIGN) * ROW_ALIGN for _, _, c in self.shapes]
+        # cells per window incl. row padding, and per direction plane (64-cell multiple: 256-byte aligned planes,
+        # the layout psc_moments assumes for pitch_z > nz)
+        ncell = np.array([a * b * p for (a, b, _), p in zip(self.shapes, self.pitch)], dtype=np.int64)
+        self.plane = np.array([-(-int(v) // PLANE_ALIGN) * PLANE_ALIGN for v in ncell], dtype=np.int64)
+        self.cell_off = np.concatenate([[0], np.cumsum(self.plane)[:-1]]).astype(np.int64)
+        self.state_off = 19 * self.cell_off
+        self.ncell = ncell                       # padded cells per window
+        self.nlive = np.array([a * b * c for a, b, c in self.shapes], dtype=np.int64)
+        self.Ntot = int(self.plane.sum())
+        dev = self.device
+        self._a = torch.zeros(19 * self.Ntot, dtype=torch.float32, device=dev)
+        self._b = torch.zeros(19 * self.Ntot, dtype=torch.float32, device=dev)
+        self._csum = torch.zeros(self.Ntot, dtype=torch.int32, device=dev)
+        self._imb = torch.zeros(3 * self.Ntot, dtype=torch.float32, device=dev)
+        # window table and work items
+        wt = (_lib.PscWindow * self.nwin)()
+        items = []
+        for w, (nx, ny, nz) in enumerate(self.shapes):
+            wt[w].state_off = int(self.state_off[w])
+            wt[w].cell_off = int(self.cell_off[w])
+            pz = self.pitch[w]
+            wt[w].nx, wt[w].ny, wt[w].nz, wt[w].pz = nx, ny, nz, pz
+            wt[w].inv_pz = 1.0 / pz
+            wt[w].inv_nypz = 1.0 / (ny * pz)
+            wt[w].ps = int(self.plane[w])
+            wt[w].fx, wt[w].fy, wt[w].fz = self.body_force
+            for start in range(0, int(ncell[w]), SPAN):
+                items.append((w, start))
+        raw = np.frombuffer(bytes(wt), dtype=np.uint8)
+        self._windows = torch.from_numpy(raw.copy()).to(dev)
+        self._items = torch.from_numpy(np.asarray(items, dtype=np.int32).reshape(-1)).to(dev)
+        self.nitems = len(items)
+        # geometry: per-window boxes and chunks, concatenated with global ids
+        self.tables = tables
+        self.pbase = np.concatenate([[0], np.cumsum([t.shape[0] for t in tables])[:-1]]).astype(np.int64)
+        tabs = [build_tables(t, shp, periodic=True, chunk_cells=chunk_cells) for t, shp in zip(tables, self.shapes)]
+        self.chunk_cells = chunk_cells
+        self.cbase = np.concatenate([[0], np.cumsum([t.chunks.shape[0] for t in tabs])[:-1]]).astype(np.int64)
+        parts = np.concatenate([t.particles for t in tabs]) if tabs else np.zeros((0, 10))
+        boxes = np.concatenate([t.boxes for t in tabs])
+        chunks = np.concatenate([t.chunks + np.array([self.pbase[w], 0], dtype=np.int32)
+                                 for w, t in enumerate(tabs)])
+        cfirst = [0]
+        for w, t in enumerate(tabs):
+            cfirst.extend((t.chunk_first[1:] + self.cbase[w]).tolist())
+        self.P = parts.shape[0]
+        self.C = chunks.shape[0]
+        self._parts = torch.from_numpy(np.ascontiguousarray(parts)).to(dev)
+        self._boxes = torch.from_numpy(np.ascontiguousarray(boxes)).to(dev)
+        self._chunks = torch.from_numpy(np.ascontiguousarray(chunks.astype(np.int32))).to(dev)
+        self._cfirst = torch.from_numpy(np.asarray(cfirst, dtype=np.int32)).to(dev)
+        cap = max(1, self.C * chunk_cells)
+        self._ent = (torch.empty(cap, dtype=torch.int32, device=dev), torch.empty(cap, dtype=torch.int32, device=dev),
+                     torch.empty(cap, dtype=torch.int16, device=dev))
+        self._nnz = torch.zeros(max(1, self.C), dtype=torch.int32, device=dev)
+        self._partials = torch.zeros(6 * max(1, self.C), dtype=torch.float64, device=dev)
+        self._loads = torch.zeros((max(1, self.P), 6), dtype=torch.float64, device=dev)
+        self._sums = torch.zeros(self.nwin * _lib.PSC_NSUM, dtype=torch.float64, device=dev)
+        self._mom_partials = torch.zeros(int(self._lib.psc_moments_blocks(int(ncell.max()))) * _lib.PSC_NSUM,
+                                         dtype=torch.float64, device=dev)
+        self._flags = 0
+        self._grid = 0
+        self._tuned = launch != "auto"
+        if launch != "auto":
+            self._flags, self._grid = int(launch[0]), int(launch[1])
+        self._graph = None
+        self._steps = 0
+        self._voxelize(tabs)
+
+    # ----------------------------------------------------------------- setup
+    def _stream(self):
+        import torch
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _voxelize(self, tabs):
+        s = self._stream()
+        ec, el, en = self._ent
+        CK = self.chunk_cells
+        for w, t in enumerate(tabs):
+            nch = t.chunks.shape[0]
+            if nch == 0:
+                continue
+            cb = int(self.cbase[w])
+            nx, ny, nz = self.shapes[w]
+            _lib.call("psc_voxelize", self._parts.data_ptr(), self._boxes.data_ptr(),
+                      self._chunks.data_ptr() + 8 * cb, nch, CK, nx, ny, nz, self.pitch[w], self._s,
+                      ec.data_ptr() + 4 * cb * CK, el.data_ptr() + 4 * cb * CK, en.data_ptr() + 2 * cb * CK,
+                      self._nnz.data_ptr() + 4 * cb, self._csum.data_ptr() + 4 * int(self.cell_off[w]), None, 0,
+                      int(self.cell_off[w]), s)
+
+    # ------------------------------------------------------------------ step
+    def _launch_step(self, a, b, flags, grid, stream):
+        _lib.call("psc_step_batched", a.data_ptr(), b.data_ptr(), self._csum.data_ptr(), self._imb.data_ptr(),
+                  self.Ntot, self._windows.data_ptr(), self._items.data_ptr(), self.nitems, self._s, self.tau,
+                  flags, grid, stream)
+
+    def _launch_loads(self, stream):
+        if self.P == 0:
+            return
+        ec, el, en = self._ent
+        _lib.call("psc_particle_loads", self._parts.data_ptr(), self._boxes.data_ptr(), self._chunks.data_ptr(),
+                  self._cfirst.data_ptr(), self.P, self.C, self.chunk_cells, ec.data_ptr(), el.data_ptr(),
+                  en.data_ptr(), self._nnz.data_ptr(), self._csum.data_ptr(), self._imb.data_ptr(), 1, 1, 1,
+                  self.Ntot, self._partials.data_ptr(), self._loads.data_ptr(), stream)
+
+    def step(self):
+        """One step of every window (collide_stream_psc) plus the loads."""
+        self._tune()
+        s = self._stream()
+        self._launch_step(self._a, self._b, self._flags, self._grid, s)
+        self._a, self._b = self._b, self._a
+        self._launch_loads(s)
+        self._steps += 1
+
+    def _tune(self):
+        """Pick non-temporal access, occupancy cap and persistent grid size."""
+        if self._tuned:
+            return
+        import torch
+        from . import autotune
+        s = self._stream()
+        ncu = torch.cuda.get_device_properties(self.device).multi_processor_count
+        best = None
+        for nt in (0, 1):
+            for bpc in (0, 2, 4, 6, 8):
+                for grid in (0, 2 * ncu, 4 * ncu, 8 * ncu):
+                    flags = autotune.flags_of(nt, bpc, 0)
+                    self._launch_step(self._a, self._b, flags, grid, s)
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    for _ in range(3):
+                        self._launch_step(self._a, self._b, flags, grid, s)
+                    e1.record()
+                    e1.synchronize()
+                    ms = e0.elapsed_time(e1) / 3
+                    if best is None or ms < best[0]:
+                        best = (ms, flags, grid)
+        self._flags, self._grid = best[1], best[2]
+        self._tuned = True
+
+    def run(self, steps: int, graph: bool = True) -> None:
+        """``steps`` steps; with graph=True two steps are captured once into a
+        hipGraph (one persistent step launch + the loads per step) and
+        replayed."""
+        import torch
+        steps = int(steps)
+        self._tune()
+        if not graph or steps < 2:
+            for _ in range(steps):
+                self.step()
+            return
+        if self._graph is None:
+            a, b = self._a, self._b
+            torch.cuda.synchronize(self.device)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                s = torch.cuda.current_stream(self.device).cuda_stream
+                self._launch_step(a, b, self._flags, self._grid, s)
+                self._launch_loads(s)
+                self._launch_step(b, a, self._flags, self._grid, s)
+                self._launch_loads(s)
+            self._graph = (g, a, b)
+        g, a, b = self._graph
+        if self._a is not a:  # odd number of eager steps since capture
+            self.step()
+            steps -= 1
+        for _ in range(steps // 2):
+            g.replay()
+        self._steps += 2 * (steps // 2)
+        if steps % 2:
+            self.step()
+
+    # -------------------------------------------------------------- readouts
+    def readout_sums(self):
+        """(nwin, PSC_NSUM) FP64 sums of each window's post-stream state."""
+        s = self._stream()
+        for w, (nx, ny, nz) in enumerate(self.shapes):
+            _lib.call("psc_moments", self._a.data_ptr() + 4 * int(self.state_off[w]), _lib.PSC_F32_DEV,
+                      _lib.PSC_PULL, self._csum.data_ptr() + 4 * int(self.cell_off[w]), None, nx, ny, nz,
+                      self.pitch[w], self._s,
+                      *self.body_force, None, self._mom_partials.data_ptr(),
+                      self._sums.data_ptr() + 8 * _lib.PSC_NSUM * w, s)
+        return self._sums.cpu().numpy().reshape(self.nwin, _lib.PSC_NSUM)
+
+    def permeability(self, axis=None):
+        """Per-window k = nu sum((1-solid) u_axis) / N_w / F_axis."""
+        if axis is None:
+            axis = int(np.argmax(np.abs(self.body_force)))
+        sums = self.readout_sums()
+        nu = (self.tau - 0.5) / 3.0
+        return nu * (sums[:, 5 + axis] / self.nlive) / self.body_force[axis]
+
+    def porosity(self):
+        sums = self.readout_sums()
+        return sums[:, 1] / self.nlive
+
+    def loads(self):
+        """Per-window (P_w, 6) FP64 force/torque arrays of the last step."""
+        L = self._loads.cpu().numpy()[: self.P]
+        out = []
+        for w, t in enumerate(self.tables):
+            p0 = int(self.pbase[w])
+            out.append(L[p0: p0 + t.shape[0]])
+        return out
+
+    def window_state(self, w):
+        """FP32 deviation state of window w (19, nx, ny, nz), post-collision."""
+        n = int(self.ncell[w])
+        ps = int(self.plane[w])
+        o = int(self.state_off[w])
+        nx, ny, nz = self.shapes[w]
+        st = self._a[o: o + 19 * ps].cpu().numpy().reshape(19, ps)[:, :n].reshape(19, nx, ny, self.pitch[w])
+        return st[..., :nz]

@@ -2,11 +2,13 @@
 
     python particuology_hydraulic-penalty-_amd/csrc/build.py
 
-Three translation units:
+Four translation units:
   psc_exact.hip  FP64 kernels in the reference's operation order, compiled
                  with -ffp-contract=off (no fused multiply-add), plus the
                  voxelizer, particle loads, moments and f import/export;
   psc_fast.hip   the FP32 deviation step, -ffp-contract=fast;
+  psc_adjoint.hip  smooth solid fraction and the reverse-mode step,
+                 -ffp-contract=fast;
   psc_abi.hip    the extern "C" layer.
 Packed FP32 instructions are disabled on every device compile.
 """
@@ -34,6 +36,7 @@ COMMON = [
 UNITS = [
     ("psc_exact.hip", ["-ffp-contract=off"]),
     ("psc_fast.hip", ["-ffp-contract=fast"]),
+    ("psc_adjoint.hip", ["-ffp-contract=fast"]),
     ("psc_abi.hip", ["-ffp-contract=off"]),
 ]
 

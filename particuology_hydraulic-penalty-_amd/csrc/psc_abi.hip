@@ -41,17 +41,29 @@ int launch_step_f64(const double*, double*, const uint32_t*, const double*, doub
                     double, double, double, int, int, hipStream_t);
 int launch_step_f32(const float*, float*, const uint32_t*, const float*, float*, int, int, int, int, int, double,
                     double, double, double, int, int, int, hipStream_t);
-int launch_voxelize(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, uint32_t*, uint32_t*,
-                    uint16_t*, int32_t*, uint32_t*, void*, int, hipStream_t);
+int launch_voxelize(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, int, uint32_t*,
+                    uint32_t*, uint16_t*, int32_t*, uint32_t*, void*, int, uint32_t, hipStream_t);
+int launch_step_f32_batched(const float*, float*, const uint32_t*, float*, size_t, const void*, const void*, int, int,
+                            double, int, int, hipStream_t);
 int launch_loads(const double*, const int32_t*, const int32_t*, const int32_t*, int, int, int, const uint32_t*,
                  const uint32_t*, const uint16_t*, const int32_t*, const uint32_t*, const void*, int, int, int, int,
                  double*, double*, hipStream_t);
-int launch_moments(const void*, int, int, const uint32_t*, const void*, int, int, int, int, double, double, double,
-                   double*, double*, double*, hipStream_t);
+int launch_moments(const void*, int, int, const uint32_t*, const void*, int, int, int, int, unsigned long long, int,
+                   double, double, double, double*, double*, double*, hipStream_t);
 int64_t moments_blocks(int64_t);
 int launch_clear_entries(const uint32_t*, const int32_t*, int, int, int, int, int, uint32_t*, void*, int, hipStream_t);
 int launch_loads_chunks(const LoadsArgs&, const void*, int, int, int, int, double*, hipStream_t);
 int launch_loads_final(const int32_t*, int, const double*, double*, hipStream_t);
+
+int launch_smooth_alpha(const double*, const int32_t*, const int32_t*, int, int, int, int, int, uint32_t*,
+                        unsigned long long*, double*, float*, hipStream_t);
+int launch_smooth_alpha_grad(const double*, const int32_t*, const int32_t*, const int32_t*, int, int, int, int, int,
+                             int, const uint32_t*, const unsigned long long*, const double*, double*, double*,
+                             hipStream_t);
+int launch_adjoint_seed(const void*, const void*, void*, double*, int, int, int, int, double, double, double, double,
+                        double, hipStream_t);
+int launch_adjoint_step(const void*, const void*, void*, const void*, double*, int, int, int, int, double, double,
+                        double, double, int, hipStream_t);
 
 int launch_export(const void*, int, int, int, int, int, double*, hipStream_t);
 int launch_import(const double*, int, int, int, int, int, void*, hipStream_t);
@@ -150,9 +162,10 @@ int psc_init_state(void* d_state, int32_t storage, int64_t ncell, void* stream) 
 }
 
 int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks, int32_t nchunks,
-                 int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, uint32_t* d_ent_cell,
+                 int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, int32_t pitch_z, int32_t subsamples,
+                 uint32_t* d_ent_cell,
                  uint32_t* d_ent_loc, uint16_t* d_ent_cnt, int32_t* d_chunk_nnz, uint32_t* d_count_sum,
-                 void* d_us_num, int32_t us_precision, void* stream) {
+                 void* d_us_num, int32_t us_precision, uint32_t cell_base, void* stream) {
   if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_voxelize: bad lattice %dx%dx%d", nx, ny, nz);
   if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
     return set_error(PSC_EINVAL, "psc_voxelize: subsamples %d outside [1, 40]", subsamples);
@@ -163,9 +176,11 @@ int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_
     return set_error(PSC_EINVAL, "psc_voxelize: null pointer");
   if (d_us_num != nullptr && us_precision != 1 && us_precision != 2)
     return set_error(PSC_EINVAL, "psc_voxelize: us_precision %d", us_precision);
-  return launch_voxelize(d_particles, d_boxes, d_chunks, nchunks, chunk_cells, nx, ny, nz, subsamples, d_ent_cell,
+  const int pz = pitch_z > 0 ? pitch_z : nz;
+  if (pz < nz || !dims_ok(nx, ny, pz)) return set_error(PSC_EINVAL, "psc_voxelize: pitch_z %d < nz %d", pitch_z, nz);
+  return launch_voxelize(d_particles, d_boxes, d_chunks, nchunks, chunk_cells, nx, ny, nz, pz, subsamples, d_ent_cell,
                          d_ent_loc, d_ent_cnt, d_chunk_nnz, d_count_sum, d_us_num, d_us_num ? us_precision : 0,
-                         (hipStream_t)stream);
+                         cell_base, (hipStream_t)stream);
 }
 
 int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, int32_t nchunks, int32_t chunk_cells,
@@ -188,8 +203,10 @@ int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const v
   if (d_in == nullptr || d_out == nullptr || d_count_sum == nullptr) return set_error(PSC_EINVAL, "psc_step: null");
   if (d_in == d_out) return set_error(PSC_EINVAL, "psc_step: input and output alias");
   if (layout != PSC_PUSH && layout != PSC_PULL) return set_error(PSC_EINVAL, "psc_step: layout %d", layout);
-  if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
+  if (subsamples < 0 || subsamples * subsamples * subsamples > 65535)
     return set_error(PSC_EINVAL, "psc_step: subsamples %d", subsamples);
+  if (subsamples == PSC_ALPHA_FIELD && d_us_num != nullptr)
+    return set_error(PSC_EINVAL, "psc_step: a solid-fraction field (subsamples 0) takes no solid velocity");
   if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_step: tau %g must exceed 1/2", tau);
   int bz = block_z, by = block_y;
   if (bz <= 0) bz = nz >= 64 ? 64 : 32;
@@ -207,6 +224,24 @@ int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const v
     return launch_step_f64((const double*)d_in, (double*)d_out, d_count_sum, (const double*)d_us_num,
                            (double*)d_imb, layout, nx, ny, nz, subsamples, tau, fx, fy, fz, bz, by, s);
   return set_error(PSC_EINVAL, "psc_step: storage %d", storage);
+}
+
+int psc_step_batched(const float* d_in, float* d_out, const uint32_t* d_count_sum, float* d_imb, int64_t imb_stride,
+                     const PscWindow* d_windows, const int32_t* d_items, int32_t nitems, int32_t subsamples,
+                     double tau, int32_t flags, int32_t grid_blocks, void* stream) {
+  if (d_in == nullptr || d_out == nullptr || d_count_sum == nullptr || d_windows == nullptr ||
+      (nitems > 0 && d_items == nullptr))
+    return set_error(PSC_EINVAL, "psc_step_batched: null pointer");
+  if (d_in == d_out) return set_error(PSC_EINVAL, "psc_step_batched: input and output alias");
+  if (nitems < 0 || grid_blocks < 0) return set_error(PSC_EINVAL, "psc_step_batched: sizes");
+  if (d_imb != nullptr && imb_stride <= 0) return set_error(PSC_EINVAL, "psc_step_batched: imb stride");
+  if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
+    return set_error(PSC_EINVAL, "psc_step_batched: subsamples %d", subsamples);
+  if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_step_batched: tau %g must exceed 1/2", tau);
+  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK)) != 0)
+    return set_error(PSC_EINVAL, "psc_step_batched: unknown flags 0x%x", flags);
+  return launch_step_f32_batched(d_in, d_out, d_count_sum, d_imb, (size_t)imb_stride, d_windows, d_items, nitems,
+                                 subsamples, tau, flags, grid_blocks, (hipStream_t)stream);
 }
 
 int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
@@ -232,7 +267,7 @@ int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const 
 int64_t psc_moments_blocks(int64_t ncell) { return moments_blocks(ncell); }
 
 int psc_moments(const void* d_state, int32_t storage, int32_t layout, const uint32_t* d_count_sum, const void* d_imb,
-                int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double fx, double fy, double fz,
+                int32_t nx, int32_t ny, int32_t nz, int32_t pitch_z, int32_t subsamples, double fx, double fy, double fz,
                 double* d_rho_u, double* d_partials, double* d_sums, void* stream) {
   if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_moments: bad lattice");
   if (storage != PSC_F32_DEV && storage != PSC_F64_PLAIN) return set_error(PSC_EINVAL, "psc_moments: storage");
@@ -240,10 +275,16 @@ int psc_moments(const void* d_state, int32_t storage, int32_t layout, const uint
   if (d_count_sum == nullptr || d_partials == nullptr || d_sums == nullptr)
     return set_error(PSC_EINVAL, "psc_moments: null");
   if (d_state == nullptr && d_rho_u != nullptr) return set_error(PSC_EINVAL, "psc_moments: fields need a state");
-  if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
+  if (subsamples < 0 || subsamples * subsamples * subsamples > 65535)
     return set_error(PSC_EINVAL, "psc_moments: subsamples %d", subsamples);
-  return launch_moments(d_state, storage, layout, d_count_sum, d_imb, nx, ny, nz, subsamples, fx, fy, fz, d_rho_u,
-                        d_partials, d_sums, (hipStream_t)stream);
+  const int pz = pitch_z > 0 ? pitch_z : nz;
+  if (pz < nz || !dims_ok(nx, ny, pz)) return set_error(PSC_EINVAL, "psc_moments: pitch_z %d < nz %d", pitch_z, nz);
+  if (pitch_z > 0 && d_rho_u != nullptr) return set_error(PSC_EINVAL, "psc_moments: fields need an unpadded lattice");
+  // explicit pitch: planes of nx*ny*pitch_z cells rounded up to 64 (batched-window layout)
+  const unsigned long long n = (unsigned long long)nx * ny * pz;
+  const unsigned long long ps = pitch_z > 0 ? (n + 63ull) / 64ull * 64ull : n;
+  return launch_moments(d_state, storage, layout, d_count_sum, d_imb, nx, ny, nz, pz, ps, subsamples, fx, fy, fz,
+                        d_rho_u, d_partials, d_sums, (hipStream_t)stream);
 }
 
 int psc_export_f64(const void* d_state, int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz,
@@ -275,6 +316,65 @@ int psc_copy_f32(const float* d_src, float* d_dst, int64_t n, void* stream) {
   else
     k_copy1<<<grid_cap((size_t)n, 256), 256, 0, s>>>(d_src, d_dst, (size_t)n);
   return check_launch("psc_copy_f32");
+}
+
+int psc_smooth_alpha(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks, int32_t nchunks,
+                     int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, uint32_t* d_nfull, int64_t* d_logsum,
+                     double* d_alpha64, float* d_alpha32, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_smooth_alpha: bad lattice");
+  if (nchunks < 0 || chunk_cells < 1) return set_error(PSC_EINVAL, "psc_smooth_alpha: sizes");
+  if (d_nfull == nullptr || d_logsum == nullptr) return set_error(PSC_EINVAL, "psc_smooth_alpha: null scratch");
+  if (nchunks > 0 && (d_particles == nullptr || d_boxes == nullptr || d_chunks == nullptr))
+    return set_error(PSC_EINVAL, "psc_smooth_alpha: null geometry");
+  return launch_smooth_alpha(d_particles, d_boxes, d_chunks, nchunks, chunk_cells, nx, ny, nz, d_nfull,
+                             (unsigned long long*)d_logsum, d_alpha64, d_alpha32, (hipStream_t)stream);
+}
+
+int psc_smooth_alpha_grad(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
+                          const int32_t* d_chunk_first, int32_t nchunks, int32_t chunk_cells, int32_t nparticles,
+                          int32_t nx, int32_t ny, int32_t nz, const uint32_t* d_nfull, const int64_t* d_logsum,
+                          const double* d_galpha, double* d_partials, double* d_grad, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_smooth_alpha_grad: bad lattice");
+  if (nchunks < 0 || chunk_cells < 1 || nparticles < 0)
+    return set_error(PSC_EINVAL, "psc_smooth_alpha_grad: sizes");
+  if (nparticles > 0 && (d_grad == nullptr || d_chunk_first == nullptr))
+    return set_error(PSC_EINVAL, "psc_smooth_alpha_grad: null output or chunk_first");
+  if (nchunks > 0 && (d_particles == nullptr || d_boxes == nullptr || d_chunks == nullptr || d_nfull == nullptr ||
+                      d_logsum == nullptr || d_galpha == nullptr || d_partials == nullptr))
+    return set_error(PSC_EINVAL, "psc_smooth_alpha_grad: null input");
+  return launch_smooth_alpha_grad(d_particles, d_boxes, d_chunks, d_chunk_first, nchunks, chunk_cells, nparticles,
+                                  nx, ny, nz, d_nfull, (const unsigned long long*)d_logsum, d_galpha, d_partials,
+                                  d_grad, (hipStream_t)stream);
+}
+
+int psc_adjoint_seed(const void* d_state, const void* d_alpha, void* d_lam, double* d_galpha, int32_t storage,
+                     int32_t nx, int32_t ny, int32_t nz, double tau, double fx, double fy, double fz, double weight,
+                     void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_adjoint_seed: bad lattice");
+  if (storage != PSC_F32_DEV && storage != PSC_F64_PLAIN) return set_error(PSC_EINVAL, "psc_adjoint_seed: storage");
+  if (d_state == nullptr || d_alpha == nullptr || d_lam == nullptr || d_galpha == nullptr)
+    return set_error(PSC_EINVAL, "psc_adjoint_seed: null");
+  if (d_state == d_lam) return set_error(PSC_EINVAL, "psc_adjoint_seed: state and adjoint alias");
+  if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_adjoint_seed: tau %g must exceed 1/2", tau);
+  if (!(fx * fx + fy * fy + fz * fz > 0.0))
+    return set_error(PSC_EINVAL, "psc_adjoint_seed: the permeability needs a non-zero body force");
+  return launch_adjoint_seed(d_state, d_alpha, d_lam, d_galpha, storage, nx, ny, nz, tau, fx, fy, fz, weight,
+                             (hipStream_t)stream);
+}
+
+int psc_adjoint_step(const void* d_state, const void* d_lam_in, void* d_lam_out, const void* d_alpha,
+                     double* d_galpha, int32_t storage, int32_t nx, int32_t ny, int32_t nz, double tau, double fx,
+                     double fy, double fz, int32_t flags, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_adjoint_step: bad lattice");
+  if (storage != PSC_F32_DEV && storage != PSC_F64_PLAIN) return set_error(PSC_EINVAL, "psc_adjoint_step: storage");
+  if (d_state == nullptr || d_lam_in == nullptr || d_lam_out == nullptr || d_alpha == nullptr || d_galpha == nullptr)
+    return set_error(PSC_EINVAL, "psc_adjoint_step: null");
+  if (d_lam_in == d_lam_out || d_state == d_lam_out)
+    return set_error(PSC_EINVAL, "psc_adjoint_step: the output adjoint aliases an input");
+  if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_adjoint_step: tau %g must exceed 1/2", tau);
+  if ((flags & ~PSC_STEP_NONTEMPORAL) != 0) return set_error(PSC_EINVAL, "psc_adjoint_step: unknown flags 0x%x", flags);
+  return launch_adjoint_step(d_state, d_lam_in, d_lam_out, d_alpha, d_galpha, storage, nx, ny, nz, tau, fx, fy, fz,
+                             flags, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -82,7 +82,16 @@ __device__ __forceinline__ void guo_all(Seq<Is...>, double pref, double ux, doub
 
 struct Dims {
   int nx, ny, nz;
+  int pz;                 // row pitch along z (cells); >= nz, padding cells are never used
+  unsigned long long ps;  // cells per direction plane (>= nx*ny*pz)
 };
+
+// Plane size of a lattice with an explicit row pitch: nx*ny*pz rounded up to
+// 64 cells, so planes start 256-byte aligned (the batched-window layout).
+__host__ __device__ inline unsigned long long plane_cells(int nx, int ny, int pz) {
+  const unsigned long long n = (unsigned long long)nx * ny * pz;
+  return (n + 63ull) / 64ull * 64ull;
+}
 
 // Cell of the thread in a (z-tiles, y-tiles, x) grid.
 __device__ __forceinline__ bool thread_cell(const Dims d, int& ix, int& iy, int& iz) {
@@ -93,7 +102,7 @@ __device__ __forceinline__ bool thread_cell(const Dims d, int& ix, int& iy, int&
 }
 
 __device__ __forceinline__ uint32_t cell_id(const Dims d, int x, int y, int z) {
-  return ((uint32_t)x * (uint32_t)d.ny + (uint32_t)y) * (uint32_t)d.nz + (uint32_t)z;
+  return ((uint32_t)x * (uint32_t)d.ny + (uint32_t)y) * (uint32_t)d.pz + (uint32_t)z;
 }
 
 // Address of direction I's value for cell (ix,iy,iz) shifted by s*c_I.
@@ -126,7 +135,7 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
                                                   double fy, double fz) {
   int ix, iy, iz;
   if (!thread_cell(d, ix, iy, iz)) return;
-  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  const size_t N = (size_t)d.ps;
   const uint32_t cell = cell_id(d, ix, iy, iz);
   double f[Q];
   load_all<PULL>(Dirs{}, fin, N, d, ix, iy, iz, f);
@@ -146,9 +155,11 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
   double feqf[Q];
   equilibrium_all(Dirs{}, rho, uxe, uye, uze, feqf);
 
-  // solid fraction and B (psc3d.py:174-175, 228-229)
-  const uint32_t cnt = csum[cell];
-  const double asum = (double)cnt / s3;
+  // solid fraction and B (psc3d.py:174-175, 228-229); s3 == 0 selects a
+  // given FP64 solid-fraction field in place of the sub-sample counts
+  const bool field = s3 == 0.0;
+  const uint32_t cnt = field ? 0u : csum[cell];
+  const double asum = field ? ((const double*)csum)[cell] : (double)cnt / s3;
   const double solid = asum < 1.0 ? asum : 1.0;
   const double b = solid / ((solid + tau * (1.0 - solid)) + 1.0e-14);
 
@@ -156,7 +167,7 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
   double gterm[Q];
   if (FORCE) guo_all(Dirs{}, pref, uxe, uye, uze, fx, fy, fz, gterm);
 
-  if (cnt == 0) {
+  if (field ? solid == 0.0 : cnt == 0) {
     // B = 0: collision = 1*Omega_bgk + 0*Omega_s
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -328,7 +339,7 @@ __global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ par
                                                   int s, uint32_t* __restrict__ ent_cell,
                                                   uint32_t* __restrict__ ent_loc, uint16_t* __restrict__ ent_cnt,
                                                   int32_t* __restrict__ chunk_nnz, uint32_t* __restrict__ csum,
-                                                  void* __restrict__ usn) {
+                                                  void* __restrict__ usn, uint32_t cell_base) {
   __shared__ int lds_waves[4];
   const int pid = chunks[2 * blockIdx.x];
   const int e0 = chunks[2 * blockIdx.x + 1];
@@ -338,7 +349,7 @@ __global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ par
   const int e1 = min(e0 + chunk_cells, vol);
   const double r2 = p.r * p.r;
   const double s3 = (double)(s * s * s);
-  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  const size_t N = (size_t)d.ps;
   const size_t out0 = (size_t)blockIdx.x * chunk_cells;
   int written = 0;
   for (int r = e0; r < e1; r += blockDim.x) {
@@ -381,7 +392,7 @@ __global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ par
     const int pos = round_position(cnt != 0, lds_waves, &tot);
     if (cnt) {
       const size_t o = out0 + written + pos;
-      ent_cell[o] = cell;
+      ent_cell[o] = cell + cell_base;
       ent_loc[o] = (uint32_t)lx | ((uint32_t)ly << kLocBits) | ((uint32_t)lz << (2 * kLocBits));
       ent_cnt[o] = (uint16_t)cnt;
     }
@@ -445,29 +456,37 @@ constexpr int64_t kMomMaxBlocks = 2048;
 
 template <typename TS, bool DEV, bool PULL, bool HAVE_STATE, typename TI>
 __global__ __launch_bounds__(kMomThreads) void k_moments(const TS* __restrict__ st, const uint32_t* __restrict__ csum,
-                                                         const TI* __restrict__ imb, Dims d, double s3, double hfx,
-                                                         double hfy, double hfz, double* __restrict__ rho_u,
-                                                         double* __restrict__ partials) {
+                                                         const TI* __restrict__ imb, Dims d, double s3, int afield,
+                                                         double hfx, double hfy, double hfz,
+                                                         double* __restrict__ rho_u, double* __restrict__ partials) {
   __shared__ double lds[PSC_NSUM * 16];
-  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  const size_t N = (size_t)d.ps;                            // plane stride
+  const size_t ncell = (size_t)d.nx * d.ny * d.pz;          // cells incl. row padding
   double acc[PSC_NSUM];
 #pragma unroll
   for (int k = 0; k < PSC_NSUM; ++k) acc[k] = 0.0;
-  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t cnt = csum[c];
-    const double asum = (double)cnt / s3;
-    const double solid = asum < 1.0 ? asum : 1.0;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
+    if (d.pz != d.nz && (int)(c % d.pz) >= d.nz) continue;  // row padding
+    // afield: 0 sub-sample counts, 1 FP32 / 2 FP64 solid-fraction field
+    double solid;
+    if (afield == 0) {
+      const double asum = (double)csum[c] / s3;
+      solid = asum < 1.0 ? asum : 1.0;
+    } else {
+      const double a = afield == 1 ? (double)((const float*)csum)[c] : ((const double*)csum)[c];
+      solid = a < 1.0 ? a : 1.0;
+    }
     const double fl = 1.0 - solid;
     acc[0] += solid;
     acc[1] += fl;
-    if (imb != nullptr && cnt != 0) {
+    if (imb != nullptr && solid != 0.0) {
       acc[8] += (double)imb[c];
       acc[9] += (double)imb[N + c];
       acc[10] += (double)imb[2 * N + c];
     }
     if (HAVE_STATE) {
-      const int iz = (int)(c % d.nz);
-      const size_t t = c / d.nz;
+      const int iz = (int)(c % d.pz);
+      const size_t t = c / d.pz;
       const int iy = (int)(t % d.ny);
       const int ix = (int)(t / d.ny);
       double f[Q];
@@ -538,10 +557,10 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
 // ---------------------------------------------------------------------------
 template <typename TS, bool DEV, bool PULL>
 __global__ void k_export(const TS* __restrict__ st, Dims d, double* __restrict__ fout) {
-  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  const size_t N = (size_t)d.ps;
   for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
-    const int iz = (int)(c % d.nz);
-    const size_t t = c / d.nz;
+    const int iz = (int)(c % d.pz);
+    const size_t t = c / d.pz;
     const int iy = (int)(t % d.ny);
     const int ix = (int)(t / d.ny);
 #pragma unroll
@@ -557,10 +576,10 @@ __global__ void k_export(const TS* __restrict__ st, Dims d, double* __restrict__
 
 template <typename TS, bool DEV, bool PULL>
 __global__ void k_import(const double* __restrict__ fin, Dims d, TS* __restrict__ st) {
-  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  const size_t N = (size_t)d.ps;
   for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
-    const int iz = (int)(c % d.nz);
-    const size_t t = c / d.nz;
+    const int iz = (int)(c % d.pz);
+    const size_t t = c / d.pz;
     const int iy = (int)(t % d.ny);
     const int ix = (int)(t / d.ny);
 #pragma unroll
@@ -595,10 +614,10 @@ int grid_for(size_t n, int threads, int cap = 8192) {
 int launch_step_f64(const double* in, double* out, const uint32_t* csum, const double* usn, double* imb,
                     int layout, int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz,
                     int bz, int by, hipStream_t stream) {
-  const Dims d{nx, ny, nz};
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
   const dim3 block(bz, by, 1);
   const dim3 grid((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
-  const double s3 = (double)(s * s * s);
+  const double s3 = (double)(s * s * s);  // 0: csum is an FP64 solid-fraction field
   const double pref = 1.0 - 0.5 / tau;
   const bool force = (fx != 0.0 || fy != 0.0 || fz != 0.0);
   const bool moving = usn != nullptr;
@@ -630,19 +649,20 @@ int launch_step_f64(const double* in, double* out, const uint32_t* csum, const d
 }
 
 int launch_voxelize(const double* parts, const int32_t* boxes, const int32_t* chunks, int nchunks, int chunk_cells,
-                    int nx, int ny, int nz, int s, uint32_t* ent_cell, uint32_t* ent_loc, uint16_t* ent_cnt,
-                    int32_t* chunk_nnz, uint32_t* csum, void* usn, int usp, hipStream_t stream) {
+                    int nx, int ny, int nz, int pz, int s, uint32_t* ent_cell, uint32_t* ent_loc, uint16_t* ent_cnt,
+                    int32_t* chunk_nnz, uint32_t* csum, void* usn, int usp, uint32_t cell_base,
+                    hipStream_t stream) {
   if (nchunks == 0) return PSC_OK;
-  const Dims d{nx, ny, nz};
+  const Dims d{nx, ny, nz, pz, plane_cells(nx, ny, pz)};
   if (usn == nullptr || usp == 0)
     k_voxelize<0><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
-                                               chunk_nnz, csum, nullptr);
+                                               chunk_nnz, csum, nullptr, cell_base);
   else if (usp == 1)
     k_voxelize<1><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
-                                               chunk_nnz, csum, usn);
+                                               chunk_nnz, csum, usn, cell_base);
   else
     k_voxelize<2><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
-                                               chunk_nnz, csum, usn);
+                                               chunk_nnz, csum, usn, cell_base);
   return check_launch("psc_voxelize");
 }
 
@@ -651,7 +671,7 @@ int launch_loads(const double* parts, const int32_t* boxes, const int32_t* chunk
                  const uint16_t* ent_cnt, const int32_t* chunk_nnz, const uint32_t* csum, const void* imb, int imbp,
                  int nx, int ny, int nz, double* partials, double* loads, hipStream_t stream) {
   if (np == 0) return PSC_OK;
-  const Dims d{nx, ny, nz};
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
   const LoadsArgs la{parts, boxes, chunks, chunk_first, np, nchunks, chunk_cells, ent_cell, ent_loc, ent_cnt,
                      chunk_nnz, csum};
   if (nchunks > 0) {
@@ -669,7 +689,7 @@ int launch_loads(const double* parts, const int32_t* boxes, const int32_t* chunk
 int launch_loads_chunks(const LoadsArgs& la, const void* imb, int imbp, int nx, int ny, int nz, double* partials,
                         hipStream_t stream) {
   if (la.nchunks == 0) return PSC_OK;
-  const Dims d{nx, ny, nz};
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
   if (imbp == 2) k_loads_chunk<double><<<la.nchunks, 256, 0, stream>>>(la, (const double*)imb, d, partials);
   else k_loads_chunk<float><<<la.nchunks, 256, 0, stream>>>(la, (const float*)imb, d, partials);
   return check_launch("psc_particle_loads(chunks)");
@@ -702,37 +722,39 @@ int64_t moments_blocks(int64_t n) {
 }
 
 int launch_moments(const void* st, int storage, int layout, const uint32_t* csum, const void* imb, int nx, int ny,
-                   int nz, int s, double fx, double fy, double fz, double* rho_u, double* partials, double* sums,
-                   hipStream_t stream) {
-  const Dims d{nx, ny, nz};
-  const size_t N = (size_t)nx * ny * nz;
+                   int nz, int pz, unsigned long long ps, int s, double fx, double fy, double fz, double* rho_u,
+                   double* partials, double* sums, hipStream_t stream) {
+  const Dims d{nx, ny, nz, pz, ps};
+  const size_t N = (size_t)nx * ny * pz;
   const int nblk = (int)moments_blocks((int64_t)N);
+  // s == 0: csum is a solid-fraction field in the storage precision
   const double s3 = (double)(s * s * s);
+  const int af = s == 0 ? (storage == PSC_F32_DEV ? 1 : 2) : 0;
   const double hx = 0.5 * fx, hy = 0.5 * fy, hz = 0.5 * fz;
   const bool pull = layout == PSC_PULL;
   if (storage == PSC_F32_DEV) {
     const float* g = (const float*)st;
     const float* im = (const float*)imb;
     if (g == nullptr)
-      k_moments<float, true, false, false, float><<<nblk, kMomThreads, 0, stream>>>(g, csum, im, d, s3, hx, hy, hz,
+      k_moments<float, true, false, false, float><<<nblk, kMomThreads, 0, stream>>>(g, csum, im, d, s3, af, hx, hy, hz,
                                                                                     rho_u, partials);
     else if (pull)
-      k_moments<float, true, true, true, float><<<nblk, kMomThreads, 0, stream>>>(g, csum, im, d, s3, hx, hy, hz,
+      k_moments<float, true, true, true, float><<<nblk, kMomThreads, 0, stream>>>(g, csum, im, d, s3, af, hx, hy, hz,
                                                                                   rho_u, partials);
     else
-      k_moments<float, true, false, true, float><<<nblk, kMomThreads, 0, stream>>>(g, csum, im, d, s3, hx, hy, hz,
+      k_moments<float, true, false, true, float><<<nblk, kMomThreads, 0, stream>>>(g, csum, im, d, s3, af, hx, hy, hz,
                                                                                    rho_u, partials);
   } else {
     const double* f = (const double*)st;
     const double* im = (const double*)imb;
     if (f == nullptr)
-      k_moments<double, false, false, false, double><<<nblk, kMomThreads, 0, stream>>>(f, csum, im, d, s3, hx, hy,
+      k_moments<double, false, false, false, double><<<nblk, kMomThreads, 0, stream>>>(f, csum, im, d, s3, af, hx, hy,
                                                                                        hz, rho_u, partials);
     else if (pull)
-      k_moments<double, false, true, true, double><<<nblk, kMomThreads, 0, stream>>>(f, csum, im, d, s3, hx, hy, hz,
+      k_moments<double, false, true, true, double><<<nblk, kMomThreads, 0, stream>>>(f, csum, im, d, s3, af, hx, hy, hz,
                                                                                      rho_u, partials);
     else
-      k_moments<double, false, false, true, double><<<nblk, kMomThreads, 0, stream>>>(f, csum, im, d, s3, hx, hy,
+      k_moments<double, false, false, true, double><<<nblk, kMomThreads, 0, stream>>>(f, csum, im, d, s3, af, hx, hy,
                                                                                       hz, rho_u, partials);
   }
   int rc = check_launch("psc_moments");
@@ -742,7 +764,7 @@ int launch_moments(const void* st, int storage, int layout, const uint32_t* csum
 }
 
 int launch_export(const void* st, int storage, int layout, int nx, int ny, int nz, double* f, hipStream_t stream) {
-  const Dims d{nx, ny, nz};
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
   const size_t N = (size_t)nx * ny * nz;
   const int g = grid_for(N, 256);
   const bool pull = layout == PSC_PULL;
@@ -757,7 +779,7 @@ int launch_export(const void* st, int storage, int layout, int nx, int ny, int n
 }
 
 int launch_import(const double* f, int storage, int layout, int nx, int ny, int nz, void* st, hipStream_t stream) {
-  const Dims d{nx, ny, nz};
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
   const size_t N = (size_t)nx * ny * nz;
   const int g = grid_for(N, 256);
   const bool pull = layout == PSC_PULL;

@@ -40,10 +40,12 @@ __device__ __forceinline__ float k_lin(int i) {
 
 struct Dims {
   int nx, ny, nz;
+  int pz;              // row pitch along z (cells); >= nz, padding cells are never used
+  unsigned long long ps;  // cells per direction plane (>= nx*ny*pz)
 };
 
 __device__ __forceinline__ uint32_t cell_id(const Dims d, int x, int y, int z) {
-  return ((uint32_t)x * (uint32_t)d.ny + (uint32_t)y) * (uint32_t)d.nz + (uint32_t)z;
+  return ((uint32_t)x * (uint32_t)d.ny + (uint32_t)y) * (uint32_t)d.pz + (uint32_t)z;
 }
 
 // Load / store of one FP32 value at byte offset `off` of a direction plane.
@@ -64,14 +66,15 @@ __device__ __forceinline__ void st(char* __restrict__ base, uint32_t off, float 
 
 struct StepConsts {
   float inv_s3, tau, itau, omt, pref, Fx, Fy, Fz;
+  int field;  // 1: the 4-byte solid input holds alpha as an FP32 field instead of a sub-sample count
 };
 
 template <bool PULL, bool MOVING, bool IMB, bool NT>
 __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* __restrict__ gout,
                                           const uint32_t* __restrict__ csum, const float* __restrict__ usn,
-                                          float* __restrict__ imb, const Dims d, const StepConsts& k, int ix, int iy,
-                                          int iz) {
-  const size_t plane = (size_t)d.nx * d.ny * d.nz * sizeof(float);  // bytes per direction plane
+                                          float* __restrict__ imb, size_t imb_stride, const Dims d,
+                                          const StepConsts& k, int ix, int iy, int iz) {
+  const size_t plane = (size_t)d.ps * sizeof(float);  // bytes per direction plane
   const int xm = wrap_step(ix, -1, d.nx), xp = wrap_step(ix, 1, d.nx);
   const int ym = wrap_step(iy, -1, d.ny), yp = wrap_step(iy, 1, d.ny);
   const int zm = wrap_step(iz, -1, d.nz), zp = wrap_step(iz, 1, d.nz);
@@ -139,13 +142,13 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
       gn[o] = fmaf(g[o], k.omt, P - M);
     }
   } else {
-    const float a = fminf((float)cnt * k.inv_s3, 1.0f);
+    const float a = k.field ? __uint_as_float(cnt) : fminf((float)cnt * k.inv_s3, 1.0f);
     const float B = a / ((a + k.tau * (1.0f - a)) + 1.0e-14f);
     const float omb = (1.0f - B) * k.itau;
     float sx = 0.f, sy = 0.f, sz = 0.f, ssq15 = 0.f;
     if (MOVING) {
       const uint32_t c = cell_id(d, ix, iy, iz);
-      const size_t N = (size_t)d.nx * d.ny * d.nz;
+      const size_t N = (size_t)d.ps;
       const float ia = 1.0f / ((float)cnt * k.inv_s3);
       sx = usn[c] * ia;
       sy = usn[N + c] * ia;
@@ -188,10 +191,9 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
     }
     if (IMB) {
       const uint32_t c = cell_id(d, ix, iy, iz);
-      const size_t N = (size_t)d.nx * d.ny * d.nz;
       imb[c] = mx;
-      imb[N + c] = my;
-      imb[2 * N + c] = mz;
+      imb[imb_stride + c] = my;
+      imb[2 * imb_stride + c] = mz;
     }
   }
 #pragma unroll
@@ -215,7 +217,63 @@ __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin,
 #pragma unroll
   for (int c = 0; c < CPT; ++c) {
     const int iy = iy0 + c;
-    if (iy < d.ny) step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, usn, imb, d, k, ix, iy, iz);
+    if (iy < d.ny)
+      step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, usn, imb, (size_t)d.nx * d.ny * d.nz, d, k, ix, iy, iz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched windows: one persistent launch steps many independent periodic
+// lattices.  Window w's state is a contiguous block of 19 planes of N_w
+// cells at element offset state_off; its count_sum and imb cells sit at
+// cell_off in arrays shared by all windows (imb plane stride = total cells).
+// Work items are spans of up to 256 consecutive cells of one window, so the
+// window parameters are uniform across a workgroup; workgroups loop over the
+// items with a grid stride.
+// ---------------------------------------------------------------------------
+struct BatchWindow {
+  long long state_off;  // elements
+  long long cell_off;   // cells
+  int nx, ny, nz, pz;   // extents and z row pitch (pz >= nz)
+  float inv_pz, inv_nypz;  // reciprocals for the index decomposition
+  float fx, fy, fz, pad2;
+  long long ps;         // cells per direction plane (multiple of 64: 256-byte aligned planes)
+};
+
+// q = n / d, r = n % d for n < 2^24 from an FP32 reciprocal, corrected.
+__device__ __forceinline__ void divmod_f(uint32_t n, uint32_t d, float inv, uint32_t& q, uint32_t& r) {
+  q = (uint32_t)((float)n * inv);
+  int rr = (int)n - (int)(q * d);
+  while (rr < 0) { --q; rr += (int)d; }
+  while (rr >= (int)d) { ++q; rr -= (int)d; }
+  r = (uint32_t)rr;
+}
+
+template <bool IMB, bool NT>
+__global__ __launch_bounds__(256) void k_step_f32_batched(const float* __restrict__ gin, float* __restrict__ gout,
+                                                          const uint32_t* __restrict__ csum,
+                                                          float* __restrict__ imb, size_t imb_stride,
+                                                          const BatchWindow* __restrict__ wins,
+                                                          const int2* __restrict__ items, int nitems,
+                                                          StepConsts k0) {
+  const int tid = threadIdx.x;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int2 item = items[it];  // (window, first cell)
+    const BatchWindow w = wins[item.x];
+    const uint32_t c = (uint32_t)(item.y + tid);
+    const uint32_t Nw = (uint32_t)w.nx * w.ny * w.pz;
+    if (c >= Nw) continue;
+    uint32_t ix, rem, iy, iz;
+    divmod_f(c, (uint32_t)(w.ny * w.pz), w.inv_nypz, ix, rem);
+    divmod_f(rem, (uint32_t)w.pz, w.inv_pz, iy, iz);
+    if (iz >= (uint32_t)w.nz) continue;  // row padding
+    StepConsts k = k0;
+    k.Fx = w.fx;
+    k.Fy = w.fy;
+    k.Fz = w.fz;
+    const Dims d{w.nx, w.ny, w.nz, w.pz, (unsigned long long)w.ps};
+    step_cell<true, false, IMB, NT>(gin + w.state_off, gout + w.state_off, csum + w.cell_off, nullptr,
+                                    IMB ? imb + w.cell_off : nullptr, imb_stride, d, k, (int)ix, (int)iy, (int)iz);
   }
 }
 
@@ -227,7 +285,7 @@ constexpr size_t kLdsHeadroom = 8192;
 int launch_step_f32(const float* in, float* out, const uint32_t* csum, const float* usn, float* imb, int layout,
                     int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz, int bz, int by,
                     int nt, hipStream_t stream) {
-  const Dims d{nx, ny, nz};
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
   const int cpt = (nt >> 16) & 3 ? (nt >> 16) & 3 : 1;
   const int bpc = (nt >> 8) & 0xff;
   size_t lds = 0;
@@ -242,7 +300,8 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
   if ((size_t)nx * ny * nz * sizeof(float) > 0xffffffffull)
     return set_error(PSC_EINVAL, "psc_step(f32): a direction plane must stay below 4 GiB (N < 2^30 cells)");
   StepConsts k;
-  k.inv_s3 = (float)(1.0 / (double)(s * s * s));
+  k.field = s == 0;
+  k.inv_s3 = k.field ? 0.0f : (float)(1.0 / (double)(s * s * s));
   k.tau = (float)tau;
   k.itau = (float)(1.0 / tau);
   k.omt = (float)(1.0 - 1.0 / tau);
@@ -276,6 +335,40 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
 #undef PSC_F32_CASES
 #undef PSC_F32_CASE
   return set_error(PSC_EINVAL, "psc_step(f32): no kernel variant");
+}
+
+// Batched launch: win/items are device tables (see BatchWindow); grid_blocks
+// = 0 sizes the persistent grid to the item count.
+int launch_step_f32_batched(const float* in, float* out, const uint32_t* csum, float* imb, size_t imb_stride,
+                            const void* wins, const void* items, int nitems, int s, double tau, int flags,
+                            int grid_blocks, hipStream_t stream) {
+  if (nitems <= 0) return PSC_OK;
+  const int bpc = (flags >> 8) & 0xff;
+  size_t lds = 0;
+  if (bpc > 0) {
+    lds = (kLdsPerCU - kLdsHeadroom) / (size_t)bpc;
+    if (lds > 65536) lds = 65536;
+  }
+  StepConsts k;
+  k.field = 0;
+  k.inv_s3 = (float)(1.0 / (double)(s * s * s));
+  k.tau = (float)tau;
+  k.itau = (float)(1.0 / tau);
+  k.omt = (float)(1.0 - 1.0 / tau);
+  k.pref = (float)(1.0 - 0.5 / tau);
+  k.Fx = k.Fy = k.Fz = 0.0f;
+  const int grid = (grid_blocks > 0 && grid_blocks < nitems) ? grid_blocks : nitems;
+  const BatchWindow* w = (const BatchWindow*)wins;
+  const int2* it = (const int2*)items;
+  const bool nt = (flags & 1) != 0;
+  if (imb != nullptr) {
+    if (nt) k_step_f32_batched<true, true><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k);
+    else k_step_f32_batched<true, false><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k);
+  } else {
+    if (nt) k_step_f32_batched<false, true><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k);
+    else k_step_f32_batched<false, false><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k);
+  }
+  return check_launch("psc_step_batched");
 }
 
 }  // namespace psc

@@ -307,8 +307,8 @@ class PscD3Q19:
         nx, ny, nz = self._shape
         ec, el, en = self._ent
         _lib.call("psc_voxelize", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), C, tabs.chunk_cells,
-                  nx, ny, nz, self._s, self._ptr(ec), self._ptr(el), self._ptr(en), self._ptr(self._chunk_nnz),
-                  self._ptr(self._csum), us_ptr, usp, stream)
+                  nx, ny, nz, 0, self._s, self._ptr(ec), self._ptr(el), self._ptr(en), self._ptr(self._chunk_nnz),
+                  self._ptr(self._csum), us_ptr, usp, 0, stream)
         self._tables = tabs
         # keep the device tables alive until the kernels that read them have run
         self._dev_tables = (parts, boxes, chunks, cfirst)
@@ -447,7 +447,7 @@ class PscD3Q19:
             self._rho_u = torch.empty(4 * self._N, dtype=torch.float64, device=self.device)
         fx, fy, fz = self._force
         _lib.call("psc_moments", self._ptr(self._a) if with_state else None, self._storage, self._layout,
-                  self._ptr(self._csum), self._ptr(self._imb), nx, ny, nz, self._s, fx, fy, fz,
+                  self._ptr(self._csum), self._ptr(self._imb), nx, ny, nz, 0, self._s, fx, fy, fz,
                   self._ptr(self._rho_u) if fields else None, self._ptr(self._mom_partials), self._ptr(self._sums),
                   self._stream())
         return self._np(self._sums)

@@ -84,3 +84,100 @@ def moving_window(seed: int = 0, n: int = 512, count: int = 2000, rmin: float = 
 def advance_centres(table: np.ndarray, n: int, dt: float = 1.0) -> None:
     """x += U dt for every sphere, wrapped into [0, n) (periodic window)."""
     table[:, 0:3] = np.mod(table[:, 0:3] + dt * table[:, 4:7], float(n))
+
+
+# ---------------------------------------------------------------------------
+# Batched pore windows: pebble skeleton + fines (skeleton / passive / preloaded)
+# ---------------------------------------------------------------------------
+VARIANTS = ("skeleton", "passive", "preloaded")
+
+
+def _min_image(d, L):
+    return d - L * np.round(d / L)
+
+
+def pebble_packing(rng, L: int, radius: float = 16.0, solid: float = 0.6, iters: int = 600, tol: float = 0.05,
+                   count: int | None = None):
+    """Periodic pebble packing of an L^3 window near the requested solid
+    fraction (or of exactly ``count`` pebbles): random centres, then iterative
+    pairwise overlap relaxation (minimum image) until the largest overlap is
+    below tol*radius or the iteration budget ends (dense packings near 0.6
+    keep small residual overlaps; the measured porosity is reported by the
+    solver)."""
+    n = int(count) if count is not None else max(1, int(round(solid * L ** 3 / (4.0 / 3.0 * np.pi * radius ** 3))))
+    x = rng.uniform(0.0, L, size=(n, 3))
+    if n == 1:
+        return x
+    for _ in range(iters):
+        d = _min_image(x[:, None, :] - x[None, :, :], L)
+        dist = np.sqrt((d * d).sum(-1)) + np.eye(n) * 1e9
+        ov = np.maximum(2.0 * radius - dist, 0.0)
+        if ov.max() < tol * radius:
+            break
+        push = (ov / dist)[:, :, None] * d
+        x = np.mod(x + 0.5 * push.sum(axis=1), L)
+    return x
+
+
+def place_fines(rng, L, pebbles, radius, count, variant, max_tries=40000, rclip=(2.0, 5.0)):
+    """Fines touching a pebble and lying in a throat (close to a second
+    pebble).  passive: no overlap with pebbles or other fines; preloaded:
+    overlaps up to 30 % of the fine radius allowed (contacts under load).
+    Radii: lognormal, median 3, sigma 0.25, clipped to ``rclip``."""
+    out = []
+    if variant == "skeleton" or count == 0 or len(pebbles) == 0:
+        return np.zeros((0, 4))
+    tries = 0
+    while len(out) < count and tries < max_tries:
+        tries += 1
+        rf = float(np.clip(rng.lognormal(np.log(3.0), 0.25), rclip[0], rclip[1]))
+        i = rng.integers(len(pebbles))
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        c = np.mod(pebbles[i] + (radius + rf) * u, L)
+        dp = np.sqrt((_min_image(pebbles - c, L) ** 2).sum(-1)) - (radius + rf)
+        dp[i] = np.inf
+        slack = 0.3 * rf if variant == "preloaded" else 0.0
+        if dp.min() < -slack or dp.min() > (rf if variant == "preloaded" else 2.0 * rf):
+            continue  # overlapping a pebble, or not in a throat
+        if out:
+            f = np.array(out)
+            df = np.sqrt((_min_image(f[:, :3] - c, L) ** 2).sum(-1)) - (f[:, 3] + rf)
+            if df.min() < -slack:
+                continue
+        out.append([c[0], c[1], c[2], rf])
+    return np.array(out).reshape(-1, 4)
+
+
+def pebble_windows(seed: int = 0, count: int = 64, lmin: int = 48, lmax: int = 160, radius: float = 16.0,
+                   max_fines: int = 400):
+    """The batched-window workload: ``count`` periodic windows with edge
+    L ~ U{lmin..lmax}, pebbles of ``radius`` near solid fraction 0.6 and
+    U{0..max_fines} fines (lognormal radius, median 3, sigma 0.25, clipped to
+    [2, 5]) placed in throats; variants cycle skeleton / passive / preloaded.
+    Returns a list of (L, table, variant)."""
+    rng = np.random.default_rng(seed)
+    wins = []
+    for w in range(count):
+        L = int(rng.integers(lmin, lmax + 1))
+        variant = VARIANTS[w % 3]
+        peb = pebble_packing(rng, L, radius)
+        nf = int(rng.integers(0, max_fines + 1))
+        fines = place_fines(rng, L, peb, radius, nf, variant)
+        cen = np.vstack([peb, fines[:, :3]]) if len(fines) else peb
+        rad = np.concatenate([np.full(len(peb), radius), fines[:, 3]]) if len(fines) else np.full(len(peb), radius)
+        wins.append((L, _table(cen, rad), variant))
+    return wins
+
+
+def adjoint_window(seed: int = 0, n: int = 96, pebbles: int = 40, radius: float = 14.0, fines: int = 300):
+    """The permeability-gradient workload: a periodic n^3 window with
+    ``pebbles`` pebbles of ``radius`` (overlap-relaxed packing) and ``fines``
+    passive throat fines with lognormal radius (median 3, sigma 0.25)
+    clipped to [2, 4].  Returns the (P, 10) table, pebbles first."""
+    rng = np.random.default_rng(seed)
+    peb = pebble_packing(rng, n, radius, count=pebbles)
+    fn = place_fines(rng, n, peb, radius, fines, "passive", max_tries=200000, rclip=(2.0, 4.0))
+    cen = np.vstack([peb, fn[:, :3]]) if len(fn) else peb
+    rad = np.concatenate([np.full(len(peb), radius), fn[:, 3]])
+    return _table(cen, rad)

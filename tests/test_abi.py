@@ -61,9 +61,9 @@ def test_argument_errors_are_reported(lib):
     rc = h.psc_step(ct.c_void_p(16), ct.c_void_p(32), ct.c_void_p(64), None, None, 1, 1, 8, 8, 8, 2, 0.8, 0, 0, 0,
                     48, 3, 0, None)
     assert rc == -2
-    rc = h.psc_voxelize(None, None, None, 0, 2048, 8, 8, 8, 41, None, None, None, None, None, None, 0, None)
+    rc = h.psc_voxelize(None, None, None, 0, 2048, 8, 8, 8, 0, 41, None, None, None, None, None, None, 0, 0, None)
     assert rc == -1 and b"subsamples" in h.psc_last_error()
-    rc = h.psc_moments(None, 3, 0, None, None, 8, 8, 8, 2, 0, 0, 0, None, None, None, None)
+    rc = h.psc_moments(None, 3, 0, None, None, 8, 8, 8, 0, 2, 0, 0, 0, None, None, None, None)
     assert rc == -1
     rc = h.psc_clear(None, 0, None)
     assert rc == 0

@@ -324,10 +324,14 @@ int psc_adjoint_seed(const void* d_state, const void* d_alpha, void* d_lam, doub
  * d_galpha in every cell with 0 < alpha < 1 (the only cells where a smooth
  * solid fraction depends on the particles).  Storage precision as for
  * psc_step (FP32: deviations, adjoint in FP32; FP64: plain, adjoint in
- * FP64); d_alpha as for PSC_ALPHA_FIELD.  flags: PSC_STEP_NONTEMPORAL. */
+ * FP64); d_alpha as for PSC_ALPHA_FIELD.  block_z/block_y: workgroup
+ * shape (0, 0 = 64 x 4, or 32 x 8 for rows that are a multiple of 32 but
+ * not of 64); flags: PSC_STEP_NONTEMPORAL and the workgroups-per-CU cap
+ * (bits 8-15) of psc_step. */
 int psc_adjoint_step(const void* d_state, const void* d_lam_in, void* d_lam_out, const void* d_alpha,
                      double* d_galpha, int32_t storage, int32_t nx, int32_t ny, int32_t nz,
-                     double tau, double fx, double fy, double fz, int32_t flags, void* stream);
+                     double tau, double fx, double fy, double fz, int32_t block_z, int32_t block_y,
+                     int32_t flags, void* stream);
 
 #ifdef __cplusplus
 }

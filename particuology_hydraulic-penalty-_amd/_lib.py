@@ -51,7 +51,8 @@ SIGNATURES = {
     "psc_smooth_alpha_grad": ([_P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P],
                               ct.c_int),
     "psc_adjoint_seed": ([_P, _P, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _D, _P], ct.c_int),
-    "psc_adjoint_step": ([_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _P], ct.c_int),
+    "psc_adjoint_step": ([_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _I32, _P],
+                         ct.c_int),
 }
 
 

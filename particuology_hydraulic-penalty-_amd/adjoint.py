@@ -122,7 +122,7 @@ class PermeabilityAdjoint:
 
     def __init__(self, shape, tau: float = 0.8, body_force=(0.0, 0.0, 1.0e-6), steps: int = 2000,
                  precision: str = "fp32", checkpoint_every: int | None = None, device=None, chunk_cells: int = 2048,
-                 nontemporal: bool = True, graph: bool = True):
+                 nontemporal: bool = True, graph: bool = True, tune: bool = True):
         torch = _torch()
         _lib.load()
         if not torch.cuda.is_available():
@@ -149,6 +149,11 @@ class PermeabilityAdjoint:
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.chunk_cells = chunk_cells
         self.flags = _lib.PSC_STEP_NONTEMPORAL if (nontemporal and precision == "fp32") else 0
+        # (block_z, block_y, flags) of psc_step and psc_adjoint_step; (0, 0) = the library default
+        # shape; tune=True times the candidates on the first forward()
+        self.step_launch = (0, 0, self.flags)
+        self.adjoint_launch = (0, 0, self.flags)
+        self._tuned = not tune
         self.graph = bool(graph)
         nx, ny, nz = self.shape
         self.N = nx * ny * nz
@@ -179,14 +184,16 @@ class PermeabilityAdjoint:
     def _step(self, src, dst, s):
         nx, ny, nz = self.shape
         fx, fy, fz = self.body_force
+        bz, by, fl = self.step_launch
         _lib.call("psc_step", src.data_ptr(), dst.data_ptr(), self._alpha_ptr(), None, None, self.storage,
-                  _lib.PSC_PULL, nx, ny, nz, _lib.PSC_ALPHA_FIELD, self.tau, fx, fy, fz, 0, 0, self.flags, s)
+                  _lib.PSC_PULL, nx, ny, nz, _lib.PSC_ALPHA_FIELD, self.tau, fx, fy, fz, bz, by, fl, s)
 
     def _reverse(self, state, lam_in, lam_out, s):
         nx, ny, nz = self.shape
         fx, fy, fz = self.body_force
+        bz, by, fl = self.adjoint_launch
         _lib.call("psc_adjoint_step", state.data_ptr(), lam_in.data_ptr(), lam_out.data_ptr(), self._alpha_ptr(),
-                  self._galpha.data_ptr(), self.storage, nx, ny, nz, self.tau, fx, fy, fz, self.flags, s)
+                  self._galpha.data_ptr(), self.storage, nx, ny, nz, self.tau, fx, fy, fz, bz, by, fl, s)
 
     def _copy(self, src, dst, s):
         _lib.call("psc_copy_f32", src.data_ptr(), dst.data_ptr(), self._words, s)
@@ -200,8 +207,72 @@ class PermeabilityAdjoint:
         p.nx, p.ny, p.nz, p.subsamples = nx, ny, nz, _lib.PSC_ALPHA_FIELD
         p.tau = self.tau
         p.fx, p.fy, p.fz = self.body_force
-        p.flags = self.flags
+        p.block_z, p.block_y, p.flags = self.step_launch
         return p
+
+    def _launch_candidates(self):
+        """(block_z, block_y) shapes to try: row tiles of 64 and 128 and, when
+        a row fits one workgroup, the row itself rounded to 32 cells."""
+        nz = self.shape[2]
+        shapes = [(64, 4), (64, 2), (128, 2), (256, 1)]
+        r = -(-nz // 32) * 32
+        if r <= 256:
+            for by in (8, 4, 2, 1):
+                if r * by <= 256 and (r * by) % 64 == 0:
+                    shapes.append((r, by))
+                    break
+        out = []
+        for bz, by in shapes:
+            if bz > 64 and nz <= bz // 2:
+                continue  # more than half of every row tile would idle
+            if (bz, by) not in out:
+                out.append((bz, by))
+        return out
+
+    def _tune(self, s):
+        """Time every candidate launch of the forward step and of the reverse
+        step (HIP events, 5 launches each on scratch buffers) and keep the
+        fastest; results are bit-identical across launches."""
+        torch = _torch()
+        if self._tuned:
+            return
+        nx, ny, nz = self.shape
+        fx, fy, fz = self.body_force
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        stream = torch.cuda.current_stream(self.device)
+
+        def timed(fn):
+            fn()
+            ev0.record(stream)
+            for _ in range(5):
+                fn()
+            ev1.record(stream)
+            ev1.synchronize()
+            return ev0.elapsed_time(ev1)
+
+        caps = (0, 4, 6)
+        nts = (0, 1) if self.precision == "fp32" else (0,)
+        _lib.call("psc_init_state", self._tmp[0].data_ptr(), self.storage, self.N, s)
+        _lib.call("psc_init_state", self._lam[0].data_ptr(), self.storage, self.N, s)
+        best_f = best_a = None
+        for bz, by in self._launch_candidates():
+            for nt in nts:
+                for cap in caps:
+                    fl = nt | (cap << 8)
+                    ms = timed(lambda: _lib.call("psc_step", self._tmp[0].data_ptr(), self._tmp[1].data_ptr(),
+                                                 self._alpha_ptr(), None, None, self.storage, _lib.PSC_PULL, nx, ny,
+                                                 nz, _lib.PSC_ALPHA_FIELD, self.tau, fx, fy, fz, bz, by,
+                                                 fl | (1 << 16), s))
+                    if best_f is None or ms < best_f[0]:
+                        best_f = (ms, bz, by, fl | (1 << 16))
+                    self.adjoint_launch = (bz, by, fl)
+                    ms = timed(lambda: self._reverse(self._tmp[0], self._lam[0], self._lam[1], s))
+                    if best_a is None or ms < best_a[0]:
+                        best_a = (ms, bz, by, fl)
+        self.step_launch = best_f[1:]
+        self.adjoint_launch = best_a[1:]
+        self._tuned = True
 
     def _segment_bounds(self, c):
         t0 = c * self.K
@@ -229,6 +300,7 @@ class PermeabilityAdjoint:
         self.geometry = SmoothGeometry(pack_particles(table), self.shape, self.device, self.chunk_cells)
         self.geometry.build(s)
         self._graphs = None  # captured graphs hold the previous geometry's pointers
+        self._tune(s)
         _lib.call("psc_init_state", self._tmp[0].data_ptr(), self.storage, self.N, s)
         plan = self._plan()
         par = ct.c_int32(0)

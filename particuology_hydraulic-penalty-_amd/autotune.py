@@ -25,7 +25,7 @@ _LOCK = threading.Lock()
 _MEM: dict = {}
 
 # (block_z, block_y, nontemporal, workgroups-per-CU cap; 0 = none)
-SHAPES = [(64, 4), (64, 2), (64, 1), (128, 2), (128, 1), (256, 1)]
+SHAPES = [(64, 4), (64, 2), (64, 1), (96, 2), (128, 2), (128, 1), (192, 1), (256, 1)]
 NT = (0, 1)
 BPC = (0, 2, 3, 4, 6, 8)
 

@@ -63,7 +63,7 @@ int launch_smooth_alpha_grad(const double*, const int32_t*, const int32_t*, cons
 int launch_adjoint_seed(const void*, const void*, void*, double*, int, int, int, int, double, double, double, double,
                         double, hipStream_t);
 int launch_adjoint_step(const void*, const void*, void*, const void*, double*, int, int, int, int, double, double,
-                        double, double, int, hipStream_t);
+                        double, double, int, int, int, hipStream_t);
 
 int launch_export(const void*, int, int, int, int, int, double*, hipStream_t);
 int launch_import(const double*, int, int, int, int, int, void*, hipStream_t);
@@ -209,7 +209,8 @@ int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const v
     return set_error(PSC_EINVAL, "psc_step: a solid-fraction field (subsamples 0) takes no solid velocity");
   if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_step: tau %g must exceed 1/2", tau);
   int bz = block_z, by = block_y;
-  if (bz <= 0) bz = nz >= 64 ? 64 : 32;
+  // default: 64-cell row tiles, 32 when rows are a multiple of 32 but not of 64
+  if (bz <= 0) bz = (nz >= 64 && !(nz % 64 != 0 && nz % 32 == 0)) ? 64 : 32;
   if (by <= 0) by = 256 / bz > 0 ? 256 / bz : 1;
   if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK)) != 0)
     return set_error(PSC_EINVAL, "psc_step: unknown flags 0x%x", flags);
@@ -364,7 +365,7 @@ int psc_adjoint_seed(const void* d_state, const void* d_alpha, void* d_lam, doub
 
 int psc_adjoint_step(const void* d_state, const void* d_lam_in, void* d_lam_out, const void* d_alpha,
                      double* d_galpha, int32_t storage, int32_t nx, int32_t ny, int32_t nz, double tau, double fx,
-                     double fy, double fz, int32_t flags, void* stream) {
+                     double fy, double fz, int32_t block_z, int32_t block_y, int32_t flags, void* stream) {
   if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_adjoint_step: bad lattice");
   if (storage != PSC_F32_DEV && storage != PSC_F64_PLAIN) return set_error(PSC_EINVAL, "psc_adjoint_step: storage");
   if (d_state == nullptr || d_lam_in == nullptr || d_lam_out == nullptr || d_alpha == nullptr || d_galpha == nullptr)
@@ -372,9 +373,14 @@ int psc_adjoint_step(const void* d_state, const void* d_lam_in, void* d_lam_out,
   if (d_lam_in == d_lam_out || d_state == d_lam_out)
     return set_error(PSC_EINVAL, "psc_adjoint_step: the output adjoint aliases an input");
   if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_adjoint_step: tau %g must exceed 1/2", tau);
-  if ((flags & ~PSC_STEP_NONTEMPORAL) != 0) return set_error(PSC_EINVAL, "psc_adjoint_step: unknown flags 0x%x", flags);
+  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK)) != 0)
+    return set_error(PSC_EINVAL, "psc_adjoint_step: unknown flags 0x%x", flags);
+  if ((block_z > 0) != (block_y > 0) ||
+      (block_z > 0 && (block_z * block_y > 256 || (block_z * block_y) % 64 != 0)))
+    return set_error(PSC_ELAUNCH, "psc_adjoint_step: block %dx%d must hold a multiple of 64 threads, at most 256",
+                     block_z, block_y);
   return launch_adjoint_step(d_state, d_lam_in, d_lam_out, d_alpha, d_galpha, storage, nx, ny, nz, tau, fx, fy, fz,
-                             flags, (hipStream_t)stream);
+                             block_z, block_y, flags, (hipStream_t)stream);
 }
 
 }  // extern "C"

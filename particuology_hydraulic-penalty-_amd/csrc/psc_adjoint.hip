@@ -189,16 +189,25 @@ struct AdjConsts {
   R tau, itau, pref, Fx, Fy, Fz;
 };
 
+// Load / store of one value at byte offset `off` of a direction plane whose
+// base is uniform across the wavefront (SGPR base + 32-bit VGPR offset).
 template <typename R, bool NT>
-__device__ __forceinline__ R ldv(const R* __restrict__ p) {
+__device__ __forceinline__ R ldo(const char* __restrict__ base, uint32_t off) {
+  const R* p = (const R*)(base + off);
   return NT ? __builtin_nontemporal_load(p) : *p;
 }
 template <typename R, bool NT>
-__device__ __forceinline__ void stv(R* __restrict__ p, R v) {
+__device__ __forceinline__ void sto(char* __restrict__ base, uint32_t off, R v) {
+  R* p = (R*)(base + off);
   if (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }
 
+// The populations enter dL/dB only through
+//   sum_i lam_i ((g_opp(i) - g_i) + g_i/tau) = sum_i g_i (lam_opp(i) - (1 - 1/tau) lam_i),
+// which is accumulated while the state is gathered, so the 19 populations
+// are never held at once: the adjoint lam_i and a few sums are all that
+// stay live.
 template <typename R, bool DEV, bool NT>
 __global__ __launch_bounds__(256) void k_adjoint_step(const R* __restrict__ st, const R* __restrict__ lin,
                                                       R* __restrict__ lout, const R* __restrict__ alpha,
@@ -207,25 +216,37 @@ __global__ __launch_bounds__(256) void k_adjoint_step(const R* __restrict__ st, 
   const int iy = blockIdx.y * blockDim.y + threadIdx.y;
   const int ix = blockIdx.z;
   if (iz >= d.nz || iy >= d.ny) return;
-  const size_t N = (size_t)d.nx * d.ny * d.nz;
+  const size_t plane = (size_t)d.nx * d.ny * d.nz * sizeof(R);  // bytes per direction plane
   const uint32_t here = acell(d, ix, iy, iz);
-  auto nb = [&](int i) -> uint32_t {  // cell x - c_i
-    return acell(d, wrap_step(ix, -cx_of(i), d.nx), wrap_step(iy, -cy_of(i), d.ny), wrap_step(iz, -cz_of(i), d.nz));
+  const int xm = wrap_step(ix, -1, d.nx), xp = wrap_step(ix, 1, d.nx);
+  const int ym = wrap_step(iy, -1, d.ny), yp = wrap_step(iy, 1, d.ny);
+  const int zm = wrap_step(iz, -1, d.nz), zp = wrap_step(iz, 1, d.nz);
+  auto X = [&](int s) { return s < 0 ? xm : (s > 0 ? xp : ix); };
+  auto Y = [&](int s) { return s < 0 ? ym : (s > 0 ? yp : iy); };
+  auto Z = [&](int s) { return s < 0 ? zm : (s > 0 ? zp : iz); };
+  // byte offset of the cell x - c_i
+  auto nb = [&](int i) -> uint32_t {
+    return acell(d, X(-cx_of(i)), Y(-cy_of(i)), Z(-cz_of(i))) * (uint32_t)sizeof(R);
   };
-  R g[Q], lam[Q];
+  const uint32_t hoff = here * (uint32_t)sizeof(R);
+  const char* sb = (const char*)st;
+  const char* lb = (const char*)lin;
+  char* ob = (char*)lout;
+
+  R lam[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) lam[i] = ldo<R, NT>(lb + (size_t)i * plane, hoff);
+  const R omt = (R)1 - k.itau;
+  R drho = 0, jx = 0, jy = 0, jz = 0, Gl = 0;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    g[i] = ldv<R, NT>(st + (size_t)i * N + nb(i));
-    if (!DEV) g[i] -= (R)w_of(i);
-    lam[i] = ldv<R, NT>(lin + (size_t)i * N + here);
-  }
-  R drho = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    drho += g[i];
-    if (cx_of(i)) jx += (R)cx_of(i) * g[i];
-    if (cy_of(i)) jy += (R)cy_of(i) * g[i];
-    if (cz_of(i)) jz += (R)cz_of(i) * g[i];
+    R g = ldo<R, NT>(sb + (size_t)i * plane, nb(i));
+    if (!DEV) g -= (R)w_of(i);
+    drho += g;
+    if (cx_of(i)) jx += (R)cx_of(i) * g;
+    if (cy_of(i)) jy += (R)cy_of(i) * g;
+    if (cz_of(i)) jz += (R)cz_of(i) * g;
+    Gl += g * (lam[opp_of(i)] - omt * lam[i]);
   }
   const R rho = (R)1 + drho;
   const R ir = (R)1 / rho;
@@ -261,17 +282,12 @@ __global__ __launch_bounds__(256) void k_adjoint_step(const R* __restrict__ st, 
   const R guy = rho * ((R)3 * V1y + (R)9 * V2y - (R)3 * uy * S0) + k.pref * ((R)9 * LFy - (R)3 * k.Fy * Lw);
   const R guz = rho * ((R)3 * V1z + (R)9 * V2z - (R)3 * uz * S0) + k.pref * ((R)9 * LFz - (R)3 * k.Fz * Lw);
   const R gu_u = gux * ux + guy * uy + guz * uz;
-#pragma unroll
-  for (int kk = 0; kk < Q; ++kk) {
-    const R gc = (R)cx_of(kk) * gux + (R)cy_of(kk) * guy + (R)cz_of(kk) * guz;
-    const R mu = (A * lam[kk] + B * lam[opp_of(kk)]) + (grho + (gc - gu_u) * ir);
-    stv<R, NT>(lout + (size_t)kk * N + nb(kk), mu);
-  }
   if (a > (R)0 && a < (R)1) {
-    // dL/dB = sum_i lam_i (Os_i - Obgk_i) in deviation form: with
-    // geq_i = w_i (drho + rho (3cu + 4.5cu^2 - 1.5u^2)) and geqs_i = w_i drho,
-    // Os_i = (g_opp - g_i) + geqs_i - geq_opp and Obgk_i = (geq_i - g_i)/tau.
-    R dLdB = 0;
+    // dL/dB = sum_i lam_i (Os_i - Obgk_i) = Gl + sum_i lam_i (geqs_i - geq_opp(i) - geq_i/tau)
+    // with geq_i = w_i (drho + rho (3cu + 4.5cu^2 - 1.5u^2)), geqs_i = w_i drho.
+    const R w0 = (R)w_of(0);
+    const R geq0 = w0 * (drho - rho * (R)1.5 * usq);
+    R dLdB = Gl + lam[0] * ((w0 * drho - geq0) - geq0 * k.itau);
 #pragma unroll
     for (int i = 1; i < Q; i += 2) {
       const int o = i + 1;
@@ -281,18 +297,16 @@ __global__ __launch_bounds__(256) void k_adjoint_step(const R* __restrict__ st, 
       const R odd = w * rho * (R)3 * cu;
       const R geq_i = even + odd, geq_o = even - odd;
       const R geqs = w * drho;
-      const R os_i = (g[o] - g[i]) + (geqs - geq_o), os_o = (g[i] - g[o]) + (geqs - geq_i);
-      const R ob_i = (geq_i - g[i]) * k.itau, ob_o = (geq_o - g[o]) * k.itau;
-      dLdB += lam[i] * (os_i - ob_i) + lam[o] * (os_o - ob_o);
-    }
-    {
-      const R w0 = (R)w_of(0);
-      const R geq0 = w0 * (drho - rho * (R)1.5 * usq);
-      const R os0 = w0 * drho - geq0;  // g_opp - g cancels for the rest direction
-      dLdB += lam[0] * (os0 - (geq0 - g[0]) * k.itau);
+      dLdB += lam[i] * ((geqs - geq_o) - geq_i * k.itau) + lam[o] * ((geqs - geq_i) - geq_o * k.itau);
     }
     const double dBda = ((double)k.tau + 1.0e-14) / ((double)den * (double)den);
     galpha[here] += (double)dLdB * dBda;
+  }
+#pragma unroll
+  for (int kk = 0; kk < Q; ++kk) {
+    const R gc = (R)cx_of(kk) * gux + (R)cy_of(kk) * guy + (R)cz_of(kk) * guz;
+    const R mu = (A * lam[kk] + B * lam[opp_of(kk)]) + (grho + (gc - gu_u) * ir);
+    sto<R, NT>(ob + (size_t)kk * plane, nb(kk), mu);
   }
 }
 
@@ -333,9 +347,12 @@ __global__ __launch_bounds__(256) void k_adjoint_seed(const R* __restrict__ st, 
   galpha[here] = -coef * eu;
 }
 
+// 64 x 4 tiles, or 32 x 8 when rows are a multiple of 32 but not of 64
+// (e.g. 96 cells: no half-empty wavefronts at the row end).
 dim3 cell_grid(const ADims d, dim3& block) {
-  block = dim3(64, 4, 1);
-  return dim3((d.nz + 63) / 64, (d.ny + 3) / 4, d.nx);
+  const int bz = (d.nz % 64 != 0 && d.nz % 32 == 0) ? 32 : 64;
+  block = dim3(bz, 256 / bz, 1);
+  return dim3((d.nz + bz - 1) / bz, (d.ny + block.y - 1) / block.y, d.nx);
 }
 
 }  // namespace
@@ -395,24 +412,38 @@ int launch_adjoint_seed(const void* st, const void* alpha, void* lam, double* ga
 }
 
 int launch_adjoint_step(const void* st, const void* lin, void* lout, const void* alpha, double* galpha, int storage,
-                        int nx, int ny, int nz, double tau, double fx, double fy, double fz, int flags,
-                        hipStream_t stream) {
+                        int nx, int ny, int nz, double tau, double fx, double fy, double fz, int bz, int by,
+                        int flags, hipStream_t stream) {
   const ADims d{nx, ny, nz};
+  const size_t esz = storage == PSC_F32_DEV ? 4 : 8;
+  if ((size_t)nx * ny * nz * esz > 0xffffffffull)
+    return set_error(PSC_EINVAL, "psc_adjoint_step: a direction plane must stay below 4 GiB");
   dim3 block;
-  const dim3 grid = cell_grid(d, block);
+  dim3 grid = cell_grid(d, block);
+  if (bz > 0 && by > 0) {
+    block = dim3(bz, by, 1);
+    grid = dim3((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
+  }
+  // bits 8-15: cap on resident workgroups per CU through an LDS reservation
+  const int bpc = (flags >> 8) & 0xff;
+  size_t lds = 0;
+  if (bpc > 0) {
+    lds = (163840 - 8192) / (size_t)bpc;
+    if (lds > 65536) lds = 65536;
+  }
   const bool nt = (flags & PSC_STEP_NONTEMPORAL) != 0;
   if (storage == PSC_F32_DEV) {
     const AdjConsts<float> k{(float)tau, (float)(1.0 / tau), (float)(1.0 - 0.5 / tau), (float)fx, (float)fy,
                              (float)fz};
     if (nt)
-      k_adjoint_step<float, true, true><<<grid, block, 0, stream>>>((const float*)st, (const float*)lin,
+      k_adjoint_step<float, true, true><<<grid, block, lds, stream>>>((const float*)st, (const float*)lin,
                                                                     (float*)lout, (const float*)alpha, galpha, d, k);
     else
-      k_adjoint_step<float, true, false><<<grid, block, 0, stream>>>((const float*)st, (const float*)lin,
+      k_adjoint_step<float, true, false><<<grid, block, lds, stream>>>((const float*)st, (const float*)lin,
                                                                      (float*)lout, (const float*)alpha, galpha, d, k);
   } else {
     const AdjConsts<double> k{tau, 1.0 / tau, 1.0 - 0.5 / tau, fx, fy, fz};
-    k_adjoint_step<double, false, false><<<grid, block, 0, stream>>>((const double*)st, (const double*)lin,
+    k_adjoint_step<double, false, false><<<grid, block, lds, stream>>>((const double*)st, (const double*)lin,
                                                                      (double*)lout, (const double*)alpha, galpha,
                                                                      d, k);
   }

@@ -618,6 +618,12 @@ class PscD3Q19:
         }
 
     # --------------------------------------------------------------- physics
+    def mean_uz(self) -> float:
+        """Mean of the post-stream u_z over all cells, np.mean(solver.uz) of
+        the drivers (couple_psc_liggghts.py:303), from the FP64 GPU sums
+        without copying the field."""
+        return float(self.readout_sums()["u"][2] / self._N)
+
     def permeability(self, axis=None) -> float:
         """k = nu sum((1 - solid) u_axis) / N / F_axis, nu = (tau - 1/2)/3
         (run_psc_permeability_resolution.py:144-150; axis defaults to the body

@@ -271,6 +271,38 @@ int psc_graph_cache_clear(void);
 int psc_copy_f32(const float* d_src, float* d_dst, int64_t n, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Plain bounce-back D3Q19 window solver of the drivers (the drag proxy and
+ * the fines permeability probes).
+ * ---------------------------------------------------------------------- */
+
+/* Binary voxelization.  Replaces voxelize (scripts/run_3d_window_lbm.py:
+ * 73-92) and voxelize_variable (scripts/run_local_fines_lbm_permeability.py:
+ * 39-59): cell (i, j, k) of the nx*ny*nz mask d_solid (uint8, C order) is set
+ * to 1 when ((ax[i]-x)^2 + (ay[j]-y)^2) + (az[k]-z)^2 <= r^2 for a sphere
+ * whose inclusive cell box (d_boxes: nspheres rows of int32 i0, i1, j0, j1,
+ * k0, k1, within the lattice) contains it.  d_spheres: nspheres rows of
+ * double x, y, z, r; d_axes: the cell-centre coordinates ax (nx), ay (ny),
+ * az (nz), concatenated, in the spheres' units.  Cells are only ever set:
+ * clear d_solid first.  FP64 with the reference's operation order. */
+int psc_voxelize_binary(const double* d_spheres, const int32_t* d_boxes, int32_t nspheres,
+                        const double* d_axes, int32_t nx, int32_t ny, int32_t nz, uint8_t* d_solid,
+                        void* stream);
+
+/* One step of run_lbm (scripts/run_3d_window_lbm.py:95-146) in the pull
+ * layout: BGK with the Guo force (fx, fy, fz); (0, 0, force_z) is the
+ * reference), velocity u = (sum f c + F/2)/rho zeroed in solid cells
+ * (d_solid != 0), streaming, and at solid cells the reference's in-place
+ * reflection loop f_i <- f_opp(i) in direction order, after which both
+ * members of each opposite pair hold the even member's incoming value.
+ * The state holds post-collision values: gather + reflection of it is the
+ * reference's f.  PSC_F64_PLAIN reproduces the reference bit for bit;
+ * PSC_F32_DEV stores g = f - w in FP32.  Start from psc_init_state.
+ * block/flags as psc_step (no cells-per-thread bits). */
+int psc_bb_step(const void* d_in, void* d_out, const uint8_t* d_solid, int32_t storage, int32_t nx,
+                int32_t ny, int32_t nz, double tau, double fx, double fy, double fz, int32_t block_z,
+                int32_t block_y, int32_t flags, void* stream);
+
+/* ------------------------------------------------------------------------
  * Differentiation: permeability gradients with respect to the particles.
  * No counterpart in the reference (its solid fraction is a piecewise-
  * constant sub-sample count, psc3d.py:83-89, with zero derivative almost

@@ -47,6 +47,8 @@ SIGNATURES = {
     "psc_run_steps": ([_P, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_graph_cache_clear": ([], ct.c_int),
     "psc_copy_f32": ([_P, _P, _I64, _P], ct.c_int),
+    "psc_voxelize_binary": ([_P, _P, _I32, _P, _I32, _I32, _I32, _P, _P], ct.c_int),
+    "psc_bb_step": ([_P, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _I32, _P], ct.c_int),
     "psc_smooth_alpha": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P], ct.c_int),
     "psc_smooth_alpha_grad": ([_P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P],
                               ct.c_int),

@@ -65,6 +65,12 @@ int launch_adjoint_seed(const void*, const void*, void*, double*, int, int, int,
 int launch_adjoint_step(const void*, const void*, void*, const void*, double*, int, int, int, int, double, double,
                         double, double, int, int, int, hipStream_t);
 
+int launch_bb_step_f64(const double*, double*, const uint8_t*, int, int, int, double, double, double, double, int, int,
+                       hipStream_t);
+int launch_bb_step_f32(const float*, float*, const uint8_t*, int, int, int, double, double, double, double, int, int,
+                       int, hipStream_t);
+int launch_voxelize_binary(const double*, const int32_t*, int, const double*, int, int, int, uint8_t*, hipStream_t);
+
 int launch_export(const void*, int, int, int, int, int, double*, hipStream_t);
 int launch_import(const double*, int, int, int, int, int, void*, hipStream_t);
 int launch_init_plain(double*, size_t, hipStream_t);
@@ -317,6 +323,38 @@ int psc_copy_f32(const float* d_src, float* d_dst, int64_t n, void* stream) {
   else
     k_copy1<<<grid_cap((size_t)n, 256), 256, 0, s>>>(d_src, d_dst, (size_t)n);
   return check_launch("psc_copy_f32");
+}
+
+int psc_voxelize_binary(const double* d_spheres, const int32_t* d_boxes, int32_t nspheres, const double* d_axes,
+                        int32_t nx, int32_t ny, int32_t nz, uint8_t* d_solid, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_voxelize_binary: bad lattice");
+  if (nspheres < 0) return set_error(PSC_EINVAL, "psc_voxelize_binary: sphere count %d", nspheres);
+  if (d_solid == nullptr || d_axes == nullptr || (nspheres > 0 && (d_spheres == nullptr || d_boxes == nullptr)))
+    return set_error(PSC_EINVAL, "psc_voxelize_binary: null");
+  return launch_voxelize_binary(d_spheres, d_boxes, nspheres, d_axes, nx, ny, nz, d_solid, (hipStream_t)stream);
+}
+
+int psc_bb_step(const void* d_in, void* d_out, const uint8_t* d_solid, int32_t storage, int32_t nx, int32_t ny,
+                int32_t nz, double tau, double fx, double fy, double fz, int32_t block_z, int32_t block_y,
+                int32_t flags, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_bb_step: bad lattice %dx%dx%d", nx, ny, nz);
+  if (d_in == nullptr || d_out == nullptr || d_solid == nullptr) return set_error(PSC_EINVAL, "psc_bb_step: null");
+  if (d_in == d_out) return set_error(PSC_EINVAL, "psc_bb_step: input and output alias");
+  if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_bb_step: tau %g must exceed 1/2", tau);
+  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK)) != 0)
+    return set_error(PSC_EINVAL, "psc_bb_step: unknown flags 0x%x", flags);
+  int bz = block_z, by = block_y;
+  if (bz <= 0) bz = (nz >= 64 && !(nz % 64 != 0 && nz % 32 == 0)) ? 64 : 32;
+  if (by <= 0) by = 256 / bz > 0 ? 256 / bz : 1;
+  if (bz * by > 256 || bz * by < 1 || ((bz * by) % 64) != 0)
+    return set_error(PSC_ELAUNCH, "psc_bb_step: block %dx%d must hold a multiple of 64 threads, at most 256", bz, by);
+  hipStream_t s = (hipStream_t)stream;
+  if (storage == PSC_F32_DEV)
+    return launch_bb_step_f32((const float*)d_in, (float*)d_out, d_solid, nx, ny, nz, tau, fx, fy, fz, bz, by, flags,
+                              s);
+  if (storage == PSC_F64_PLAIN)
+    return launch_bb_step_f64((const double*)d_in, (double*)d_out, d_solid, nx, ny, nz, tau, fx, fy, fz, bz, by, s);
+  return set_error(PSC_EINVAL, "psc_bb_step: storage %d", storage);
 }
 
 int psc_smooth_alpha(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks, int32_t nchunks,

@@ -210,6 +210,82 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
 }
 
 // ---------------------------------------------------------------------------
+// Plain D3Q19 BGK with Guo forcing and bounce-back at a binary solid mask
+// (scripts/run_3d_window_lbm.py:95-146), pull form: the state holds the
+// post-collision values; a step gathers the streamed populations, applies the
+// solid-node loop f_i <- f_opp(i) in direction order (each odd member takes
+// its even partner's value, the even member keeps it: the loop runs in place),
+// then collides with u = 0 in solid cells:
+//   u = (sum f c + F/2)/rho (the half shift only on components with F != 0),
+//   f' = (f - (f - feq)/tau) + Guo(u).
+// ---------------------------------------------------------------------------
+template <bool FORCE>
+__global__ __launch_bounds__(256) void k_bb_step_f64(const double* __restrict__ fin, double* __restrict__ fout,
+                                                     const uint8_t* __restrict__ solid, Dims d, double tau,
+                                                     double pref, double fx, double fy, double fz) {
+  int ix, iy, iz;
+  if (!thread_cell(d, ix, iy, iz)) return;
+  const size_t N = (size_t)d.ps;
+  const uint32_t cell = cell_id(d, ix, iy, iz);
+  double f[Q];
+  load_all<true>(Dirs{}, fin, N, d, ix, iy, iz, f);
+  const bool s = solid[cell] != 0;
+  if (s) {
+#pragma unroll
+    for (int i = 1; i < Q; i += 2) f[i] = f[i + 1];
+  }
+  double rho = f[0];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) rho = rho + f[i];
+  double ux = (fx != 0.0 ? moment<0>(f) + 0.5 * fx : moment<0>(f)) / rho;
+  double uy = (fy != 0.0 ? moment<1>(f) + 0.5 * fy : moment<1>(f)) / rho;
+  double uz = (fz != 0.0 ? moment<2>(f) + 0.5 * fz : moment<2>(f)) / rho;
+  if (s) ux = uy = uz = 0.0;
+  double feq[Q];
+  equilibrium_all(Dirs{}, rho, ux, uy, uz, feq);
+  double g[Q];
+  if (FORCE) guo_all(Dirs{}, pref, ux, uy, uz, fx, fy, fz, g);
+  double out[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const double v = f[i] - (f[i] - feq[i]) / tau;
+    out[i] = FORCE ? v + g[i] : v;
+  }
+  store_all<true>(Dirs{}, fout, N, d, ix, iy, iz, out);
+}
+
+// Binary voxelization (scripts/run_3d_window_lbm.py:73-92,
+// run_local_fines_lbm_permeability.py:39-59): one workgroup per sphere over
+// its inclusive cell box; a cell is solid when
+// ((ax - x)^2 + (ay - y)^2) + (az - z)^2 <= r^2 with the cell-centre
+// coordinates ax, ay, az given per axis.  Every writer stores 1: no atomics.
+__global__ __launch_bounds__(256) void k_voxelize_binary(const double* __restrict__ spheres,
+                                                         const int32_t* __restrict__ boxes,
+                                                         const double* __restrict__ axes, int nx, int ny, int nz,
+                                                         uint8_t* __restrict__ solid) {
+  const double* q = spheres + 4 * (size_t)blockIdx.x;
+  const int32_t* b = boxes + 6 * (size_t)blockIdx.x;
+  const int i0 = b[0], j0 = b[2], k0 = b[4];
+  const int ei = b[1] - i0 + 1, ej = b[3] - j0 + 1, ek = b[5] - k0 + 1;
+  if (ei <= 0 || ej <= 0 || ek <= 0) return;
+  const double x = q[0], y = q[1], z = q[2], r = q[3];
+  const double r2 = r * r;
+  const double* ax = axes;
+  const double* ay = axes + nx;
+  const double* az = axes + nx + ny;
+  const int vol = ei * ej * ek;
+  for (int e = threadIdx.x; e < vol; e += blockDim.x) {
+    const int lk = e % ek;
+    const int t = e / ek;
+    const int lj = t % ej;
+    const int li = t / ej;
+    const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+    const double dx = ax[i] - x, dy = ay[j] - y, dz = az[k] - z;
+    if ((dx * dx + dy * dy) + dz * dz <= r2) solid[((size_t)i * ny + j) * nz + k] = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // K1: sub-sample voxelization (psc3d.py:60-94 / 151-173) and the solid
 // velocity numerator (psc3d.py:190-201 / 352-367).
 // ---------------------------------------------------------------------------
@@ -611,6 +687,26 @@ int grid_for(size_t n, int threads, int cap = 8192) {
 }  // namespace
 
 // ------------------------------- launchers ---------------------------------
+int launch_bb_step_f64(const double* in, double* out, const uint8_t* solid, int nx, int ny, int nz, double tau,
+                       double fx, double fy, double fz, int bz, int by, hipStream_t stream) {
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
+  const dim3 block(bz, by, 1);
+  const dim3 grid((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
+  const double pref = 1.0 - 0.5 / tau;
+  if (fx != 0.0 || fy != 0.0 || fz != 0.0)
+    k_bb_step_f64<true><<<grid, block, 0, stream>>>(in, out, solid, d, tau, pref, fx, fy, fz);
+  else
+    k_bb_step_f64<false><<<grid, block, 0, stream>>>(in, out, solid, d, tau, pref, fx, fy, fz);
+  return check_launch("psc_bb_step(f64)");
+}
+
+int launch_voxelize_binary(const double* spheres, const int32_t* boxes, int nspheres, const double* axes, int nx,
+                           int ny, int nz, uint8_t* solid, hipStream_t stream) {
+  if (nspheres == 0) return PSC_OK;
+  k_voxelize_binary<<<nspheres, 256, 0, stream>>>(spheres, boxes, axes, nx, ny, nz, solid);
+  return check_launch("psc_voxelize_binary");
+}
+
 int launch_step_f64(const double* in, double* out, const uint32_t* csum, const double* usn, double* imb,
                     int layout, int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz,
                     int bz, int by, hipStream_t stream) {

@@ -223,6 +223,86 @@ __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin,
 }
 
 // ---------------------------------------------------------------------------
+// Plain D3Q19 BGK + Guo + bounce-back at a binary solid mask
+// (scripts/run_3d_window_lbm.py:95-146) in deviation form, pull layout: the
+// streamed populations are gathered, solid cells apply the in-place
+// reflection loop of the reference (odd members take their even partner's
+// value), then the cell collides with u = (j + F/2)/rho, zero in solids:
+//   g' = (1 - 1/tau) g + geq/tau + Guo(u).
+// The linear equilibrium and Guo terms use the momentum-exact coefficients
+// k_i and j + F/2, as in step_cell.
+// ---------------------------------------------------------------------------
+template <bool NT>
+__device__ __forceinline__ void bb_cell(const float* __restrict__ gin, float* __restrict__ gout,
+                                        const uint8_t* __restrict__ solid, const Dims d, const StepConsts& k,
+                                        int ix, int iy, int iz) {
+  const size_t plane = (size_t)d.ps * sizeof(float);
+  const int xm = wrap_step(ix, -1, d.nx), xp = wrap_step(ix, 1, d.nx);
+  const int ym = wrap_step(iy, -1, d.ny), yp = wrap_step(iy, 1, d.ny);
+  const int zm = wrap_step(iz, -1, d.nz), zp = wrap_step(iz, 1, d.nz);
+  auto X = [&](int s) { return s < 0 ? xm : (s > 0 ? xp : ix); };
+  auto Y = [&](int s) { return s < 0 ? ym : (s > 0 ? yp : iy); };
+  auto Z = [&](int s) { return s < 0 ? zm : (s > 0 ? zp : iz); };
+  auto off = [&](int sx, int sy, int sz) -> uint32_t { return cell_id(d, X(sx), Y(sy), Z(sz)) << 2; };
+  const uint32_t here = cell_id(d, ix, iy, iz) << 2;
+  const char* gi = (const char*)gin;
+  char* go = (char*)gout;
+  float g[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) g[i] = ld<NT>(gi + (size_t)i * plane, off(-cx_of(i), -cy_of(i), -cz_of(i)));
+  const bool s = solid[cell_id(d, ix, iy, iz)] != 0;
+  if (s) {
+#pragma unroll
+    for (int i = 1; i < Q; i += 2) g[i] = g[i + 1];
+  }
+  const float drho = ((g[0] + (g[1] + g[2])) + ((g[3] + g[4]) + (g[5] + g[6]))) +
+                     (((g[7] + g[8]) + (g[9] + g[10])) + ((g[11] + g[12]) + (g[13] + g[14]))) +
+                     ((g[15] + g[16]) + (g[17] + g[18]));
+  const float jx = (g[1] - g[2]) + ((g[7] - g[8]) + (g[9] - g[10])) + ((g[11] - g[12]) + (g[13] - g[14]));
+  const float jy = (g[3] - g[4]) + ((g[7] - g[8]) - (g[9] - g[10])) + ((g[15] - g[16]) + (g[17] - g[18]));
+  const float jz = (g[5] - g[6]) + ((g[11] - g[12]) - (g[13] - g[14])) + ((g[15] - g[16]) - (g[17] - g[18]));
+  const float rho = 1.0f + drho;
+  const float ir = __builtin_amdgcn_rcpf(rho);
+  // rho * u with the single half-force shift; zero in solid cells
+  const float jFx = s ? 0.0f : jx + 0.5f * k.Fx, jFy = s ? 0.0f : jy + 0.5f * k.Fy, jFz = s ? 0.0f : jz + 0.5f * k.Fz;
+  const float ux = jFx * ir, uy = jFy * ir, uz = jFz * ir;
+  const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+  const float uF3 = 3.0f * (ux * k.Fx + uy * k.Fy + uz * k.Fz);
+  auto wdr = [&](int i) { return (float)w_of(i) * drho; };
+  auto wrh = [&](int i) { return (float)w_of(i) * rho; };
+  auto pw = [&](int i) { return k.pref * (float)w_of(i); };
+  float gn[Q];
+  gn[0] = fmaf(g[0], k.omt, fmaf(wdr(0) - wrh(0) * usq15, k.itau, -pw(0) * uF3));
+#pragma unroll
+  for (int i = 1; i < Q; i += 2) {
+    const int o = i + 1;
+    const float cu = (float)cx_of(i) * ux + (float)cy_of(i) * uy + (float)cz_of(i) * uz;
+    const float cj = (float)cx_of(i) * jFx + (float)cy_of(i) * jFy + (float)cz_of(i) * jFz;
+    const float cf = (float)cx_of(i) * k.Fx + (float)cy_of(i) * k.Fy + (float)cz_of(i) * k.Fz;
+    const float S = fmaf(wrh(i), fmaf(4.5f * cu, cu, -usq15), wdr(i));
+    const float A = k_lin(i) * cj;
+    const float Gs = pw(i) * fmaf(9.0f * cu, cf, -uF3);
+    const float Ga = k.pref * (k_lin(i) * cf);
+    const float P = fmaf(S, k.itau, Gs);
+    const float M = fmaf(A, k.itau, Ga);
+    gn[i] = fmaf(g[i], k.omt, P + M);
+    gn[o] = fmaf(g[o], k.omt, P - M);
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) st<NT>(go + (size_t)i * plane, here, gn[i]);
+}
+
+template <bool NT>
+__global__ __launch_bounds__(256) void k_bb_step_f32(const float* __restrict__ gin, float* __restrict__ gout,
+                                                     const uint8_t* __restrict__ solid, Dims d, StepConsts k) {
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  const int ix = blockIdx.z;
+  if (iz >= d.nz || iy >= d.ny) return;
+  bb_cell<NT>(gin, gout, solid, d, k, ix, iy, iz);
+}
+
+// ---------------------------------------------------------------------------
 // Batched windows: one persistent launch steps many independent periodic
 // lattices.  Window w's state is a contiguous block of 19 planes of N_w
 // cells at element offset state_off; its count_sum and imb cells sit at
@@ -335,6 +415,34 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
 #undef PSC_F32_CASES
 #undef PSC_F32_CASE
   return set_error(PSC_EINVAL, "psc_step(f32): no kernel variant");
+}
+
+int launch_bb_step_f32(const float* in, float* out, const uint8_t* solid, int nx, int ny, int nz, double tau,
+                       double fx, double fy, double fz, int bz, int by, int flags, hipStream_t stream) {
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
+  const int bpc = (flags >> 8) & 0xff;
+  size_t lds = 0;
+  if (bpc > 0) {
+    lds = (kLdsPerCU - kLdsHeadroom) / (size_t)bpc;
+    if (lds > 65536) lds = 65536;
+  }
+  if ((size_t)nx * ny * nz * sizeof(float) > 0xffffffffull)
+    return set_error(PSC_EINVAL, "psc_bb_step(f32): a direction plane must stay below 4 GiB");
+  StepConsts k;
+  k.field = 0;
+  k.inv_s3 = 0.0f;
+  k.tau = (float)tau;
+  k.itau = (float)(1.0 / tau);
+  k.omt = (float)(1.0 - 1.0 / tau);
+  k.pref = (float)(1.0 - 0.5 / tau);
+  k.Fx = (float)fx;
+  k.Fy = (float)fy;
+  k.Fz = (float)fz;
+  const dim3 block(bz, by, 1);
+  const dim3 grid((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
+  if (flags & 1) k_bb_step_f32<true><<<grid, block, lds, stream>>>(in, out, solid, d, k);
+  else k_bb_step_f32<false><<<grid, block, lds, stream>>>(in, out, solid, d, k);
+  return check_launch("psc_bb_step(f32)");
 }
 
 // Batched launch: win/items are device tables (see BatchWindow); grid_blocks

@@ -54,9 +54,11 @@ def parse():
     ap.add_argument("--launch", default=None,
                     help="fixed psc_step launch 'block_z,block_y,flags' instead of autotuning (for profiling)")
     ap.add_argument("--no-copy", action="store_true")
-    ap.add_argument("--workload", default="fixed_array", choices=["fixed_array", "moving", "windows", "adjoint"],
+    ap.add_argument("--workload", default="fixed_array",
+                    choices=["fixed_array", "moving", "windows", "adjoint", "bounce_back"],
                     help="fixed_array: 128^3 permeability (default); moving: 512^3 window, 2000 moving spheres; "
-                         "windows: 64 batched pore windows; adjoint: 96^3 permeability gradient")
+                         "windows: 64 batched pore windows; adjoint: 96^3 permeability gradient; "
+                         "bounce_back: 256^3 plain bounce-back window")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -393,6 +395,80 @@ def bench_adjoint(args, workloads, torch, np):
     print(json.dumps(out))
 
 
+def bench_bounce_back(args, workloads, torch, np):
+    """Plain bounce-back window solver (run_lbm of the drivers): periodic
+    256^3 voxel window of 400 RSA spheres r = 10 (centre-in-sphere voxels),
+    tau 0.72, force 2e-7 along z (the drivers' defaults), FP32 timed over the
+    steps after a warm-up, FP64 run for the k error and its own MLUPS."""
+    lbm = importlib.import_module(PKG + ".lbm")
+    n = 256
+    steps = args.steps if args.steps != 4900 else 1000
+    warm = min(args.warmup, 50)
+    pts = workloads.rsa_spheres(n, 400, 10.0, seed=0)
+    solid, _ = lbm.voxelize(pts, np.full(3, n / 2.0), np.full(3, float(n)), 10.0, n, device="cuda:0")
+    N = n ** 3
+    res = {}
+    for prec in ("fp32", "fp64"):
+        s = lbm.BounceBackD3Q19(solid, tau=0.72, force_z=2.0e-7, precision=prec, device="cuda:0",
+                                block=(128, 2), nontemporal=(prec == "fp32"))
+        s.run(warm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        stream = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        reps = 50
+        e0.record(stream)
+        for _ in range(reps):
+            s._launch(s._a, s._b, stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        kern = e0.elapsed_time(e1) / reps
+        out = s.result()
+        res[prec] = {"MLUPS": N * steps / wall / 1e6, "kernel_us": kern * 1e3, "k": out["permeability_lu"],
+                     "bytes_per_cell": (19 * 4 * 2 + 1) if prec == "fp32" else (19 * 8 * 2 + 1)}
+        log(f"{prec}: {res[prec]['MLUPS']:.0f} MLUPS, kernel {kern*1e3:.1f} us, k={out['permeability_lu']:.9e}",
+            args.quiet)
+        del s
+    r32 = res["fp32"]
+    ach = r32["bytes_per_cell"] * N / (r32["kernel_us"] * 1e-6) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import bounce_back as BB
+        # the drivers' default resolution, spheres scaled to keep the solid fraction
+        cn, cs = 56, 20
+        cr = 10.0 * cn / n * 2.0
+        ccount = max(1, int(round(float(solid.mean()) * cn ** 3 / (4.0 / 3.0 * np.pi * cr ** 3))))
+        csolid = BB.voxelize(workloads.rsa_spheres(cn, ccount, cr, seed=0), np.full(3, cn / 2.0),
+                             np.full(3, float(cn)), cr, cn)[0]
+        t0 = time.perf_counter()
+        BB.run_lbm(csolid, 0.72, 2.0e-7, cs)
+        dt = time.perf_counter() - t0
+        cpu = {"value": cn ** 3 * cs / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+               "sample": f"NumPy FP64 restatement of run_lbm (oracle/bounce_back.py), {cs} steps of a {cn}^3 window "
+                         f"(the drivers' default resolution) of {ccount} spheres r={cr:.2f}, solid fraction "
+                         f"{float(csolid.mean()):.3f}, in {dt:.1f} s, one thread"}
+    out = {
+        "metric": METRIC + " (plain bounce-back window configuration)",
+        "value": r32["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": N / (r32["MLUPS"] * 1e6) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "run_lbm bounce-back window: periodic 256^3, 400 RSA spheres r=10 (seed 0), "
+                               "centre-in-sphere voxels, tau 0.72, force 2e-7 z, FP32 deviation storage; "
+                               "two steps per hipGraph replay",
+                   "lattice": [n, n, n], "porosity": float(1.0 - solid.mean())},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_bb_step_f32",
+                     "bytes_per_cell": r32["bytes_per_cell"], "kernel_us": r32["kernel_us"]},
+        "fp64": res["fp64"], "k_rel_err_fp32_vs_fp64": abs(r32["k"] - res["fp64"]["k"]) / abs(res["fp64"]["k"]),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -413,6 +489,8 @@ def main():
         return bench_windows(args, workloads, torch, np)
     if args.workload == "adjoint":
         return bench_adjoint(args, workloads, torch, np)
+    if args.workload == "bounce_back":
+        return bench_bounce_back(args, workloads, torch, np)
     n = args.n
     table = workloads.fixed_array(seed=0, n=n)
     total = args.warmup + args.steps

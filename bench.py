@@ -289,7 +289,8 @@ def bench_windows(args, workloads, torch, np):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_step_f32_batched",
                      "bytes_per_cell": BYTES_F32_STATIC, "kernel_us": kern * 1e3,
-                     "launch": {"flags": B._flags, "grid": B._grid}},
+                     "launch": {"flags": B._flags, "grid": B._grid, "items": B._mode,
+                                "workgroup": 512 if B._flags & 0x40000 else 256}},
         "permeability": {v: [float(x) for x, vv in zip(k, variants) if vv == v][:4] for v in ("skeleton", "passive",
                                                                                             "preloaded")},
         "porosity_range": [float(por.min()), float(por.max())],

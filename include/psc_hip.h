@@ -55,6 +55,7 @@ extern "C" {
 #define PSC_STEP_NONTEMPORAL 1      /* bit 0: non-temporal loads/stores (FP32)      */
 #define PSC_STEP_BPC_MASK 0xff00    /* bits 8-15: cap on resident workgroups per CU */
 #define PSC_STEP_CPT_MASK 0x30000   /* bits 16-17: cells per thread along y (1, 2)  */
+#define PSC_BATCH_WG512 0x40000     /* psc_step_batched: 512-thread workgroups      */
 
 #define PSC_NSUM 13 /* number of FP64 sums produced by psc_moments           */
 
@@ -145,7 +146,9 @@ typedef struct PscWindow {
   int32_t nx, ny, nz; /* window extents                                               */
   int32_t pz;         /* z row pitch >= nz (rows padded for aligned access)           */
   float inv_pz, inv_nypz; /* 1/pz and 1/(ny*pz) in FP32 (index decomposition)        */
-  float fx, fy, fz, pad2; /* body force of the window                                 */
+  float fx, fy, fz;   /* body force of the window                                    */
+  int32_t span;       /* cells per work item, at most the workgroup size (<= 0: the   */
+                      /* workgroup size); items of whole rows keep rows unsplit       */
   int64_t ps;         /* cells per direction plane: nx*ny*pz rounded up to 64 (256 B) */
 } PscWindow;
 
@@ -155,12 +158,15 @@ typedef struct PscWindow {
  * pz; padding cells are never read or written), each plane ps cells long,
  * at d_in + state_off; its
  * count_sum/imb cells start at cell_off (imb has 3 planes of imb_stride
- * cells).  d_items lists (window, first cell) pairs of spans of up to 256
- * cells; each workgroup of 256 threads steps one span at a time, looping
- * over the items with a stride of grid_blocks (0 = one workgroup per item).
+ * cells).  d_items lists (window, first cell) pairs; an item covers the
+ * window's `span` cells from there (the workgroup size when span <= 0).  Each
+ * workgroup (256 threads, 512 with PSC_BATCH_WG512) steps one item at a time,
+ * looping over the items with a stride of grid_blocks (0 = one workgroup per
+ * item).
  * Each window's step is collide_stream_psc (psc3d.py:208-246), the same
  * arithmetic as psc_step with PSC_F32_DEV.  N_w must stay below 2^22 cells.
- * flags: PSC_STEP_NONTEMPORAL and the workgroups-per-CU cap of psc_step. */
+ * flags: PSC_STEP_NONTEMPORAL, the workgroups-per-CU cap of psc_step and
+ * PSC_BATCH_WG512. */
 int psc_step_batched(const float* d_in, float* d_out, const uint32_t* d_count_sum, float* d_imb,
                      int64_t imb_stride, const PscWindow* d_windows, const int32_t* d_items,
                      int32_t nitems, int32_t subsamples, double tau, int32_t flags,

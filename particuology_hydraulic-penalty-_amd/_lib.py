@@ -20,6 +20,7 @@ PSC_PULL = 1
 PSC_STEP_NONTEMPORAL = 1
 PSC_NSUM = 13
 PSC_ALPHA_FIELD = 0
+PSC_BATCH_WG512 = 0x40000
 
 # Names and argument types of every exported entry point, in header order.
 _P = ct.c_void_p
@@ -78,7 +79,7 @@ class PscWindow(ct.Structure):
 
     _fields_ = [("state_off", _I64), ("cell_off", _I64), ("nx", _I32), ("ny", _I32), ("nz", _I32), ("pz", _I32),
                 ("inv_pz", ct.c_float), ("inv_nypz", ct.c_float), ("fx", ct.c_float), ("fy", ct.c_float),
-                ("fz", ct.c_float), ("pad2", ct.c_float), ("ps", _I64)]
+                ("fz", ct.c_float), ("span", _I32), ("ps", _I64)]
 
 
 class PscError(RuntimeError):

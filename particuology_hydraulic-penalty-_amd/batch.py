@@ -9,11 +9,12 @@ windows of uneven sizes shares one set of device buffers:
 * particles, boxes, chunks and voxel entries: concatenated, with global
   particle ids, so one launch of the load kernels covers every window.
 
-Each step is ONE persistent launch of ``psc_step_batched`` (work items are
-256-cell spans of a window), followed by the particle-load reduction; ``run``
-captures two steps into a hipGraph (torch.cuda.CUDAGraph around the C-ABI
-launches) and replays it.  The launch configuration (non-temporal access,
-occupancy cap, persistent grid size) is autotuned on first use.
+Each step is ONE launch of ``psc_step_batched`` (work items are spans of
+cells or groups of whole rows of a window), followed by the particle-load
+reduction; ``run`` captures two steps into a hipGraph (torch.cuda.CUDAGraph
+around the C-ABI launches) and replays it.  The launch configuration
+(workgroup size, item mode, non-temporal access, occupancy cap) is autotuned
+on first use.
 Every window is the reference's ``collide_stream_psc`` (psc3d.py:208-253) with
 periodic geometry; per-window permeability follows
 run_psc_permeability_resolution.py:144-150.
@@ -28,7 +29,6 @@ import numpy as np
 from . import _lib
 from .geometry import build_tables, pack_particles
 
-SPAN = 256
 MAX_WINDOW_CELLS = 1 << 22
 ROW_ALIGN = 4  # z rows padded to a multiple of 4 cells (16-byte aligned rows)
 PLANE_ALIGN = 64  # direction planes padded to a multiple of 64 cells (256-byte aligned planes)
@@ -70,24 +70,8 @@ class BatchedPscD3Q19:
         self._b = torch.zeros(19 * self.Ntot, dtype=torch.float32, device=dev)
         self._csum = torch.zeros(self.Ntot, dtype=torch.int32, device=dev)
         self._imb = torch.zeros(3 * self.Ntot, dtype=torch.float32, device=dev)
-        # window table and work items
-        wt = (_lib.PscWindow * self.nwin)()
-        items = []
-        for w, (nx, ny, nz) in enumerate(self.shapes):
-            wt[w].state_off = int(self.state_off[w])
-            wt[w].cell_off = int(self.cell_off[w])
-            pz = self.pitch[w]
-            wt[w].nx, wt[w].ny, wt[w].nz, wt[w].pz = nx, ny, nz, pz
-            wt[w].inv_pz = 1.0 / pz
-            wt[w].inv_nypz = 1.0 / (ny * pz)
-            wt[w].ps = int(self.plane[w])
-            wt[w].fx, wt[w].fy, wt[w].fz = self.body_force
-            for start in range(0, int(ncell[w]), SPAN):
-                items.append((w, start))
-        raw = np.frombuffer(bytes(wt), dtype=np.uint8)
-        self._windows = torch.from_numpy(raw.copy()).to(dev)
-        self._items = torch.from_numpy(np.asarray(items, dtype=np.int32).reshape(-1)).to(dev)
-        self.nitems = len(items)
+        # window tables and work items per (workgroup size, item mode), built on demand
+        self._item_tables = {}
         # geometry: per-window boxes and chunks, concatenated with global ids
         self.tables = tables
         self.pbase = np.concatenate([[0], np.cumsum([t.shape[0] for t in tables])[:-1]]).astype(np.int64)
@@ -118,9 +102,12 @@ class BatchedPscD3Q19:
                                          dtype=torch.float64, device=dev)
         self._flags = 0
         self._grid = 0
+        self._mode = "spans"
         self._tuned = launch != "auto"
         if launch != "auto":
             self._flags, self._grid = int(launch[0]), int(launch[1])
+            if len(launch) > 2:
+                self._mode = str(launch[2])
         self._graph = None
         self._steps = 0
         self._voxelize(tabs)
@@ -146,10 +133,47 @@ class BatchedPscD3Q19:
                       self._nnz.data_ptr() + 4 * cb, self._csum.data_ptr() + 4 * int(self.cell_off[w]), None, 0,
                       int(self.cell_off[w]), s)
 
+    def _items_for(self, bs: int, mode: str):
+        """(window table, item list, item count) for workgroups of ``bs``
+        threads.  mode "spans": items of bs consecutive cells (rows may be
+        split between workgroups); mode "rows": items of floor(bs/pz) whole
+        rows, so a row's cache lines are only ever touched by one workgroup
+        (windows whose rows exceed bs fall back to spans)."""
+        import torch
+        key = (int(bs), mode)
+        if key in self._item_tables:
+            return self._item_tables[key]
+        if mode not in ("spans", "rows"):
+            raise ValueError(f"unknown item mode {mode!r}")
+        wt = (_lib.PscWindow * self.nwin)()
+        items = []
+        for w, (nx, ny, nz) in enumerate(self.shapes):
+            pz = self.pitch[w]
+            wt[w].state_off = int(self.state_off[w])
+            wt[w].cell_off = int(self.cell_off[w])
+            wt[w].nx, wt[w].ny, wt[w].nz, wt[w].pz = nx, ny, nz, pz
+            wt[w].inv_pz = 1.0 / pz
+            wt[w].inv_nypz = 1.0 / (ny * pz)
+            wt[w].ps = int(self.plane[w])
+            wt[w].fx, wt[w].fy, wt[w].fz = self.body_force
+            rows = bs // pz if mode == "rows" else 0
+            span = rows * pz if rows >= 1 else bs
+            wt[w].span = span
+            starts = np.arange(0, int(self.ncell[w]), span, dtype=np.int64)
+            items.append(np.stack([np.full(len(starts), w, dtype=np.int64), starts], axis=1))
+        raw = np.frombuffer(bytes(wt), dtype=np.uint8)
+        windows = torch.from_numpy(raw.copy()).to(self.device)
+        it = np.concatenate(items).astype(np.int32).reshape(-1)
+        entry = (windows, torch.from_numpy(it).to(self.device), len(it) // 2)
+        self._item_tables[key] = entry
+        return entry
+
     # ------------------------------------------------------------------ step
-    def _launch_step(self, a, b, flags, grid, stream):
+    def _launch_step(self, a, b, flags, grid, stream, mode=None):
+        bs = 512 if (flags & _lib.PSC_BATCH_WG512) else 256
+        windows, items, nitems = self._items_for(bs, mode or self._mode)
         _lib.call("psc_step_batched", a.data_ptr(), b.data_ptr(), self._csum.data_ptr(), self._imb.data_ptr(),
-                  self.Ntot, self._windows.data_ptr(), self._items.data_ptr(), self.nitems, self._s, self.tau,
+                  self.Ntot, windows.data_ptr(), items.data_ptr(), nitems, self._s, self.tau,
                   flags, grid, stream)
 
     def _launch_loads(self, stream):
@@ -171,30 +195,33 @@ class BatchedPscD3Q19:
         self._steps += 1
 
     def _tune(self):
-        """Pick non-temporal access, occupancy cap and persistent grid size."""
+        """Pick the workgroup size (256 or 512), item mode (cell spans or
+        whole rows), non-temporal access and occupancy cap, timing each
+        candidate with HIP events on the solver's own buffers (the state is
+        not modified: A -> B)."""
         if self._tuned:
             return
         import torch
         from . import autotune
         s = self._stream()
-        ncu = torch.cuda.get_device_properties(self.device).multi_processor_count
         best = None
-        for nt in (0, 1):
-            for bpc in (0, 2, 4, 6, 8):
-                for grid in (0, 2 * ncu, 4 * ncu, 8 * ncu):
-                    flags = autotune.flags_of(nt, bpc, 0)
-                    self._launch_step(self._a, self._b, flags, grid, s)
-                    e0 = torch.cuda.Event(enable_timing=True)
-                    e1 = torch.cuda.Event(enable_timing=True)
-                    e0.record()
-                    for _ in range(3):
-                        self._launch_step(self._a, self._b, flags, grid, s)
-                    e1.record()
-                    e1.synchronize()
-                    ms = e0.elapsed_time(e1) / 3
-                    if best is None or ms < best[0]:
-                        best = (ms, flags, grid)
-        self._flags, self._grid = best[1], best[2]
+        for bs in (256, 512):
+            for mode in ("rows", "spans"):
+                for nt in (0, 1):
+                    for bpc in (0, 2, 4, 8):
+                        flags = autotune.flags_of(nt, bpc, 0) | (_lib.PSC_BATCH_WG512 if bs == 512 else 0)
+                        self._launch_step(self._a, self._b, flags, 0, s, mode)
+                        e0 = torch.cuda.Event(enable_timing=True)
+                        e1 = torch.cuda.Event(enable_timing=True)
+                        e0.record()
+                        for _ in range(3):
+                            self._launch_step(self._a, self._b, flags, 0, s, mode)
+                        e1.record()
+                        e1.synchronize()
+                        ms = e0.elapsed_time(e1) / 3
+                        if best is None or ms < best[0]:
+                            best = (ms, flags, mode)
+        self._flags, self._grid, self._mode = best[1], 0, best[2]
         self._tuned = True
 
     def run(self, steps: int, graph: bool = True) -> None:
@@ -204,6 +231,8 @@ class BatchedPscD3Q19:
         import torch
         steps = int(steps)
         self._tune()
+        # the item tables are uploaded here, never inside the graph capture
+        self._items_for(512 if (self._flags & _lib.PSC_BATCH_WG512) else 256, self._mode)
         if not graph or steps < 2:
             for _ in range(steps):
                 self.step()

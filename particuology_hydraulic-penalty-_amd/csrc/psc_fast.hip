@@ -316,7 +316,8 @@ struct BatchWindow {
   long long cell_off;   // cells
   int nx, ny, nz, pz;   // extents and z row pitch (pz >= nz)
   float inv_pz, inv_nypz;  // reciprocals for the index decomposition
-  float fx, fy, fz, pad2;
+  float fx, fy, fz;
+  int span;             // cells per work item (<= workgroup size; <= 0: the workgroup size)
   long long ps;         // cells per direction plane (multiple of 64: 256-byte aligned planes)
 };
 
@@ -329,20 +330,21 @@ __device__ __forceinline__ void divmod_f(uint32_t n, uint32_t d, float inv, uint
   r = (uint32_t)rr;
 }
 
-template <bool IMB, bool NT>
-__global__ __launch_bounds__(256) void k_step_f32_batched(const float* __restrict__ gin, float* __restrict__ gout,
-                                                          const uint32_t* __restrict__ csum,
-                                                          float* __restrict__ imb, size_t imb_stride,
-                                                          const BatchWindow* __restrict__ wins,
-                                                          const int2* __restrict__ items, int nitems,
-                                                          StepConsts k0) {
+template <bool IMB, bool NT, int BS>
+__global__ __launch_bounds__(BS) void k_step_f32_batched(const float* __restrict__ gin, float* __restrict__ gout,
+                                                         const uint32_t* __restrict__ csum,
+                                                         float* __restrict__ imb, size_t imb_stride,
+                                                         const BatchWindow* __restrict__ wins,
+                                                         const int2* __restrict__ items, int nitems,
+                                                         StepConsts k0) {
   const int tid = threadIdx.x;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int2 item = items[it];  // (window, first cell)
     const BatchWindow w = wins[item.x];
+    const int span = w.span > 0 ? w.span : BS;
     const uint32_t c = (uint32_t)(item.y + tid);
     const uint32_t Nw = (uint32_t)w.nx * w.ny * w.pz;
-    if (c >= Nw) continue;
+    if (tid >= span || c >= Nw) continue;
     uint32_t ix, rem, iy, iz;
     divmod_f(c, (uint32_t)(w.ny * w.pz), w.inv_nypz, ix, rem);
     divmod_f(rem, (uint32_t)w.pz, w.inv_pz, iy, iz);
@@ -469,14 +471,21 @@ int launch_step_f32_batched(const float* in, float* out, const uint32_t* csum, f
   const BatchWindow* w = (const BatchWindow*)wins;
   const int2* it = (const int2*)items;
   const bool nt = (flags & 1) != 0;
-  if (imb != nullptr) {
-    if (nt) k_step_f32_batched<true, true><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k);
-    else k_step_f32_batched<true, false><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k);
-  } else {
-    if (nt) k_step_f32_batched<false, true><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k);
-    else k_step_f32_batched<false, false><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k);
+  const bool wide = (flags & PSC_BATCH_WG512) != 0;
+#define PSC_BATCH_CASE(I, T)                                                                                  \
+  if ((imb != nullptr) == I && nt == T) {                                                                    \
+    if (wide)                                                                                                \
+      k_step_f32_batched<I, T, 512><<<grid, 512, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k); \
+    else                                                                                                     \
+      k_step_f32_batched<I, T, 256><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k); \
+    return check_launch("psc_step_batched");                                                                 \
   }
-  return check_launch("psc_step_batched");
+  PSC_BATCH_CASE(true, true)
+  PSC_BATCH_CASE(true, false)
+  PSC_BATCH_CASE(false, true)
+  PSC_BATCH_CASE(false, false)
+#undef PSC_BATCH_CASE
+  return set_error(PSC_EINVAL, "psc_step_batched: no kernel variant");
 }
 
 }  // namespace psc

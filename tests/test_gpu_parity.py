@@ -261,11 +261,15 @@ def test_moving_fp32_loads_track_fp64(psc3d):
     assert err < 1e-4
 
 
-def test_batched_windows_match_single_solvers(psc3d):
-    """Every window of a batch (uneven sizes, persistent launch, graph
-    replays) equals a standalone FP32 solver of that window bit for bit,
-    state and loads, and the per-window permeability matches."""
+@pytest.mark.parametrize("launch", ["auto", (1, 0, "spans"), (1 | 0x40000, 0, "rows"), (0x200, 0, "rows"),
+                                    (0x40000, 64, "spans")])
+def test_batched_windows_match_single_solvers(psc3d, launch):
+    """Every window of a batch (uneven sizes, every workgroup size and item
+    mode, persistent grid, graph replays) equals a standalone FP32 solver of
+    that window bit for bit, state and loads, and the per-window
+    permeability matches."""
     batch = load_package("batch")
+    print(f"[batched windows] launch {launch}", flush=True)
     rng = np.random.default_rng(31)
     wins = []
     for L in (20, 33, 48, 27):
@@ -273,7 +277,7 @@ def test_batched_windows_match_single_solvers(psc3d):
         t[:, 0:3] = rng.uniform(0, L, (4, 3))
         t[:, 3] = rng.uniform(2.0, 4.5, 4)
         wins.append((L, t))
-    B = batch.BatchedPscD3Q19(wins, tau=0.8, subsamples=3, body_force=(0, 0, 2e-6), device="cuda:0")
+    B = batch.BatchedPscD3Q19(wins, tau=0.8, subsamples=3, body_force=(0, 0, 2e-6), device="cuda:0", launch=launch)
     B.run(13)
     singles = []
     for L, t in wins:

@@ -124,8 +124,10 @@ int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, in
  * the solid velocity is us_num/(count_sum/s^3) (zero if d_us_num is NULL).
  * The body force (fx, fy, fz) generalises body_force_z; (0, 0, Fz) is the
  * reference.  d_imb (3 planes, optional) receives sum_i (B Omega_s)_i c_i
- * (psc3d.py:234-236) in every cell with count_sum > 0; other cells are not
- * written.  d_in and d_out must not alias.  block_z/block_y select the
+ * (psc3d.py:234-236) in every cell with count_sum > 0; other cells are left
+ * untouched or set to 0 (FP32 storage writes every lane of a wavefront that
+ * holds a solid cell), so readers mask imb by count_sum > 0.  d_in and d_out
+ * must not alias.  block_z/block_y select the
  * workgroup shape (0 = default); flags (FP32 storage): PSC_STEP_NONTEMPORAL
  * streams the distributions with non-temporal loads/stores, bits 8-15 cap
  * the resident workgroups per CU (through an LDS reservation, 0 = no cap),

@@ -30,13 +30,13 @@ from . import _lib
 from .geometry import build_tables, pack_particles
 
 MAX_WINDOW_CELLS = 1 << 22
-ROW_ALIGN = 4  # z rows padded to a multiple of 4 cells (16-byte aligned rows)
+ROW_ALIGN = 4  # default z-row padding: a multiple of 4 cells (16-byte aligned rows)
 PLANE_ALIGN = 64  # direction planes padded to a multiple of 64 cells (256-byte aligned planes)
 
 
 class BatchedPscD3Q19:
     def __init__(self, windows, tau: float = 0.8, subsamples: int = 3, body_force=(0.0, 0.0, 1.0e-6),
-                 device=None, chunk_cells: int = 2048, launch="auto"):
+                 device=None, chunk_cells: int = 2048, launch="auto", row_align: int = ROW_ALIGN):
         import torch
         self._lib = _lib.load()
         if not torch.cuda.is_available():
@@ -55,7 +55,10 @@ class BatchedPscD3Q19:
             self.shapes.append(shp)
             tables.append(pack_particles(parts))
         self.nwin = len(self.shapes)
-        self.pitch = [-(-c // ROW_ALIGN) * ROW_ALIGN for _, _, c in self.shapes]
+        # rows padded to row_align cells (32: every row a whole number of 128-byte cache lines)
+        ra = max(1, int(row_align))
+        self.row_align = ra
+        self.pitch = [-(-c // ra) * ra for _, _, c in self.shapes]
         # cells per window incl. row padding, and per direction plane (64-cell multiple: 256-byte aligned planes,
         # the layout psc_moments assumes for pitch_z > nz)
         ncell = np.array([a * b * p for (a, b, _), p in zip(self.shapes, self.pitch)], dtype=np.int64)

@@ -119,6 +119,7 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
   // Post-collision values; stored by one store per direction after the
   // branch, so every store keeps the uniform-base + 32-bit-offset form.
   float gn[Q];
+  float mx = 0.f, my = 0.f, mz = 0.f;  // imb of a solid cell (0 in fluid)
   if (cnt == 0) {
     // fluid cell: g' = (1 - 1/tau) g + geq/tau + Guo, pairwise even/odd parts
     {
@@ -155,7 +156,6 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
       sz = usn[2 * N + c] * ia;
       ssq15 = 1.5f * (sx * sx + sy * sy + sz * sz);
     }
-    float mx = 0.f, my = 0.f, mz = 0.f;
     {
       const float geq0 = wdr(0) - wrh(0) * usq15;
       const float geqs0 = MOVING ? wdr(0) - wrh(0) * ssq15 : wdr(0);
@@ -189,12 +189,15 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
       if (cy_of(i)) my += (float)cy_of(i) * dm;
       if (cz_of(i)) mz += (float)cz_of(i) * dm;
     }
-    if (IMB) {
-      const uint32_t c = cell_id(d, ix, iy, iz);
-      imb[c] = mx;
-      imb[imb_stride + c] = my;
-      imb[2 * imb_stride + c] = mz;
-    }
+  }
+  // imb is defined at solid cells only; a wavefront with any solid cell
+  // writes all its lanes (zeros at fluid cells) so the stores fill whole
+  // cache lines instead of leaving partial lines to merge.
+  if (IMB && __any(cnt != 0)) {
+    const uint32_t c = cell_id(d, ix, iy, iz);
+    imb[c] = mx;
+    imb[imb_stride + c] = my;
+    imb[2 * imb_stride + c] = mz;
   }
 #pragma unroll
   for (int i = 0; i < Q; ++i) st<NT>(go + (size_t)i * plane, dst(i), gn[i]);

@@ -257,16 +257,55 @@ class PscD3Q19:
         return self._force
 
     # --------------------------------------------------------------- geometry
+    def _upload_tables(self, arrays):
+        """Device copies of host tables without stalling the host: the arrays
+        are packed into one of two pinned staging buffers and copied
+        asynchronously on the solver's stream into the matching device buffer
+        (8-byte aligned views are returned).  A slot is reused two rebuilds
+        later, after the copy out of it has completed (event wait), and the
+        kernels that read the previous contents of its device buffer were
+        queued before the new copy on the same stream."""
+        torch = _torch()
+        offs, total = [], 0
+        for a in arrays:
+            offs.append(total)
+            total += (a.nbytes + 7) // 8 * 8
+        total = max(total, 8)
+        slot = self._stage_slot = 1 - getattr(self, "_stage_slot", 1)
+        if not hasattr(self, "_stage"):
+            self._stage = [None, None]
+        entry = self._stage[slot]
+        if entry is not None and entry[2] is not None:
+            entry[2].synchronize()
+        if entry is None or entry[0].numel() < total:
+            cap = int(total * 1.25) + 64
+            entry = [torch.empty(cap, dtype=torch.uint8, pin_memory=True),
+                     torch.empty(cap, dtype=torch.uint8, device=self.device), None]
+            self._stage[slot] = entry
+        host, dev = entry[0], entry[1]
+        hv = host.numpy()
+        for a, o in zip(arrays, offs):
+            hv[o:o + a.nbytes] = np.ascontiguousarray(a).view(np.uint8).ravel()
+        stream = torch.cuda.current_stream(self.device)
+        dev[:total].copy_(host[:total], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        entry[2] = ev
+        out = []
+        for a, o in zip(arrays, offs):
+            dt = {np.dtype(np.float64): torch.float64, np.dtype(np.int32): torch.int32}[a.dtype]
+            out.append(dev[o:o + max(a.nbytes, 8)].view(dt)[: a.size].view(*a.shape) if a.size else
+                       torch.empty(a.shape, dtype=dt, device=self.device))
+        return out
+
     def rebuild_geometry(self) -> None:
         """Voxelize every sphere (psc3d.py:151-179 / 328-345) on the GPU."""
         torch = _torch()
         table = pack_particles(self._particles)
         tabs = build_tables(table, self._shape, self.periodic_geometry, self._chunk_cells)
         dev = self.device
-        parts = torch.from_numpy(tabs.particles).to(dev)
-        boxes = torch.from_numpy(tabs.boxes).to(dev)
-        chunks = torch.from_numpy(tabs.chunks).to(dev)
-        cfirst = torch.from_numpy(tabs.chunk_first).to(dev)
+        parts, boxes, chunks, cfirst = self._upload_tables(
+            [tabs.particles, tabs.boxes, tabs.chunks, tabs.chunk_first])
         C = int(tabs.chunks.shape[0])
         cap = max(1, C * tabs.chunk_cells)
         self._old_ent_cell = self._ent[0] if self._ent is not None else None

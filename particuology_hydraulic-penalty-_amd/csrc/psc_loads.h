@@ -1,7 +1,6 @@
-// Particle-load reduction bodies (psc3d.py:255-281, 375-407), shared by the
-// standalone K3 kernels and by the pipelined FP32 step launch, which runs
-// them in extra workgroups for earlier steps.  FP contraction is switched
-// off inside so that both translation units produce the same bits.
+// Particle-load reduction bodies (psc3d.py:255-281, 375-407) of the K3
+// kernels.  FP contraction is switched off inside, so the bits do not depend
+// on the translation unit that instantiates them.
 #pragma once
 
 #include "psc_common.h"

@@ -142,3 +142,71 @@ def test_vector_force_coupler_leaves_unlisted_atoms():
     lib.arrays["dragforce"][:] = -1.0
     cp.VectorForceCoupler(FakeLmp(lib)).set_dragforces(np.array([9, 3]), np.array([[1.0, 2, 3], [4, 5, 6]]))
     assert np.array_equal(lib.arrays["dragforce"], [[-1, -1, -1], [4, 5, 6], [1, 2, 3], [-1, -1, -1]])
+
+
+class FakeSolver:
+    """A solver whose mean u_z relaxes as 1 - exp(-t/50): enough to drive the
+    convergence loops without a GPU."""
+
+    def __init__(self):
+        self.t = 0
+        self.static_geometry = False
+        self.rebuilds = 0
+        self.readouts = 0
+
+    def _m(self):
+        return 1.0 - math.exp(-self.t / 50.0)
+
+    def rebuild_geometry(self):
+        self.rebuilds += 1
+
+    def collide_stream_psc(self):
+        self.t += 1
+        self.readouts += 1
+        return {"mean_uz": self._m(), "solid_volume_lu": 0.0}
+
+    def run(self, n):
+        self.t += int(n)
+
+    def mean_uz(self):
+        return self._m()
+
+
+def _reference_loop(min_steps, max_steps, tol):
+    """couple_psc_liggghts.py:284-296 on the fake solver."""
+    s = FakeSolver()
+    prev, residual = None, math.inf
+    for it in range(1, max_steps + 1):
+        cur = s.collide_stream_psc()["mean_uz"]
+        if prev is not None:
+            residual = abs(cur - prev) / max(abs(cur), 1.0e-30)
+        prev = cur
+        if it >= min_steps and residual < tol:
+            return it, residual, True
+    return max_steps, residual, False
+
+
+@pytest.mark.parametrize("K", [1, 2, 5, 16, 64])
+@pytest.mark.parametrize("tol", [1e-3, 1e-4, 1e-12])
+def test_converge_block_logic(K, tol):
+    it_ref, res_ref, ok_ref = _reference_loop(20, 600, tol)
+    s = FakeSolver()
+    it, res, ok, diag = cp.converge(s, min_steps=20, max_steps=600, tol=tol, check_every=K)
+    assert s.static_geometry is False and s.rebuilds == 1  # voxelized once, flag restored
+    assert ok == ok_ref and s.t == it
+    if K == 1:
+        assert (it, res) == (it_ref, res_ref)
+    elif ok:
+        assert it_ref <= it < it_ref + K and res < tol
+        assert s.readouts <= 2 * (-(-it // K)) + 2  # two readouts per block of K
+    else:
+        assert it == 600
+
+
+def test_transient_advance_steps_and_residual():
+    s = FakeSolver()
+    s.t = 10
+    prev = s.mean_uz()
+    n, res, ok, diag = cp.transient_advance(s, 7)
+    assert n == 7 and ok and s.t == 17 and s.readouts == 1
+    assert res == abs(diag["mean_uz"] - prev) / abs(diag["mean_uz"])

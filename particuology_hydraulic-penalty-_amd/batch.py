@@ -112,6 +112,7 @@ class BatchedPscD3Q19:
             if len(launch) > 2:
                 self._mode = str(launch[2])
         self._graph = None
+        self._imb_alt = None
         self._steps = 0
         self._voxelize(tabs)
 
@@ -172,20 +173,22 @@ class BatchedPscD3Q19:
         return entry
 
     # ------------------------------------------------------------------ step
-    def _launch_step(self, a, b, flags, grid, stream, mode=None):
+    def _launch_step(self, a, b, flags, grid, stream, mode=None, imb=None):
         bs = 512 if (flags & _lib.PSC_BATCH_WG512) else 256
         windows, items, nitems = self._items_for(bs, mode or self._mode)
-        _lib.call("psc_step_batched", a.data_ptr(), b.data_ptr(), self._csum.data_ptr(), self._imb.data_ptr(),
+        imb = self._imb if imb is None else imb
+        _lib.call("psc_step_batched", a.data_ptr(), b.data_ptr(), self._csum.data_ptr(), imb.data_ptr(),
                   self.Ntot, windows.data_ptr(), items.data_ptr(), nitems, self._s, self.tau,
                   flags, grid, stream)
 
-    def _launch_loads(self, stream):
+    def _launch_loads(self, stream, imb=None):
         if self.P == 0:
             return
         ec, el, en = self._ent
+        imb = self._imb if imb is None else imb
         _lib.call("psc_particle_loads", self._parts.data_ptr(), self._boxes.data_ptr(), self._chunks.data_ptr(),
                   self._cfirst.data_ptr(), self.P, self.C, self.chunk_cells, ec.data_ptr(), el.data_ptr(),
-                  en.data_ptr(), self._nnz.data_ptr(), self._csum.data_ptr(), self._imb.data_ptr(), 1, 1, 1,
+                  en.data_ptr(), self._nnz.data_ptr(), self._csum.data_ptr(), imb.data_ptr(), 1, 1, 1,
                   self.Ntot, self._partials.data_ptr(), self._loads.data_ptr(), stream)
 
     def step(self):
@@ -227,38 +230,54 @@ class BatchedPscD3Q19:
         self._flags, self._grid, self._mode = best[1], 0, best[2]
         self._tuned = True
 
-    def run(self, steps: int, graph: bool = True) -> None:
-        """``steps`` steps; with graph=True two steps are captured once into a
-        hipGraph (one persistent step launch + the loads per step) and
-        replayed."""
+    def run(self, steps: int, graph: bool = True, graph_steps: int = 8) -> None:
+        """``steps`` steps.  With graph=True, ``graph_steps`` (even) steps are
+        captured once into a hipGraph and replayed.  Inside the graph the load
+        reduction of step t runs on a side stream while step t+1 runs on the
+        main one: the steps alternate between two imb buffers, step t+2 waits
+        for the loads of step t (which read the buffer it overwrites), and the
+        graph ends joined.  Results are those of stepping one by one."""
         import torch
         steps = int(steps)
         self._tune()
         # the item tables are uploaded here, never inside the graph capture
         self._items_for(512 if (self._flags & _lib.PSC_BATCH_WG512) else 256, self._mode)
-        if not graph or steps < 2:
+        G = max(2, int(graph_steps) // 2 * 2)
+        if not graph or steps < G:
             for _ in range(steps):
                 self.step()
             return
         if self._graph is None:
             a, b = self._a, self._b
+            if self._imb_alt is None:
+                self._imb_alt = torch.zeros_like(self._imb)
+            imbs = (self._imb, self._imb_alt)
+            side = torch.cuda.Stream(device=self.device)
             torch.cuda.synchronize(self.device)
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                s = torch.cuda.current_stream(self.device).cuda_stream
-                self._launch_step(a, b, self._flags, self._grid, s)
-                self._launch_loads(s)
-                self._launch_step(b, a, self._flags, self._grid, s)
-                self._launch_loads(s)
-            self._graph = (g, a, b)
-        g, a, b = self._graph
+                main = torch.cuda.current_stream(self.device)
+                done = []
+                for j in range(G):
+                    src, dst = (a, b) if j % 2 == 0 else (b, a)
+                    if j >= 2:
+                        main.wait_event(done[j - 2])  # imbs[j % 2] is free again
+                    self._launch_step(src, dst, self._flags, self._grid, main.cuda_stream, imb=imbs[j % 2])
+                    side.wait_stream(main)
+                    self._launch_loads(side.cuda_stream, imb=imbs[j % 2])
+                    ev = torch.cuda.Event()
+                    ev.record(side)
+                    done.append(ev)
+                main.wait_stream(side)
+            self._graph = (g, a, b, side, done)
+        g, a, b = self._graph[:3]
         if self._a is not a:  # odd number of eager steps since capture
             self.step()
             steps -= 1
-        for _ in range(steps // 2):
+        for _ in range(steps // G):
             g.replay()
-        self._steps += 2 * (steps // 2)
-        if steps % 2:
+        self._steps += G * (steps // G)
+        for _ in range(steps % G):
             self.step()
 
     # -------------------------------------------------------------- readouts

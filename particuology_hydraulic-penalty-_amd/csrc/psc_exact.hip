@@ -511,9 +511,14 @@ __global__ __launch_bounds__(256) void k_clear_entries(const uint32_t* __restric
 // 2: one wavefront per particle adds its chunk partials.  Fixed thread
 // assignment and reduction trees: deterministic.  Bodies in psc_loads.h.
 // ---------------------------------------------------------------------------
+// Threads per chunk workgroup (64, 128 or 256: the same bits, see psc_loads.h).
+#ifndef PSC_LOADS_THREADS
+#define PSC_LOADS_THREADS 256
+#endif
+
 template <typename TI>
-__global__ __launch_bounds__(256) void k_loads_chunk(LoadsArgs a, const TI* __restrict__ imb, Dims d,
-                                                     double* __restrict__ partials) {
+__global__ __launch_bounds__(PSC_LOADS_THREADS) void k_loads_chunk(LoadsArgs a, const TI* __restrict__ imb, Dims d,
+                                                                   double* __restrict__ partials) {
   __shared__ double lds[6 * 4];
   loads_chunk_body<TI>(a, imb, d.nx, d.ny, d.nz, blockIdx.x, threadIdx.x, blockDim.x, partials, lds);
 }
@@ -772,9 +777,9 @@ int launch_loads(const double* parts, const int32_t* boxes, const int32_t* chunk
                      chunk_nnz, csum};
   if (nchunks > 0) {
     if (imbp == 2)
-      k_loads_chunk<double><<<nchunks, 256, 0, stream>>>(la, (const double*)imb, d, partials);
+      k_loads_chunk<double><<<nchunks, PSC_LOADS_THREADS, 0, stream>>>(la, (const double*)imb, d, partials);
     else
-      k_loads_chunk<float><<<nchunks, 256, 0, stream>>>(la, (const float*)imb, d, partials);
+      k_loads_chunk<float><<<nchunks, PSC_LOADS_THREADS, 0, stream>>>(la, (const float*)imb, d, partials);
     const int rc = check_launch("psc_particle_loads(chunks)");
     if (rc) return rc;
   }
@@ -786,8 +791,8 @@ int launch_loads_chunks(const LoadsArgs& la, const void* imb, int imbp, int nx, 
                         hipStream_t stream) {
   if (la.nchunks == 0) return PSC_OK;
   const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
-  if (imbp == 2) k_loads_chunk<double><<<la.nchunks, 256, 0, stream>>>(la, (const double*)imb, d, partials);
-  else k_loads_chunk<float><<<la.nchunks, 256, 0, stream>>>(la, (const float*)imb, d, partials);
+  if (imbp == 2) k_loads_chunk<double><<<la.nchunks, PSC_LOADS_THREADS, 0, stream>>>(la, (const double*)imb, d, partials);
+  else k_loads_chunk<float><<<la.nchunks, PSC_LOADS_THREADS, 0, stream>>>(la, (const float*)imb, d, partials);
   return check_launch("psc_particle_loads(chunks)");
 }
 

@@ -240,6 +240,24 @@ def bench_moving(args, psc3d, workloads, lib, torch, np):
         "torque_rel_err_fp32_vs_fp64": {"max": float(eT.max()), "median": float(np.median(eT))},
         "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        from oracle import psc_oracle as O
+        cn = 64
+        cnt = max(1, int(round(base.shape[0] * (cn / n) ** 3)))
+        ct_ = workloads.moving_window(seed=0, n=cn, count=cnt)
+        ora = O.OracleSolver(cn, [O.Sphere(*row) for row in ct_], tau=0.8, subsamples=3, static_geometry=False,
+                             layout="sparse", periodic_geometry=True)
+        ora.collide_stream_psc()
+        cs = 4
+        tc = time.perf_counter()
+        for _ in range(cs):
+            workloads.advance_centres(ct_, cn)
+            ora.particles = [O.Sphere(*row) for row in ct_]
+            ora.collide_stream_psc()
+        dt = time.perf_counter() - tc
+        out["cpu_baseline"] = {"value": cn ** 3 * cs / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+                               "sample": f"NumPy FP64 oracle, {cs} moving steps (re-voxelized each step) of a {cn}^3 "
+                                         f"periodic window with {cnt} spheres (same density) in {dt:.1f} s, one thread"}
     print(json.dumps(out))
 
 
@@ -296,6 +314,21 @@ def bench_windows(args, workloads, torch, np):
         "porosity_range": [float(por.min()), float(por.max())],
         "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        from oracle import psc_oracle as O
+        w0 = int(np.argmin([L for L, _, _ in wins]))
+        L0, t0_, _ = wins[w0]
+        ora = O.OracleSolver(L0, [O.Sphere(*row) for row in t0_], tau=0.8, subsamples=3, body_force=(0.0, 0.0, 1.0e-6),
+                             static_geometry=True, layout="sparse", periodic_geometry=True)
+        ora.collide_stream_psc()
+        cs = 8
+        tc = time.perf_counter()
+        for _ in range(cs):
+            ora.collide_stream_psc()
+        dt = time.perf_counter() - tc
+        out["cpu_baseline"] = {"value": L0 ** 3 * cs / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+                               "sample": f"NumPy FP64 oracle, {cs} steps of window {w0} ({L0}^3, {len(t0_)} particles, "
+                                         f"periodic) in {dt:.1f} s, one thread"}
     print(json.dumps(out))
 
 

@@ -516,11 +516,23 @@ __global__ __launch_bounds__(256) void k_clear_entries(const uint32_t* __restric
 #define PSC_LOADS_THREADS 256
 #endif
 
+// Workgroups of the chunk pass: 0 = one per chunk; otherwise at most this
+// many, each summing chunks blockIdx.x, blockIdx.x + gridDim.x, ... (the
+// per-chunk sums and their bits do not depend on the grid).
+#ifndef PSC_LOADS_GRID
+#define PSC_LOADS_GRID 0
+#endif
+
 template <typename TI>
 __global__ __launch_bounds__(PSC_LOADS_THREADS) void k_loads_chunk(LoadsArgs a, const TI* __restrict__ imb, Dims d,
                                                                    double* __restrict__ partials) {
   __shared__ double lds[6 * 4];
-  loads_chunk_body<TI>(a, imb, d.nx, d.ny, d.nz, blockIdx.x, threadIdx.x, blockDim.x, partials, lds);
+  for (int c = blockIdx.x; c < a.nchunks; c += gridDim.x)
+    loads_chunk_body<TI>(a, imb, d.nx, d.ny, d.nz, c, threadIdx.x, blockDim.x, partials, lds);
+}
+
+inline int loads_grid(int nchunks) {
+  return (PSC_LOADS_GRID > 0 && PSC_LOADS_GRID < nchunks) ? PSC_LOADS_GRID : nchunks;
 }
 
 __global__ __launch_bounds__(64) void k_loads_final(const int32_t* __restrict__ chunk_first,
@@ -777,9 +789,9 @@ int launch_loads(const double* parts, const int32_t* boxes, const int32_t* chunk
                      chunk_nnz, csum};
   if (nchunks > 0) {
     if (imbp == 2)
-      k_loads_chunk<double><<<nchunks, PSC_LOADS_THREADS, 0, stream>>>(la, (const double*)imb, d, partials);
+      k_loads_chunk<double><<<loads_grid(nchunks), PSC_LOADS_THREADS, 0, stream>>>(la, (const double*)imb, d, partials);
     else
-      k_loads_chunk<float><<<nchunks, PSC_LOADS_THREADS, 0, stream>>>(la, (const float*)imb, d, partials);
+      k_loads_chunk<float><<<loads_grid(nchunks), PSC_LOADS_THREADS, 0, stream>>>(la, (const float*)imb, d, partials);
     const int rc = check_launch("psc_particle_loads(chunks)");
     if (rc) return rc;
   }
@@ -791,8 +803,8 @@ int launch_loads_chunks(const LoadsArgs& la, const void* imb, int imbp, int nx, 
                         hipStream_t stream) {
   if (la.nchunks == 0) return PSC_OK;
   const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
-  if (imbp == 2) k_loads_chunk<double><<<la.nchunks, PSC_LOADS_THREADS, 0, stream>>>(la, (const double*)imb, d, partials);
-  else k_loads_chunk<float><<<la.nchunks, PSC_LOADS_THREADS, 0, stream>>>(la, (const float*)imb, d, partials);
+  if (imbp == 2) k_loads_chunk<double><<<loads_grid(la.nchunks), PSC_LOADS_THREADS, 0, stream>>>(la, (const double*)imb, d, partials);
+  else k_loads_chunk<float><<<loads_grid(la.nchunks), PSC_LOADS_THREADS, 0, stream>>>(la, (const float*)imb, d, partials);
   return check_launch("psc_particle_loads(chunks)");
 }
 

@@ -55,10 +55,10 @@ def parse():
                     help="fixed psc_step launch 'block_z,block_y,flags' instead of autotuning (for profiling)")
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--workload", default="fixed_array",
-                    choices=["fixed_array", "moving", "windows", "adjoint", "bounce_back"],
+                    choices=["fixed_array", "moving", "windows", "adjoint", "bounce_back", "coupling"],
                     help="fixed_array: 128^3 permeability (default); moving: 512^3 window, 2000 moving spheres; "
                          "windows: 64 batched pore windows; adjoint: 96^3 permeability gradient; "
-                         "bounce_back: 256^3 plain bounce-back window")
+                         "bounce_back: 256^3 plain bounce-back window; coupling: one exchange's flow solve")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -503,6 +503,56 @@ def bench_bounce_back(args, workloads, torch, np):
     print(json.dumps(out))
 
 
+def bench_coupling(args, workloads, torch, np):
+    """One coupling exchange's flow solve (the drivers' lbm_converge): a
+    40^3 window (the coupling driver's default resolution) of 12 pebbles r=6
+    and 60 fines r=1.2, tau 0.72, force 2e-7 along z, s=2, residual 1e-8 on
+    mean u_z, at most 2000 steps.  Times converge with the residual read after
+    every step (the reference's loop) and read every 32 steps (native graph
+    loop between reads), FP64 and FP32."""
+    psc3d = importlib.import_module(PKG + ".psc3d")
+    cpl = importlib.import_module(PKG + ".coupling")
+    n = 40
+    rng = np.random.default_rng(0)
+    peb = workloads.rsa_spheres(n, 12, 6.0, seed=0, margin=-6.0)
+    fines = rng.uniform(0, n, (60, 3))
+    table = np.zeros((72, 10))
+    table[:12, 0:3], table[:12, 3] = peb, 6.0
+    table[12:, 0:3], table[12:, 3] = fines, 1.2
+    rows = {}
+    for prec in ("fp64", "fp32"):
+        for K in (1, 32):
+            sol = psc3d.SparsePscD3Q19(n, [psc3d.Particle3D(*r) for r in table], tau=0.72, subsamples=2,
+                                       body_force_z=2.0e-7, precision=prec, device="cuda:0")
+            cpl.converge(sol, 20, 40, 1e-8, check_every=K)  # warm-up (kernel loading, graph capture)
+            sol = psc3d.SparsePscD3Q19(n, [psc3d.Particle3D(*r) for r in table], tau=0.72, subsamples=2,
+                                       body_force_z=2.0e-7, precision=prec, device="cuda:0")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            it, res, ok, diag = cpl.converge(sol, 20, 2000, 1e-8, check_every=K)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            rows[f"{prec}_K{K}"] = {"steps": it, "residual": res, "converged": ok, "seconds": wall,
+                                    "MLUPS": n ** 3 * it / wall / 1e6, "us_per_step": wall / it * 1e6}
+            log(f"{prec} K={K}: {it} steps in {wall*1e3:.1f} ms ({wall/it*1e6:.1f} us/step)", args.quiet)
+    best = rows["fp32_K32"]
+    out = {
+        "metric": METRIC + " (coupling-exchange configuration)",
+        "value": best["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": best["steps"], "warmup": 40,
+        "ms_per_step": best["us_per_step"] / 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "coupling exchange flow solve: 40^3 window, 12 pebbles r=6 + 60 fines r=1.2 (seed 0), "
+                               "tau 0.72, Fz 2e-7, s=2, converge to 1e-8 relative change of mean u_z (<= 2000 "
+                               "steps); value = FP32 with the residual read every 32 steps"},
+        "roofline": None,
+        "runs": rows,
+        "speedup_K32_over_K1": {p: rows[f"{p}_K1"]["us_per_step"] / rows[f"{p}_K32"]["us_per_step"]
+                                for p in ("fp64", "fp32")},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -525,6 +575,8 @@ def main():
         return bench_adjoint(args, workloads, torch, np)
     if args.workload == "bounce_back":
         return bench_bounce_back(args, workloads, torch, np)
+    if args.workload == "coupling":
+        return bench_coupling(args, workloads, torch, np)
     n = args.n
     table = workloads.fixed_array(seed=0, n=n)
     total = args.warmup + args.steps

@@ -136,17 +136,19 @@ def test_voxelizer_counts_random(psc3d):
         assert np.array_equal(got, ref_cnt)
 
 
-@pytest.mark.parametrize("precision,block", [("fp64", (64, 4)), ("fp32", (64, 4)), ("fp32", (64, 2)),
-                                             ("fp32", (64, 1))])
-def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, block):
+@pytest.mark.parametrize("precision,block,pipeline", [("fp64", (64, 4), None), ("fp32", (64, 4), True),
+                                                      ("fp32", (64, 2), True), ("fp32", (64, 1), True),
+                                                      ("fp32", (64, 4), None), ("fp64", (64, 4), False)])
+def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, block, pipeline):
     """psc_run_steps (direct launches and hipGraph replays, with eager reads in
-    between the replays; pipelined load reduction for FP32 with 64/128/256-
-    thread workgroups) gives the same state and loads as stepping one call at
-    a time."""
+    between the replays; load reduction in line or pipelined on the side
+    stream, with 64/128/256-thread workgroups) gives the same state and loads
+    as stepping one call at a time."""
     g = G.load("perm24")
     a, _ = _solver(psc3d, g, precision, block=block)
     b, _ = _solver(psc3d, g, precision, block=block)
     c, _ = _solver(psc3d, g, precision, block=block)
+    b.pipeline_loads = c.pipeline_loads = pipeline
     for chunk in (16, 16, 5, 32):
         for _ in range(chunk):
             a.collide_stream_psc()

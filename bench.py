@@ -314,6 +314,20 @@ def bench_windows(args, workloads, torch, np):
         "porosity_range": [float(por.min()), float(por.max())],
         "cpu_baseline": None,
     }
+    # FP32 batch vs FP64 single-window solvers on the three smallest windows
+    psc3d = importlib.import_module(PKG + ".psc3d")
+    small = list(np.argsort([L for L, _, _ in wins])[:3])
+    errs = []
+    for w in small:
+        L, t, _ = wins[w]
+        s64 = psc3d.SparsePscD3Q19(L, t, tau=0.8, subsamples=3, body_force=(0.0, 0.0, 1.0e-6), static_geometry=True,
+                                   periodic_geometry=True, precision="fp64", device="cuda:0")
+        s64.run(warm + steps)
+        k64 = s64.permeability(axis=2)
+        errs.append({"window": int(w), "L": int(L), "k_fp32": float(k[w]), "k_fp64": k64,
+                     "rel_err": abs(float(k[w]) - k64) / abs(k64)})
+        del s64
+    out["k_fp32_vs_fp64_smallest_windows"] = errs
     if not args.no_cpu_baseline:
         from oracle import psc_oracle as O
         w0 = int(np.argmin([L for L, _, _ in wins]))

@@ -252,6 +252,7 @@ typedef struct PscStepPlan {
   double* partials;
   double* partials_alt; /* optional second partials buffer (pipelining)      */
   double* loads;
+  int32_t fuse_loads;   /* k > 0: fuse the load reduction into the step kernels */
 } PscStepPlan;
 
 /* nsteps repetitions of collide_stream_psc with static geometry
@@ -265,6 +266,15 @@ typedef struct PscStepPlan {
  * of step k runs on a library-owned side stream concurrently with step k+1
  * (fork/join through events, captured into the graph as parallel branches);
  * every step's loads are computed and d_loads ends with the last step's.
+ * With fuse_loads = k > 0, FP32 pull storage, no solid velocity, particles
+ * and imb_alt/partials_alt set, the loads are fused instead: the kernel of
+ * step k also sums step k-1's chunks (extra workgroups, one per chunk) and
+ * adds up step k-2's chunk partials per particle (one per particle), so a
+ * step is one launch; the extra workgroups lead the grid, one in every k
+ * workgroups (k = 1: all ahead of the cell workgroups).  The loads still
+ * pending after the last step run after the loop.  Every step's loads are
+ * computed, with the bits of the separate reduction, and d_loads ends with
+ * the last step's.
  * use_graph = 1 captures `graph_steps` steps (even, >= 2) once per plan into
  * a hipGraph and replays it; the remainder runs as direct launches.  Graphs
  * are cached per plan contents until psc_graph_cache_clear(). */

@@ -70,7 +70,7 @@ class PscStepPlan(ct.Structure):
         ("particles", _P), ("boxes", _P), ("chunks", _P), ("chunk_first", _P),
         ("nparticles", _I32), ("nchunks", _I32), ("chunk_cells", _I32),
         ("ent_cell", _P), ("ent_loc", _P), ("ent_cnt", _P), ("chunk_nnz", _P), ("partials", _P),
-        ("partials_alt", _P), ("loads", _P),
+        ("partials_alt", _P), ("loads", _P), ("fuse_loads", _I32),
     ]
 
 

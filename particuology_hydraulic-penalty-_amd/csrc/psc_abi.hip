@@ -54,6 +54,9 @@ int64_t moments_blocks(int64_t);
 int launch_clear_entries(const uint32_t*, const int32_t*, int, int, int, int, int, uint32_t*, void*, int, hipStream_t);
 int launch_loads_chunks(const LoadsArgs&, const void*, int, int, int, int, double*, hipStream_t);
 int launch_loads_final(const int32_t*, int, const double*, double*, hipStream_t);
+int launch_step_f32_loads(const float*, float*, const uint32_t*, float*, const float*, const LoadsArgs&, double*,
+                          const double*, double*, int, int, int, int, int, double, double, double, double, int, int,
+                          int, hipStream_t);
 
 int launch_smooth_alpha(const double*, const int32_t*, const int32_t*, int, int, int, int, int, uint32_t*,
                         unsigned long long*, double*, float*, hipStream_t);
@@ -124,6 +127,31 @@ bool dims_ok(int nx, int ny, int nz) {
 }  // namespace psc
 
 using namespace psc;
+
+namespace psc {
+// Argument checks of psc_step; resolves the default block shape into *bz, *by.
+int step_args(const void* d_in, const void* d_out, const uint32_t* d_count_sum, const void* d_us_num, int layout,
+              int nx, int ny, int nz, int subsamples, double tau, int flags, int* bz, int* by) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_step: bad lattice %dx%dx%d", nx, ny, nz);
+  if (d_in == nullptr || d_out == nullptr || d_count_sum == nullptr) return set_error(PSC_EINVAL, "psc_step: null");
+  if (d_in == d_out) return set_error(PSC_EINVAL, "psc_step: input and output alias");
+  if (layout != PSC_PUSH && layout != PSC_PULL) return set_error(PSC_EINVAL, "psc_step: layout %d", layout);
+  if (subsamples < 0 || subsamples * subsamples * subsamples > 65535)
+    return set_error(PSC_EINVAL, "psc_step: subsamples %d", subsamples);
+  if (subsamples == PSC_ALPHA_FIELD && d_us_num != nullptr)
+    return set_error(PSC_EINVAL, "psc_step: a solid-fraction field (subsamples 0) takes no solid velocity");
+  if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_step: tau %g must exceed 1/2", tau);
+  // default: 64-cell row tiles, 32 when rows are a multiple of 32 but not of 64
+  if (*bz <= 0) *bz = (nz >= 64 && !(nz % 64 != 0 && nz % 32 == 0)) ? 64 : 32;
+  if (*by <= 0) *by = 256 / *bz > 0 ? 256 / *bz : 1;
+  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK)) != 0)
+    return set_error(PSC_EINVAL, "psc_step: unknown flags 0x%x", flags);
+  if (((flags & PSC_STEP_CPT_MASK) >> 16) > 2) return set_error(PSC_EINVAL, "psc_step: cells per thread > 2");
+  if (*bz * *by > 256 || *bz * *by < 1 || ((*bz * *by) % 64) != 0)
+    return set_error(PSC_ELAUNCH, "psc_step: block %dx%d must hold a multiple of 64 threads, at most 256", *bz, *by);
+  return PSC_OK;
+}
+}  // namespace psc
 
 extern "C" {
 
@@ -205,24 +233,10 @@ int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, in
 int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const void* d_us_num, void* d_imb,
              int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double tau,
              double fx, double fy, double fz, int32_t block_z, int32_t block_y, int32_t flags, void* stream) {
-  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_step: bad lattice %dx%dx%d", nx, ny, nz);
-  if (d_in == nullptr || d_out == nullptr || d_count_sum == nullptr) return set_error(PSC_EINVAL, "psc_step: null");
-  if (d_in == d_out) return set_error(PSC_EINVAL, "psc_step: input and output alias");
-  if (layout != PSC_PUSH && layout != PSC_PULL) return set_error(PSC_EINVAL, "psc_step: layout %d", layout);
-  if (subsamples < 0 || subsamples * subsamples * subsamples > 65535)
-    return set_error(PSC_EINVAL, "psc_step: subsamples %d", subsamples);
-  if (subsamples == PSC_ALPHA_FIELD && d_us_num != nullptr)
-    return set_error(PSC_EINVAL, "psc_step: a solid-fraction field (subsamples 0) takes no solid velocity");
-  if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_step: tau %g must exceed 1/2", tau);
   int bz = block_z, by = block_y;
-  // default: 64-cell row tiles, 32 when rows are a multiple of 32 but not of 64
-  if (bz <= 0) bz = (nz >= 64 && !(nz % 64 != 0 && nz % 32 == 0)) ? 64 : 32;
-  if (by <= 0) by = 256 / bz > 0 ? 256 / bz : 1;
-  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK)) != 0)
-    return set_error(PSC_EINVAL, "psc_step: unknown flags 0x%x", flags);
-  if (((flags & PSC_STEP_CPT_MASK) >> 16) > 2) return set_error(PSC_EINVAL, "psc_step: cells per thread > 2");
-  if (bz * by > 256 || bz * by < 1 || ((bz * by) % 64) != 0)
-    return set_error(PSC_ELAUNCH, "psc_step: block %dx%d must hold a multiple of 64 threads, at most 256", bz, by);
+  const int rc = psc::step_args(d_in, d_out, d_count_sum, d_us_num, layout, nx, ny, nz, subsamples, tau, flags, &bz,
+                                &by);
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (storage == PSC_F32_DEV)
     return launch_step_f32((const float*)d_in, (float*)d_out, d_count_sum, (const float*)d_us_num, (float*)d_imb,
@@ -433,6 +447,7 @@ struct GraphEntry {
   PscStepPlan plan;
   int steps;
   int start_parity;
+  int kind;  // 0: run_sequence; 1: fused steps continuing a sequence (both loads stages pending, no flush)
   hipGraphExec_t exec;
 };
 std::mutex g_graph_mu;
@@ -476,6 +491,7 @@ PscStepPlan normalized(const PscStepPlan& p) {
   k.partials = p.partials;
   k.partials_alt = p.partials_alt;
   k.loads = p.loads;
+  k.fuse_loads = p.fuse_loads;
   return k;
 }
 
@@ -554,7 +570,64 @@ int side_resources(SideRes** out) {
 // joined back into `s` at the end, so on return every step's loads have been
 // computed and d_loads holds the last step's.  Otherwise steps and loads run
 // in order on `s`.
+// Fused form (fuse_loads > 0, FP32 pull storage, no solid velocity, particles,
+// imb_alt and partials_alt set): a step with input parity q writes imb[q]
+// and, in the same launch, sums the chunks of the previous step's imb[q^1]
+// into partials[q^1] and adds up partials[q] (the step before that) into
+// d_loads, so a step is one launch.  *pc / *pf say whether a previous step's
+// chunks / partials are pending; a sequence starts with neither.
+int run_fused(const PscStepPlan& p, int n, int* par, bool* pc, bool* pf, hipStream_t s) {
+  const LoadsArgs la{p.particles, p.boxes, p.chunks, p.chunk_first, p.nparticles, p.nchunks, p.chunk_cells,
+                     p.ent_cell, p.ent_loc, p.ent_cnt, p.chunk_nnz, p.count_sum};
+  void* imbs[2] = {p.imb, p.imb_alt};
+  double* prt[2] = {p.partials, p.partials_alt};
+  int bz = p.block_z, by = p.block_y;
+  if (bz <= 0) bz = (p.nz >= 64 && !(p.nz % 64 != 0 && p.nz % 32 == 0)) ? 64 : 32;
+  if (by <= 0) by = 256 / bz > 0 ? 256 / bz : 1;
+  for (int k = 0; k < n; ++k) {
+    const int q = *par;
+    const int rc = launch_step_f32_loads((const float*)(q ? p.state_b : p.state_a),
+                                         (float*)(q ? p.state_a : p.state_b), p.count_sum, (float*)imbs[q],
+                                         *pc ? (const float*)imbs[q ^ 1] : nullptr, la, prt[q ^ 1],
+                                         *pf ? prt[q] : nullptr, p.loads, p.fuse_loads, p.nx, p.ny, p.nz,
+                                         p.subsamples, p.tau,
+                                         p.fx, p.fy, p.fz, bz, by, p.flags, s);
+    if (rc) return rc;
+    *pf = *pc;
+    *pc = true;
+    *par ^= 1;
+  }
+  return PSC_OK;
+}
+
+// The pending loads after run_fused, oldest first, so d_loads ends with the
+// last step's (the final pass also writes zeros for particles without chunks).
+int flush_fused(const PscStepPlan& p, int par, bool pc, bool pf, hipStream_t s) {
+  const LoadsArgs la{p.particles, p.boxes, p.chunks, p.chunk_first, p.nparticles, p.nchunks, p.chunk_cells,
+                     p.ent_cell, p.ent_loc, p.ent_cnt, p.chunk_nnz, p.count_sum};
+  void* imbs[2] = {p.imb, p.imb_alt};
+  double* prt[2] = {p.partials, p.partials_alt};
+  int rc = PSC_OK;
+  if (pf) rc = launch_loads_final(p.chunk_first, p.nparticles, prt[par], p.loads, s);
+  if (rc == PSC_OK && pc) {
+    rc = launch_loads_chunks(la, imbs[par ^ 1], 1, p.nx, p.ny, p.nz, prt[par ^ 1], s);
+    if (rc == PSC_OK) rc = launch_loads_final(p.chunk_first, p.nparticles, prt[par ^ 1], p.loads, s);
+  }
+  return rc;
+}
+
+bool fused_plan(const PscStepPlan& p) {
+  return p.fuse_loads > 0 && p.nparticles > 0 && p.storage == PSC_F32_DEV && p.layout == PSC_PULL &&
+         p.us_num == nullptr && p.imb != nullptr && p.imb_alt != nullptr && p.partials != nullptr &&
+         p.partials_alt != nullptr;
+}
+
 int run_sequence(const PscStepPlan& p, int n, int* par, hipStream_t s) {
+  if (fused_plan(p)) {
+    bool pc = false, pf = false;
+    const int rc = run_fused(p, n, par, &pc, &pf, s);
+    return rc ? rc : flush_fused(p, *par, pc, pf, s);
+  }
   const bool overlap = p.nparticles > 0 && p.imb != nullptr && p.imb_alt != nullptr && p.partials != nullptr &&
                        p.partials_alt != nullptr;
   if (!overlap) {
@@ -597,10 +670,11 @@ int run_sequence(const PscStepPlan& p, int n, int* par, hipStream_t s) {
   return PSC_OK;
 }
 
-int graph_for(const PscStepPlan& p, int steps, int parity, hipGraphExec_t* out) {
+int graph_for(const PscStepPlan& p, int steps, int parity, int kind, hipGraphExec_t* out) {
   std::lock_guard<std::mutex> lock(g_graph_mu);
   for (const GraphEntry& e : g_graphs) {
-    if (e.steps == steps && e.start_parity == parity && std::memcmp(&e.plan, &p, sizeof(p)) == 0) {
+    if (e.steps == steps && e.start_parity == parity && e.kind == kind &&
+        std::memcmp(&e.plan, &p, sizeof(p)) == 0) {
       *out = e.exec;
       return PSC_OK;
     }
@@ -611,7 +685,8 @@ int graph_for(const PscStepPlan& p, int steps, int parity, hipGraphExec_t* out) 
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return set_error((int)e, "psc_run_steps: begin capture: %s", hipGetErrorString(e));
   int par = parity;
-  const int rc = run_sequence(p, steps, &par, s);
+  bool pc = true, pf = true;
+  const int rc = kind == 1 ? run_fused(p, steps, &par, &pc, &pf, s) : run_sequence(p, steps, &par, s);
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(s, &graph);
   if (rc) {
@@ -627,7 +702,7 @@ int graph_for(const PscStepPlan& p, int steps, int parity, hipGraphExec_t* out) 
     (void)hipGraphExecDestroy(g_graphs.front().exec);
     g_graphs.erase(g_graphs.begin());
   }
-  g_graphs.push_back(GraphEntry{p, steps, parity, exec});
+  g_graphs.push_back(GraphEntry{p, steps, parity, kind, exec});
   *out = exec;
   return PSC_OK;
 }
@@ -645,9 +720,39 @@ extern "C" int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t us
   const PscStepPlan key = normalized(*plan);
   int par = *parity;
   int done = 0;
+  if (fused_plan(key)) {
+    // two direct steps (after them both loads stages are pending), replays
+    // of a graph of fused steps, the remainder, then the pending loads
+    int bz = key.block_z, by = key.block_y;
+    int rc = step_args(key.state_a, key.state_b, key.count_sum, key.us_num, key.layout, key.nx, key.ny, key.nz,
+                       key.subsamples, key.tau, key.flags, &bz, &by);
+    if (rc) return rc;
+    if (key.imb == key.imb_alt || key.partials == key.partials_alt)
+      return set_error(PSC_EINVAL, "psc_run_steps: imb/partials buffers alias");
+    bool pc = false, pf = false;
+    const int head = nsteps < 2 ? nsteps : 2;
+    rc = run_fused(key, head, &par, &pc, &pf, s);
+    if (rc) return rc;
+    done = head;
+    if (use_graph && nsteps - done >= graph_steps) {
+      hipGraphExec_t exec = nullptr;
+      rc = graph_for(key, graph_steps, par, 1, &exec);
+      if (rc) return rc;
+      while (nsteps - done >= graph_steps) {
+        const hipError_t e = hipGraphLaunch(exec, s);
+        if (e != hipSuccess) return set_error((int)e, "psc_run_steps: graph launch: %s", hipGetErrorString(e));
+        done += graph_steps;
+      }
+    }
+    rc = run_fused(key, nsteps - done, &par, &pc, &pf, s);
+    if (rc == PSC_OK) rc = flush_fused(key, par, pc, pf, s);
+    if (rc) return rc;
+    *parity = par;
+    return PSC_OK;
+  }
   if (use_graph && nsteps >= graph_steps) {
     hipGraphExec_t exec = nullptr;
-    int rc = graph_for(key, graph_steps, par, &exec);
+    int rc = graph_for(key, graph_steps, par, 0, &exec);
     if (rc) return rc;
     while (nsteps - done >= graph_steps) {
       const hipError_t e = hipGraphLaunch(exec, s);

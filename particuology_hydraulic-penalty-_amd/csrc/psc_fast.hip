@@ -28,6 +28,7 @@
 // in any arithmetic, because a direction and its opposite compute them
 // identically.  The Guo term uses the same k_i for its linear part.
 #include "psc_common.h"
+#include "psc_loads.h"
 #include "../../include/psc_hip.h"
 
 namespace psc {
@@ -222,6 +223,66 @@ __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin,
     const int iy = iy0 + c;
     if (iy < d.ny)
       step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, usn, imb, (size_t)d.nx * d.ny * d.nz, d, k, ix, iy, iz);
+  }
+}
+
+// Static pull step with the particle-load reduction of earlier steps in the
+// same launch.  Besides the cell workgroups (the (z tile, y tile, x) order of
+// k_step_f32) the grid holds nchunk chunk workgroups, which sum the chunks of
+// the previous step's imb into part_out, and nfinal final workgroups, which
+// add up the per-particle partials that the previous launch wrote into
+// part_prev (the loads of the step before the previous one).  Each launch
+// only reads what earlier launches wrote, so kernel boundaries order all of
+// it, with no device-scope fences.  The extra workgroups come first, one
+// after every period - 1 cell workgroups (period 1: all of them ahead of the
+// cells), so their latency-bound gathers start at once and overlap the cell
+// workgroups' streaming while they hold only part of the CU slots.
+struct FusedLoads {
+  LoadsArgs a;
+  const float* imb_prev;     // imb written by the previous step
+  double* part_out;          // 6 per chunk: the previous step's chunk sums
+  const double* part_prev;   // 6 per chunk, written by the previous launch
+  double* loads;             // 6 per particle
+  int nchunk, nfinal;        // extra workgroups of each kind (0: none)
+  int period;                // extra workgroups lead, one per period workgroups (>= 1)
+};
+
+template <bool NT, int CPT>
+__global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict__ gin, float* __restrict__ gout,
+                                                        const uint32_t* __restrict__ csum, float* __restrict__ imb,
+                                                        Dims d, StepConsts k, FusedLoads fl, int gz, int gy) {
+  int b = blockIdx.x;
+  const int nextra = fl.nchunk + fl.nfinal;
+  if (nextra > 0) {
+    if (b < nextra * fl.period) {
+      const int e = b / fl.period, r = b - e * fl.period;
+      if (r == fl.period - 1) {
+        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+        if (e < fl.nchunk) {
+          __shared__ double lds[6 * 4];
+          loads_chunk_body<float, 4>(fl.a, fl.imb_prev, d.nx, d.ny, d.nz, e, tid, blockDim.x * blockDim.y,
+                                     fl.part_out, lds);
+        } else {
+          loads_final_body(fl.a.chunk_first, e - fl.nchunk, tid, fl.part_prev, fl.loads);
+        }
+        return;
+      }
+      b = e * (fl.period - 1) + r;
+    } else {
+      b -= nextra;
+    }
+  }
+  const int t = b / gz;
+  const int ix = t / gy;
+  if (ix >= d.nx) return;
+  const int iz = (b - t * gz) * blockDim.x + threadIdx.x;
+  const int iy0 = ((t - ix * gy) * blockDim.y + threadIdx.y) * CPT;
+  if (iz >= d.nz) return;
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int iy = iy0 + c;
+    if (iy < d.ny)
+      step_cell<true, false, true, NT>(gin, gout, csum, nullptr, imb, (size_t)d.nx * d.ny * d.nz, d, k, ix, iy, iz);
   }
 }
 
@@ -420,6 +481,67 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
 #undef PSC_F32_CASES
 #undef PSC_F32_CASE
   return set_error(PSC_EINVAL, "psc_step(f32): no kernel variant");
+}
+
+// Static FP32 pull step writing imb.  With imb_prev set, the launch also sums
+// the chunks of imb_prev (the previous step's imb) into part_out; with
+// part_prev set, it also adds up part_prev's per-particle partials into loads.
+// period: the extra workgroups lead, one in every `period` workgroups.  Same
+// flags as launch_step_f32.
+int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, float* imb, const float* imb_prev,
+                          const LoadsArgs& la, double* part_out, const double* part_prev, double* loads, int period,
+                          int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz, int bz, int by,
+                          int nt, hipStream_t stream) {
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
+  const int cpt = (nt >> 16) & 3 ? (nt >> 16) & 3 : 1;
+  const int bpc = (nt >> 8) & 0xff;
+  size_t lds = 0;
+  if (bpc > 0) {
+    lds = (kLdsPerCU - kLdsHeadroom) / (size_t)bpc;
+    if (lds > 65536) lds = 65536;
+  }
+  if (cpt != 1 && cpt != 2) return set_error(PSC_EINVAL, "psc_step(f32+loads): cells per thread %d", cpt);
+  if ((bz * by) % 64 != 0 || bz * by > 256) return set_error(PSC_ELAUNCH, "psc_step(f32+loads): block %dx%d", bz, by);
+  if ((size_t)nx * ny * nz * sizeof(float) > 0xffffffffull)
+    return set_error(PSC_EINVAL, "psc_step(f32+loads): a direction plane must stay below 4 GiB (N < 2^30 cells)");
+  const int rows = (ny + cpt - 1) / cpt;
+  const int gz = (nz + bz - 1) / bz, gy = (rows + by - 1) / by;
+  const long long ncell = (long long)gz * gy * nx;
+  FusedLoads fl{la, imb_prev, part_out, part_prev, loads, 0, 0, 1};
+  if (imb_prev != nullptr) {
+    if (part_out == nullptr) return set_error(PSC_EINVAL, "psc_step(f32+loads): null partials");
+    fl.nchunk = la.nchunks;
+  }
+  if (part_prev != nullptr) {
+    if (loads == nullptr) return set_error(PSC_EINVAL, "psc_step(f32+loads): null loads");
+    fl.nfinal = la.np;
+  }
+  if (period < 1) return set_error(PSC_EINVAL, "psc_step(f32+loads): period %d", period);
+  fl.period = period;
+  long long grid = ncell;
+  const long long extra = (long long)fl.nchunk + fl.nfinal;
+  if (extra > 0) {
+    grid = ncell + extra;
+    if (grid < extra * period) grid = extra * period;  // every extra workgroup inside the interleaved head
+  }
+  if (grid > 0x7fffffffll) return set_error(PSC_EINVAL, "psc_step(f32+loads): grid too large");
+  StepConsts k;
+  k.field = s == 0;
+  k.inv_s3 = k.field ? 0.0f : (float)(1.0 / (double)(s * s * s));
+  k.tau = (float)tau;
+  k.itau = (float)(1.0 / tau);
+  k.omt = (float)(1.0 - 1.0 / tau);
+  k.pref = (float)(1.0 - 0.5 / tau);
+  k.Fx = (float)fx;
+  k.Fy = (float)fy;
+  k.Fz = (float)fz;
+  const dim3 block(bz, by, 1);
+  const bool ntb = (nt & 1) != 0;
+  if (ntb && cpt == 2) k_step_f32_loads<true, 2><<<(unsigned)grid, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
+  else if (ntb) k_step_f32_loads<true, 1><<<(unsigned)grid, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
+  else if (cpt == 2) k_step_f32_loads<false, 2><<<(unsigned)grid, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
+  else k_step_f32_loads<false, 1><<<(unsigned)grid, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
+  return check_launch("psc_step(f32+loads)");
 }
 
 int launch_bb_step_f32(const float* in, float* out, const uint8_t* solid, int nx, int ny, int nz, double tau,

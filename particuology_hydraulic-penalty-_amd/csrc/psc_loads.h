@@ -8,7 +8,7 @@
 namespace psc {
 
 constexpr int kLocBits = 10;  // box-local coordinate bits of an entry (extent <= 1024)
-constexpr int kLoadsKB = 8;   // entries per thread per round
+constexpr int kLoadsKB = 8;   // entries per thread per round (register batch; does not change the bits)
 
 struct LoadsArgs {
   const double* parts;       // P x 10
@@ -26,12 +26,13 @@ struct LoadsArgs {
 // Sum over the entries of chunk c of -owner*imb and its torque, into
 // partials[6c..6c+5].  The summation order is fixed by a 256-lane virtual
 // layout (entry k goes to virtual lane k % 256, lanes summed by a wave
-// butterfly, the four virtual waves in order), so a workgroup of nt = 64, 128
-// or 256 threads (each thread running 256/nt virtual lanes) produces the same
-// bits.  lds needs 4*6 doubles.
+// butterfly, the four virtual waves in order): physical wave p of a
+// workgroup of nt threads (nt a multiple of 64, at most 256) runs virtual
+// waves p, p + nt/64, ..., so every such nt produces the same bits, and so
+// does every register batch KB.  lds needs 4*6 doubles.
 constexpr int kLoadsVL = 256;
 
-template <typename TI>
+template <typename TI, int KB = kLoadsKB>
 __device__ __forceinline__ void loads_chunk_body(const LoadsArgs& a, const TI* __restrict__ imb, int nx, int ny,
                                                  int nz, int c, int tid, int nt, double* __restrict__ partials,
                                                  double* lds) {
@@ -45,16 +46,15 @@ __device__ __forceinline__ void loads_chunk_body(const LoadsArgs& a, const TI* _
   const size_t N = (size_t)nx * ny * nz;
   const size_t out0 = (size_t)c * a.chunk_cells;
   constexpr uint32_t kMask = (1u << kLocBits) - 1u;
-  const int M = kLoadsVL / nt;  // virtual lanes per thread
   const int lane = tid & 63, pwave = tid >> 6, nwave = nt >> 6;
-  for (int m = 0; m < M; ++m) {
-    const int v = tid + m * nt;  // virtual lane
+  for (int vw = pwave; vw < kLoadsVL / 64; vw += nwave) {
+    const int v = vw * 64 + lane;  // virtual lane
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int k0 = 0; k0 < nnz; k0 += kLoadsKB * kLoadsVL) {
-      uint32_t cell[kLoadsKB], loc[kLoadsKB], cnt[kLoadsKB], tot[kLoadsKB];
-      TI ix_[kLoadsKB], iy_[kLoadsKB], iz_[kLoadsKB];
+    for (int k0 = 0; k0 < nnz; k0 += KB * kLoadsVL) {
+      uint32_t cell[KB], loc[KB], cnt[KB], tot[KB];
+      TI ix_[KB], iy_[KB], iz_[KB];
 #pragma unroll
-      for (int j = 0; j < kLoadsKB; ++j) {
+      for (int j = 0; j < KB; ++j) {
         const int k = k0 + j * kLoadsVL + v;
         const bool ok = k < nnz;
         cell[j] = ok ? a.ent_cell[out0 + k] : 0u;
@@ -62,7 +62,7 @@ __device__ __forceinline__ void loads_chunk_body(const LoadsArgs& a, const TI* _
         cnt[j] = ok ? (uint32_t)a.ent_cnt[out0 + k] : 0u;
       }
 #pragma unroll
-      for (int j = 0; j < kLoadsKB; ++j) {
+      for (int j = 0; j < KB; ++j) {
         const bool ok = cnt[j] != 0u;
         tot[j] = ok ? a.csum[cell[j]] : 1u;
         ix_[j] = ok ? imb[cell[j]] : (TI)0;
@@ -70,7 +70,7 @@ __device__ __forceinline__ void loads_chunk_body(const LoadsArgs& a, const TI* _
         iz_[j] = ok ? imb[2 * N + cell[j]] : (TI)0;
       }
 #pragma unroll
-      for (int j = 0; j < kLoadsKB; ++j) {
+      for (int j = 0; j < KB; ++j) {
         if (cnt[j] == 0u) continue;
         // owner = alpha_p / sum alpha = cnt / count_sum (exactly 1 for one owner)
         const double owner = (cnt[j] == tot[j]) ? 1.0 : (double)cnt[j] / (double)tot[j];
@@ -94,7 +94,6 @@ __device__ __forceinline__ void loads_chunk_body(const LoadsArgs& a, const TI* _
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
     if (lane == 0) {
-      const int vw = pwave + m * nwave;
 #pragma unroll
       for (int j = 0; j < 6; ++j) lds[vw * 6 + j] = acc[j];
     }

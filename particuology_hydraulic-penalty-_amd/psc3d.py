@@ -32,7 +32,9 @@ import numpy as np
 from . import _lib
 from .geometry import build_tables, pack_particles
 
-PIPELINE_MIN_CELLS = 1 << 19  # FP32 lattices from this size overlap the loads with the next step
+PIPELINE_MIN_CELLS = 1 << 19  # unfused FP32 lattices from this size overlap the loads with the next step
+FUSE_MAX_CELLS = 1 << 23      # FP32 lattices below this size fuse the loads into the step kernels
+FUSE_PERIOD = 5               # fused loads: one load workgroup in every 5 at the head of the grid
 
 # Lattice constants (psc3d.py:17-43): identical values and order.
 C = np.array(
@@ -179,7 +181,15 @@ class PscD3Q19:
         else:
             self._force = (0.0, 0.0, float(body_force_z))
         self.static_geometry = bool(static_geometry)
-        # run(): overlap the loads of step t with step t+1 (None = by lattice size and precision)
+        # run(): reduce the loads of step t inside the kernels of steps t+1 and t+2 (None =
+        # where the kernel supports it -- FP32 pull storage, stationary particles -- on
+        # lattices below FUSE_MAX_CELLS); fuse_period: the load workgroups lead those
+        # grids, one in every fuse_period (odd: even periods put them on half the XCDs)
+        self.fuse_loads = None
+        self.fuse_period = FUSE_PERIOD
+        self.loads_mode = None  # how the last native run() reduced the loads
+        # run() without fused loads: overlap the loads of step t with step t+1 on a side
+        # stream (None = by lattice size and precision)
         self.pipeline_loads = None
         self.periodic_geometry = bool(periodic_geometry)
         if precision not in ("fp32", "fp64"):
@@ -386,23 +396,30 @@ class PscD3Q19:
         if self._autotune:
             self._tune()
         import ctypes as ct
-        # Loads of step t overlap step t+1 on a side stream when the step is long
-        # enough to hide them (FP64, or FP32 from ~80^3 up); on small FP32 lattices the
-        # fork/join costs more than it hides and the loads run in line (measured:
-        # 40^3 16 vs 19 us/step, 96^3 34 vs 33, 128^3 73 vs 69).
+        # FP32 pull lattices with stationary particles below FUSE_MAX_CELLS reduce the
+        # loads inside the next steps' kernels (measured, us/step, fused vs side stream:
+        # 64^3 13.0 vs 23.3, 96^3 30.8 vs 35.9, 128^3 62.9 vs 70.7, 256^3 517 vs 510).
+        # Otherwise the loads of step t overlap step t+1 on a side stream when the step
+        # is long enough to hide them (FP64, or FP32 from ~80^3 up); on small lattices
+        # the fork/join costs more than it hides and the loads run in line.
         has_loads = self._tables is not None and self._tables.particles.shape[0] > 0
+        fusable = (has_loads and self._storage == _lib.PSC_F32_DEV and self._layout == _lib.PSC_PULL
+                   and not self._moving)
+        fused = fusable and (self._N < FUSE_MAX_CELLS if self.fuse_loads is None else bool(self.fuse_loads))
         auto = self._storage == _lib.PSC_F64_PLAIN or self._N >= PIPELINE_MIN_CELLS
-        pipelined = has_loads and (auto if self.pipeline_loads is None else bool(self.pipeline_loads))
-        if pipelined:
+        pipelined = (has_loads and not fused
+                     and (auto if self.pipeline_loads is None else bool(self.pipeline_loads)))
+        self.loads_mode = "fused" if fused else ("pipelined" if pipelined else "in line")
+        if pipelined or fused:
             self._ensure_pipeline_buffers()
-        plan = self._plan(pipelined)
+        plan = self._plan(pipelined, fused)
         parity = ct.c_int32(0)
         _lib.call("psc_run_steps", ct.byref(plan), steps, 1 if graph else 0, int(graph_steps), ct.byref(parity),
                   self._stream())
         if parity.value:
             self._a, self._b = self._b, self._a
         # the last step read parity 1 - p and wrote that imb buffer
-        self._imb = self._imb_bufs[1 - parity.value] if pipelined else self._imb_bufs[0]
+        self._imb = self._imb_bufs[1 - parity.value] if (pipelined or fused) else self._imb_bufs[0]
         self._step_count += steps
         self._cache.clear()
 
@@ -414,7 +431,7 @@ class PscD3Q19:
         if self._partials_alt is None or self._partials_alt.numel() < need:
             self._partials_alt = torch.empty(need, dtype=torch.float64, device=self.device)
 
-    def _plan(self, pipelined: bool = False):
+    def _plan(self, pipelined: bool = False, fused: bool = False):
         """PscStepPlan of the current buffers (state in self._a)."""
         p = _lib.PscStepPlan()
         nx, ny, nz = self._shape
@@ -422,7 +439,7 @@ class PscD3Q19:
         p.count_sum = self._ptr(self._csum)
         p.us_num = self._ptr(self._us) if self._moving else None
         p.imb = self._ptr(self._imb_bufs[0])
-        p.imb_alt = self._ptr(self._imb_bufs[1]) if pipelined else None
+        p.imb_alt = self._ptr(self._imb_bufs[1]) if (pipelined or fused) else None
         p.storage, p.layout = self._storage, self._layout
         p.nx, p.ny, p.nz, p.subsamples = nx, ny, nz, self._s
         p.tau = self.tau
@@ -439,8 +456,9 @@ class PscD3Q19:
             ec, el, en = self._ent
             p.ent_cell, p.ent_loc, p.ent_cnt = self._ptr(ec), self._ptr(el), self._ptr(en)
             p.chunk_nnz, p.partials = self._ptr(self._chunk_nnz), self._ptr(self._partials)
-            p.partials_alt = self._ptr(self._partials_alt) if pipelined else None
+            p.partials_alt = self._ptr(self._partials_alt) if (pipelined or fused) else None
             p.loads = self._ptr(self._loads)
+            p.fuse_loads = max(1, int(self.fuse_period)) if fused else 0
         return p
 
     def _tune(self):

@@ -136,20 +136,23 @@ def test_voxelizer_counts_random(psc3d):
         assert np.array_equal(got, ref_cnt)
 
 
-@pytest.mark.parametrize("precision,block,pipeline", [("fp64", (64, 4), None), ("fp32", (64, 4), True),
-                                                      ("fp32", (64, 2), True), ("fp32", (64, 1), True),
-                                                      ("fp32", (64, 4), None), ("fp64", (64, 4), False)])
-def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, block, pipeline):
+@pytest.mark.parametrize("precision,block,fuse,pipeline", [
+    ("fp64", (64, 4), None, None), ("fp64", (64, 4), None, False),
+    ("fp32", (64, 4), None, None), ("fp32", (64, 2), None, None), ("fp32", (64, 1), None, None),
+    ("fp32", (96, 2), None, None), ("fp32", (64, 4), False, True), ("fp32", (64, 2), False, True),
+    ("fp32", (64, 4), False, False)])
+def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, block, fuse, pipeline):
     """psc_run_steps (direct launches and hipGraph replays, with eager reads in
-    between the replays; load reduction in line or pipelined on the side
-    stream, with 64/128/256-thread workgroups) gives the same state and loads
-    as stepping one call at a time."""
+    between the replays; load reduction fused into the next step's kernel, in
+    line, or pipelined on the side stream, with 64/128/192/256-thread
+    workgroups) gives the same state and loads as stepping one call at a time."""
     g = G.load("perm24")
     a, _ = _solver(psc3d, g, precision, block=block)
     b, _ = _solver(psc3d, g, precision, block=block)
     c, _ = _solver(psc3d, g, precision, block=block)
     b.pipeline_loads = c.pipeline_loads = pipeline
-    for chunk in (16, 16, 5, 32):
+    b.fuse_loads = c.fuse_loads = fuse
+    for chunk in (16, 16, 5, 32, 1, 2):
         for _ in range(chunk):
             a.collide_stream_psc()
         b.run(chunk, graph=True, graph_steps=8)
@@ -162,7 +165,43 @@ def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, bl
         assert np.array_equal(a.hydro_forces, c.hydro_forces)
         assert np.array_equal(a.hydro_torques, c.hydro_torques)
         assert np.array_equal(np.stack([a.imb_mx, a.imb_my, a.imb_mz]), np.stack([b.imb_mx, b.imb_my, b.imb_mz]))
-    assert a._step_count == b._step_count == 69
+        assert np.array_equal(np.stack([a.imb_mx, a.imb_my, a.imb_mz]), np.stack([c.imb_mx, c.imb_my, c.imb_mz]))
+    assert a._step_count == b._step_count == 72
+
+
+@pytest.mark.parametrize("block,flags,period", [((128, 1), 0x10601, 1), ((192, 1), 0x10601, 5),
+                                                ((64, 4), 0x20001, 3), ((96, 2), 0x00301, 2),
+                                                ((128, 1), 0x10601, 1001)])
+def test_fused_loads_match_separate_reduction(psc3d, block, flags, period):
+    """A 64^3 lattice with 40 spheres (overlapping; 256-cell chunks, so
+    hundreds of chunks and several per particle): the step kernels that also
+    reduce earlier steps' loads (with the tuner's flag forms: non-temporal,
+    occupancy cap, two cells per thread; load workgroups all ahead of the
+    cells, interleaved, or spread over more than the grid) give the state, imb
+    and loads of separate launches, bit for bit."""
+    rng = np.random.default_rng(7)
+    parts = [psc3d.Particle3D(*rng.uniform(0, 64, 3), rng.uniform(3.0, 7.5)) for _ in range(40)]
+
+    def make():
+        s = psc3d.SparsePscD3Q19(64, [psc3d.Particle3D(p.x, p.y, p.z, p.r) for p in parts], tau=0.8, subsamples=3,
+                                 body_force=(1e-5, 0.0, 2e-5), static_geometry=True, precision="fp32",
+                                 device="cuda:0", block=block, chunk_cells=256)
+        s._flags = flags
+        return s
+
+    a, b = make(), make()
+    b.fuse_loads, b.fuse_period = True, period
+    print(f"[fused] block {block} flags {flags:#x} period {period}", flush=True)
+    for chunk in (1, 3, 24, 9):
+        for _ in range(chunk):
+            a.collide_stream_psc()
+        b.run(chunk, graph=True, graph_steps=8)  # the first call builds the geometry, stepping in Python
+        assert np.array_equal(a.f, b.f)
+        assert np.array_equal(a.hydro_forces, b.hydro_forces)
+        assert np.array_equal(a.hydro_torques, b.hydro_torques)
+        assert np.array_equal(np.stack([a.imb_mx, a.imb_my, a.imb_mz]), np.stack([b.imb_mx, b.imb_my, b.imb_mz]))
+    assert np.abs(a.hydro_forces).max() > 0
+    assert int(a._tables.chunks.shape[0]) > 100 and int(np.diff(a._tables.chunk_first).max()) > 4
 
 
 def test_every_launch_configuration_gives_identical_results(psc3d):

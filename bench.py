@@ -284,14 +284,20 @@ def bench_windows(args, workloads, torch, np):
     B = batch.BatchedPscD3Q19([(L, t) for L, t, _ in wins], tau=0.8, subsamples=3, body_force=(0.0, 0.0, 1.0e-6),
                               device="cuda:0")
     B.run(warm)
+    stream = torch.cuda.current_stream()
+    g0 = torch.cuda.Event(enable_timing=True)
+    g1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    g0.record(stream)
     B.run(steps)
+    g1.record(stream)
     k = B.permeability()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    gpu_ms = g0.elapsed_time(g1)
+    fused = bool(B.fuse_loads and B.P > 0)
     # batched step kernel alone, HIP events on its stream
-    stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     reps = 20
@@ -302,7 +308,20 @@ def bench_windows(args, workloads, torch, np):
     torch.cuda.synchronize()
     kern = e0.elapsed_time(e1) / reps
     cells = int(B.nlive.sum())  # live cells only: row and plane padding is neither read nor written
-    achieved = BYTES_F32_STATIC * cells / (kern * 1e-3) / 1e9
+    nnz = int(B._nnz[:B.C].sum().item())
+    lbytes = 26 * nnz + 96 * B.C + 48 * B.P
+    if fused:
+        # every timed step is one k_step_f32_batched<..., loads> launch: the HIP-event time of
+        # the timed region over the step count is its average duration (plus the pending
+        # loads once at the end of the run)
+        kern_l = gpu_ms / steps
+        launch_bytes = BYTES_F32_STATIC * cells + lbytes
+        achieved = launch_bytes / (kern_l * 1e-3) / 1e9
+        kname, kus = "k_step_f32_batched (with the loads of the two previous steps)", kern_l * 1e3
+    else:
+        launch_bytes = BYTES_F32_STATIC * cells
+        achieved = launch_bytes / (kern * 1e-3) / 1e9
+        kname, kus = "k_step_f32_batched", kern * 1e3
     por = B.porosity()
     variants = [v for _, _, v in wins]
     out = {
@@ -312,13 +331,21 @@ def bench_windows(args, workloads, torch, np):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "64 periodic pore windows, L~U{48..160} (seed 0), pebbles r=16 near solid fraction "
                                "0.6 + 0-400 throat fines (skeleton/passive/preloaded), tau 0.8, F=1e-6 z, s=3, "
-                               "FP32; one persistent launch + loads per step, hipGraph replay",
+                               "FP32; one persistent launch per step (loads "
+                               f"{'fused into the launches' if fused else 'on a side stream'}), hipGraph replay",
                    "windows": len(wins), "total_cells": cells, "particles": int(B.P)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_step_f32_batched",
-                     "bytes_per_cell": BYTES_F32_STATIC, "kernel_us": kern * 1e3,
+                     "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": read_traffic("k_step_f32_batched_loads_windows_pmc.txt" if fused
+                                             else "k_step_f32_batched_windows_pmc.txt"),
+                     "kernel": kname,
+                     "bytes_per_cell": BYTES_F32_STATIC, "bytes_per_launch": launch_bytes,
+                     "loads_bytes_per_step": lbytes, "loads_entries": nnz, "kernel_us": kus,
+                     "step_kernel_alone_us": kern * 1e3,
+                     "step_kernel_alone_GBps": BYTES_F32_STATIC * cells / (kern * 1e-3) / 1e9,
                      "launch": {"flags": B._flags, "grid": B._grid, "items": B._mode,
-                                "workgroup": 512 if B._flags & 0x40000 else 256}},
+                                "workgroup": 512 if B._flags & 0x40000 else 256,
+                                "fuse_period": B.fuse_period if fused else None}},
         "permeability": {v: [float(x) for x, vv in zip(k, variants) if vv == v][:4] for v in ("skeleton", "passive",
                                                                                             "preloaded")},
         "porosity_range": [float(por.min()), float(por.max())],

@@ -174,6 +174,40 @@ int psc_step_batched(const float* d_in, float* d_out, const uint32_t* d_count_su
                      int32_t nitems, int32_t subsamples, double tau, int32_t flags,
                      int32_t grid_blocks, void* stream);
 
+/* Tables of the particle-load reduction (the arguments of
+ * psc_particle_loads that do not change from step to step). */
+typedef struct PscLoadsTables {
+  const double* particles;
+  const int32_t* boxes;
+  const int32_t* chunks;
+  const int32_t* chunk_first;
+  int32_t nparticles, nchunks, chunk_cells;
+  const uint32_t* ent_cell;
+  const uint32_t* ent_loc;
+  const uint16_t* ent_cnt;
+  const int32_t* chunk_nnz;
+  const uint32_t* count_sum;
+} PscLoadsTables;
+
+/* psc_step_batched with the particle-load reduction of earlier steps in the
+ * same launch (the batched form of psc_run_steps' fused loads; it replaces
+ * the pair psc_step_batched + psc_particle_loads of the drivers' per-step
+ * loop, psc3d.py:208-281, for many windows).  The tables describe every
+ * window's particles over the flattened cells of all windows (nx = ny = 1,
+ * nz = imb_stride in psc_particle_loads' terms); FP32 imb.  With d_imb_prev
+ * set the launch also sums its chunks (the previous step's imb) into
+ * d_partials_out; with d_partials_prev set it also adds those partials up
+ * per particle into d_loads (the loads of the step before the previous
+ * one).  The extra work items lead the item sequence, one in every `period`
+ * (>= 1).  d_imb must be set, and differ from d_imb_prev, d_partials_out from
+ * d_partials_prev.  The loads have the bits of psc_particle_loads. */
+int psc_step_batched_loads(const float* d_in, float* d_out, const uint32_t* d_count_sum, float* d_imb,
+                           int64_t imb_stride, const PscWindow* d_windows, const int32_t* d_items,
+                           int32_t nitems, int32_t subsamples, double tau, int32_t flags,
+                           int32_t grid_blocks, const PscLoadsTables* tables, const float* d_imb_prev,
+                           double* d_partials_out, const double* d_partials_prev, double* d_loads,
+                           int32_t period, void* stream);
+
 /* K3 -- node-summed hydrodynamic force and torque per particle.
  * Replaces accumulate_particle_loads (psc3d.py:255-281 dense, :375-407
  * sparse): owner = alpha_p / sum alpha = cnt / count_sum, node force

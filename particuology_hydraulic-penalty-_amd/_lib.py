@@ -36,6 +36,8 @@ SIGNATURES = {
     "psc_voxelize": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, ct.c_uint32,
                       _P], ct.c_int),
     "psc_step_batched": ([_P, _P, _P, _P, _I64, _P, _P, _I32, _I32, _D, _I32, _I32, _P], ct.c_int),
+    "psc_step_batched_loads": ([_P, _P, _P, _P, _I64, _P, _P, _I32, _I32, _D, _I32, _I32, _P, _P, _P, _P, _P,
+                                _I32, _P], ct.c_int),
     "psc_clear_entries": ([_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _I32, _P], ct.c_int),
     "psc_step": ([_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _I32, _P],
                  ct.c_int),
@@ -72,6 +74,14 @@ class PscStepPlan(ct.Structure):
         ("ent_cell", _P), ("ent_loc", _P), ("ent_cnt", _P), ("chunk_nnz", _P), ("partials", _P),
         ("partials_alt", _P), ("loads", _P), ("fuse_loads", _I32),
     ]
+
+
+class PscLoadsTables(ct.Structure):
+    """Mirror of struct PscLoadsTables (include/psc_hip.h)."""
+
+    _fields_ = [("particles", _P), ("boxes", _P), ("chunks", _P), ("chunk_first", _P),
+                ("nparticles", _I32), ("nchunks", _I32), ("chunk_cells", _I32),
+                ("ent_cell", _P), ("ent_loc", _P), ("ent_cnt", _P), ("chunk_nnz", _P), ("count_sum", _P)]
 
 
 class PscWindow(ct.Structure):

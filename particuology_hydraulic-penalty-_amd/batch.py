@@ -11,8 +11,10 @@ windows of uneven sizes shares one set of device buffers:
 
 Each step is ONE launch of ``psc_step_batched`` (work items are spans of
 cells or groups of whole rows of a window), followed by the particle-load
-reduction; ``run`` captures two steps into a hipGraph (torch.cuda.CUDAGraph
-around the C-ABI launches) and replays it.  The launch configuration
+reduction.  ``run`` captures a few steps into a hipGraph (torch.cuda.CUDAGraph
+around the C-ABI launches) and replays it; there each launch
+(``psc_step_batched_loads``) also carries the load reduction of the two
+previous steps, so a step stays one launch.  The launch configuration
 (workgroup size, item mode, non-temporal access, occupancy cap) is autotuned
 on first use.
 Every window is the reference's ``collide_stream_psc`` (psc3d.py:208-253) with
@@ -30,6 +32,7 @@ from . import _lib
 from .geometry import build_tables, pack_particles
 
 MAX_WINDOW_CELLS = 1 << 22
+FUSE_PERIOD = 5  # fused loads: one load item in every 5 at the head of a launch's item sequence
 ROW_ALIGN = 4  # default z-row padding: a multiple of 4 cells (16-byte aligned rows)
 PLANE_ALIGN = 64  # direction planes padded to a multiple of 64 cells (256-byte aligned planes)
 
@@ -113,6 +116,11 @@ class BatchedPscD3Q19:
                 self._mode = str(launch[2])
         self._graph = None
         self._imb_alt = None
+        self._partials_alt = None
+        # run(): reduce each step's loads inside the next two step launches (False: on a
+        # side stream, concurrently with the next step)
+        self.fuse_loads = True
+        self.fuse_period = FUSE_PERIOD
         self._steps = 0
         self._voxelize(tabs)
 
@@ -191,6 +199,30 @@ class BatchedPscD3Q19:
                   en.data_ptr(), self._nnz.data_ptr(), self._csum.data_ptr(), imb.data_ptr(), 1, 1, 1,
                   self.Ntot, self._partials.data_ptr(), self._loads.data_ptr(), stream)
 
+    def _loads_tables(self):
+        t = _lib.PscLoadsTables()
+        ec, el, en = self._ent
+        t.particles, t.boxes, t.chunks = self._parts.data_ptr(), self._boxes.data_ptr(), self._chunks.data_ptr()
+        t.chunk_first = self._cfirst.data_ptr()
+        t.nparticles, t.nchunks, t.chunk_cells = self.P, self.C, self.chunk_cells
+        t.ent_cell, t.ent_loc, t.ent_cnt = ec.data_ptr(), el.data_ptr(), en.data_ptr()
+        t.chunk_nnz, t.count_sum = self._nnz.data_ptr(), self._csum.data_ptr()
+        return t
+
+    def _launch_fused(self, a, b, q, pc, pf, stream):
+        """One step from a into b writing imb[q], plus the chunk sums of
+        imb[1-q] (pc) and the per-particle sums of partials[q] (pf)."""
+        bs = 512 if (self._flags & _lib.PSC_BATCH_WG512) else 256
+        windows, items, nitems = self._items_for(bs, self._mode)
+        imbs = (self._imb, self._imb_alt)
+        prt = (self._partials, self._partials_alt)
+        t = self._loads_tables()
+        _lib.call("psc_step_batched_loads", a.data_ptr(), b.data_ptr(), self._csum.data_ptr(), imbs[q].data_ptr(),
+                  self.Ntot, windows.data_ptr(), items.data_ptr(), nitems, self._s, self.tau, self._flags,
+                  self._grid, ct.byref(t), imbs[1 - q].data_ptr() if pc else None, prt[1 - q].data_ptr(),
+                  prt[q].data_ptr() if pf else None, self._loads.data_ptr(), max(1, int(self.fuse_period)),
+                  stream)
+
     def step(self):
         """One step of every window (collide_stream_psc) plus the loads."""
         self._tune()
@@ -232,17 +264,24 @@ class BatchedPscD3Q19:
 
     def run(self, steps: int, graph: bool = True, graph_steps: int = 8) -> None:
         """``steps`` steps.  With graph=True, ``graph_steps`` (even) steps are
-        captured once into a hipGraph and replayed.  Inside the graph the load
-        reduction of step t runs on a side stream while step t+1 runs on the
-        main one: the steps alternate between two imb buffers, step t+2 waits
-        for the loads of step t (which read the buffer it overwrites), and the
-        graph ends joined.  Results are those of stepping one by one."""
+        captured once into a hipGraph and replayed.  The steps alternate
+        between two imb buffers.  With fuse_loads (the default), the launch of
+        step t also sums the chunks of step t-1's imb and adds up, per
+        particle, the chunk partials of step t-2 (two partial-sum buffers
+        alternate too); the loads still pending after the last step are
+        reduced at the end.  Otherwise the load reduction of step t runs on a
+        side stream while step t+1 runs on the main one, step t+2 waits for
+        it (it overwrites the imb those loads read), and the graph ends
+        joined.  Results are those of stepping one by one."""
         import torch
         steps = int(steps)
         self._tune()
         # the item tables are uploaded here, never inside the graph capture
         self._items_for(512 if (self._flags & _lib.PSC_BATCH_WG512) else 256, self._mode)
         G = max(2, int(graph_steps) // 2 * 2)
+        if self.fuse_loads and self.P > 0 and steps > 0:
+            self._run_fused(steps, graph, G)
+            return
         if not graph or steps < G:
             for _ in range(steps):
                 self.step()
@@ -279,6 +318,52 @@ class BatchedPscD3Q19:
         self._steps += G * (steps // G)
         for _ in range(steps % G):
             self.step()
+
+    def _run_fused(self, steps, graph, G):
+        import torch
+        if self._imb_alt is None:
+            self._imb_alt = torch.zeros_like(self._imb)
+        if self._partials_alt is None:
+            self._partials_alt = torch.zeros_like(self._partials)
+        s = self._stream()
+        if not hasattr(self, "_buf0"):
+            self._buf0 = self._a.data_ptr()
+        q = 0 if self._a.data_ptr() == self._buf0 else 1
+        pc = pf = False
+        done = 0
+
+        def direct(n):
+            nonlocal q, pc, pf
+            for _ in range(n):
+                self._launch_fused(self._a, self._b, q, pc, pf, s)
+                self._a, self._b = self._b, self._a
+                pf, pc, q = pc, True, 1 - q
+
+        head = min(steps, 2)
+        direct(head)
+        done += head
+        if graph and steps - done >= G:
+            key = ("fused", q, G)
+            if getattr(self, "_fgraph", None) is None or self._fgraph[0] != key:
+                a, b = self._a, self._b
+                torch.cuda.synchronize(self.device)
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    cs = torch.cuda.current_stream(self.device).cuda_stream
+                    for j in range(G):
+                        src, dst = (a, b) if j % 2 == 0 else (b, a)
+                        self._launch_fused(src, dst, (q + j) % 2, True, True, cs)
+                self._fgraph = (key, g, a, b)
+            g = self._fgraph[1]
+            while steps - done >= G:
+                g.replay()
+                done += G
+        direct(steps - done)
+        # the loads still pending: step t-1 (its imb is intact) and step t
+        if pf:
+            self._launch_loads(s, imb=(self._imb, self._imb_alt)[q])
+        self._launch_loads(s, imb=(self._imb, self._imb_alt)[1 - q])
+        self._steps += steps
 
     # -------------------------------------------------------------- readouts
     def readout_sums(self):

@@ -44,7 +44,8 @@ int launch_step_f32(const float*, float*, const uint32_t*, const float*, float*,
 int launch_voxelize(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, int, uint32_t*,
                     uint32_t*, uint16_t*, int32_t*, uint32_t*, void*, int, uint32_t, hipStream_t);
 int launch_step_f32_batched(const float*, float*, const uint32_t*, float*, size_t, const void*, const void*, int, int,
-                            double, int, int, hipStream_t);
+                            double, int, int, hipStream_t, const LoadsArgs*, const float*, double*, const double*,
+                            double*, int);
 int launch_loads(const double*, const int32_t*, const int32_t*, const int32_t*, int, int, int, const uint32_t*,
                  const uint32_t*, const uint16_t*, const int32_t*, const uint32_t*, const void*, int, int, int, int,
                  double*, double*, hipStream_t);
@@ -262,7 +263,40 @@ int psc_step_batched(const float* d_in, float* d_out, const uint32_t* d_count_su
   if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK | PSC_BATCH_WG512)) != 0)
     return set_error(PSC_EINVAL, "psc_step_batched: unknown flags 0x%x", flags);
   return launch_step_f32_batched(d_in, d_out, d_count_sum, d_imb, (size_t)imb_stride, d_windows, d_items, nitems,
-                                 subsamples, tau, flags, grid_blocks, (hipStream_t)stream);
+                                 subsamples, tau, flags, grid_blocks, (hipStream_t)stream, nullptr, nullptr, nullptr,
+                                 nullptr, nullptr, 1);
+}
+
+int psc_step_batched_loads(const float* d_in, float* d_out, const uint32_t* d_count_sum, float* d_imb,
+                           int64_t imb_stride, const PscWindow* d_windows, const int32_t* d_items, int32_t nitems,
+                           int32_t subsamples, double tau, int32_t flags, int32_t grid_blocks,
+                           const PscLoadsTables* t, const float* d_imb_prev, double* d_partials_out,
+                           const double* d_partials_prev, double* d_loads, int32_t period, void* stream) {
+  if (d_in == nullptr || d_out == nullptr || d_count_sum == nullptr || d_windows == nullptr || d_imb == nullptr ||
+      t == nullptr || (nitems > 0 && d_items == nullptr))
+    return set_error(PSC_EINVAL, "psc_step_batched_loads: null pointer");
+  if (d_in == d_out) return set_error(PSC_EINVAL, "psc_step_batched_loads: input and output alias");
+  if (nitems < 0 || grid_blocks < 0 || imb_stride <= 0 || imb_stride > 0x7fffffffll || period < 1)
+    return set_error(PSC_EINVAL, "psc_step_batched_loads: sizes");
+  if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
+    return set_error(PSC_EINVAL, "psc_step_batched_loads: subsamples %d", subsamples);
+  if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_step_batched_loads: tau %g must exceed 1/2", tau);
+  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK | PSC_BATCH_WG512)) != 0)
+    return set_error(PSC_EINVAL, "psc_step_batched_loads: unknown flags 0x%x", flags);
+  if (t->nparticles < 0 || t->nchunks < 0 || t->chunk_cells < 1)
+    return set_error(PSC_EINVAL, "psc_step_batched_loads: table sizes");
+  const bool tables_ok = t->particles != nullptr && t->boxes != nullptr && t->chunks != nullptr &&
+                         t->ent_cell != nullptr && t->ent_loc != nullptr && t->ent_cnt != nullptr &&
+                         t->chunk_nnz != nullptr && t->count_sum != nullptr;
+  if (d_imb_prev != nullptr && (d_imb_prev == d_imb || (t->nchunks > 0 && !tables_ok)))
+    return set_error(PSC_EINVAL, "psc_step_batched_loads: chunk tables / imb buffers");
+  if (d_partials_prev != nullptr && (t->chunk_first == nullptr || d_partials_prev == d_partials_out))
+    return set_error(PSC_EINVAL, "psc_step_batched_loads: partial-sum buffers");
+  const LoadsArgs la{t->particles, t->boxes, t->chunks, t->chunk_first, t->nparticles, t->nchunks, t->chunk_cells,
+                     t->ent_cell, t->ent_loc, t->ent_cnt, t->chunk_nnz, t->count_sum};
+  return launch_step_f32_batched(d_in, d_out, d_count_sum, d_imb, (size_t)imb_stride, d_windows, d_items, nitems,
+                                 subsamples, tau, flags, grid_blocks, (hipStream_t)stream, &la, d_imb_prev,
+                                 d_partials_out, d_partials_prev, d_loads, period);
 }
 
 int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,

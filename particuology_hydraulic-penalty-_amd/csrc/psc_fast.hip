@@ -394,15 +394,38 @@ __device__ __forceinline__ void divmod_f(uint32_t n, uint32_t d, float inv, uint
   r = (uint32_t)rr;
 }
 
-template <bool IMB, bool NT, int BS>
+// With LOADS, the item sequence also holds the load work of earlier steps
+// (fl: as for k_step_f32_loads, over the flattened cell index of all
+// windows): nchunk + nfinal extra items lead it, one in every fl.period.
+template <bool IMB, bool NT, int BS, bool LOADS>
 __global__ __launch_bounds__(BS) void k_step_f32_batched(const float* __restrict__ gin, float* __restrict__ gout,
                                                          const uint32_t* __restrict__ csum,
                                                          float* __restrict__ imb, size_t imb_stride,
                                                          const BatchWindow* __restrict__ wins,
                                                          const int2* __restrict__ items, int nitems,
-                                                         StepConsts k0) {
+                                                         StepConsts k0, FusedLoads fl, int total) {
   const int tid = threadIdx.x;
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+  const int nextra = LOADS ? fl.nchunk + fl.nfinal : 0;
+  for (int it0 = blockIdx.x; it0 < (LOADS ? total : nitems); it0 += gridDim.x) {
+    int it = it0;
+    if (LOADS && nextra > 0) {
+      if (it0 < nextra * fl.period) {
+        const int e = it0 / fl.period, r = it0 - e * fl.period;
+        if (r == fl.period - 1) {
+          if (e < fl.nchunk) {
+            __shared__ double lds[6 * 4];
+            loads_chunk_body<float, 4>(fl.a, fl.imb_prev, 1, 1, (int)imb_stride, e, tid, BS, fl.part_out, lds);
+          } else {
+            loads_final_body(fl.a.chunk_first, e - fl.nchunk, tid, fl.part_prev, fl.loads);
+          }
+          continue;
+        }
+        it = e * (fl.period - 1) + r;
+      } else {
+        it = it0 - nextra;
+      }
+      if (it >= nitems) continue;
+    }
     const int2 item = items[it];  // (window, first cell)
     const BatchWindow w = wins[item.x];
     const int span = w.span > 0 ? w.span : BS;
@@ -573,11 +596,43 @@ int launch_bb_step_f32(const float* in, float* out, const uint8_t* solid, int nx
 }
 
 // Batched launch: win/items are device tables (see BatchWindow); grid_blocks
-// = 0 sizes the persistent grid to the item count.
+// = 0 sizes the persistent grid to the item count.  With `la` set, the launch
+// also carries the load work of earlier steps (see k_step_f32_batched):
+// chunk sums of imb_prev into part_out when imb_prev is set, per-particle
+// sums of part_prev into loads when part_prev is set, the extra items leading
+// one in every `period`.
 int launch_step_f32_batched(const float* in, float* out, const uint32_t* csum, float* imb, size_t imb_stride,
                             const void* wins, const void* items, int nitems, int s, double tau, int flags,
-                            int grid_blocks, hipStream_t stream) {
-  if (nitems <= 0) return PSC_OK;
+                            int grid_blocks, hipStream_t stream, const LoadsArgs* la = nullptr,
+                            const float* imb_prev = nullptr, double* part_out = nullptr,
+                            const double* part_prev = nullptr, double* loads = nullptr, int period = 1) {
+  FusedLoads fl{};
+  fl.period = period;
+  long long total = nitems;
+  if (la != nullptr) {
+    if (period < 1) return set_error(PSC_EINVAL, "psc_step_batched_loads: period %d", period);
+    if (imb == nullptr) return set_error(PSC_EINVAL, "psc_step_batched_loads: the step must write imb");
+    fl.a = *la;
+    fl.imb_prev = imb_prev;
+    fl.part_out = part_out;
+    fl.part_prev = part_prev;
+    fl.loads = loads;
+    if (imb_prev != nullptr) {
+      if (part_out == nullptr) return set_error(PSC_EINVAL, "psc_step_batched_loads: null partials");
+      fl.nchunk = la->nchunks;
+    }
+    if (part_prev != nullptr) {
+      if (loads == nullptr) return set_error(PSC_EINVAL, "psc_step_batched_loads: null loads");
+      fl.nfinal = la->np;
+    }
+    const long long nextra = (long long)fl.nchunk + fl.nfinal;
+    if (nextra > 0) {
+      total = nitems + nextra;
+      if (total < nextra * period) total = nextra * period;
+    }
+    if (total > 0x7fffffffll) return set_error(PSC_EINVAL, "psc_step_batched_loads: too many items");
+  }
+  if (total <= 0) return PSC_OK;
   const int bpc = (flags >> 8) & 0xff;
   size_t lds = 0;
   if (bpc > 0) {
@@ -592,23 +647,29 @@ int launch_step_f32_batched(const float* in, float* out, const uint32_t* csum, f
   k.omt = (float)(1.0 - 1.0 / tau);
   k.pref = (float)(1.0 - 0.5 / tau);
   k.Fx = k.Fy = k.Fz = 0.0f;
-  const int grid = (grid_blocks > 0 && grid_blocks < nitems) ? grid_blocks : nitems;
+  const int tot = (int)total;
+  const int grid = (grid_blocks > 0 && grid_blocks < tot) ? grid_blocks : tot;
   const BatchWindow* w = (const BatchWindow*)wins;
   const int2* it = (const int2*)items;
   const bool nt = (flags & 1) != 0;
   const bool wide = (flags & PSC_BATCH_WG512) != 0;
-#define PSC_BATCH_CASE(I, T)                                                                                  \
-  if ((imb != nullptr) == I && nt == T) {                                                                    \
-    if (wide)                                                                                                \
-      k_step_f32_batched<I, T, 512><<<grid, 512, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k); \
-    else                                                                                                     \
-      k_step_f32_batched<I, T, 256><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, k); \
-    return check_launch("psc_step_batched");                                                                 \
+  const bool ld = la != nullptr;
+#define PSC_BATCH_CASE(I, T, L)                                                                                \
+  if ((imb != nullptr) == I && nt == T && ld == L) {                                                          \
+    if (wide)                                                                                                 \
+      k_step_f32_batched<I, T, 512, L><<<grid, 512, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, \
+                                                                   k, fl, tot);                               \
+    else                                                                                                      \
+      k_step_f32_batched<I, T, 256, L><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, \
+                                                                   k, fl, tot);                               \
+    return check_launch("psc_step_batched");                                                                  \
   }
-  PSC_BATCH_CASE(true, true)
-  PSC_BATCH_CASE(true, false)
-  PSC_BATCH_CASE(false, true)
-  PSC_BATCH_CASE(false, false)
+  PSC_BATCH_CASE(true, true, false)
+  PSC_BATCH_CASE(true, false, false)
+  PSC_BATCH_CASE(false, true, false)
+  PSC_BATCH_CASE(false, false, false)
+  PSC_BATCH_CASE(true, true, true)
+  PSC_BATCH_CASE(true, false, true)
 #undef PSC_BATCH_CASE
   return set_error(PSC_EINVAL, "psc_step_batched: no kernel variant");
 }

@@ -302,6 +302,40 @@ def test_moving_fp32_loads_track_fp64(psc3d):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("launch,period", [("auto", 5), ((1 | 0x40000, 0, "rows"), 1), ((0x40000, 64, "spans"), 3),
+                                           ((1, 0, "spans"), 2), ((0x200, 0, "rows"), 10001)])
+def test_batched_fused_loads_match_separate_reduction(launch, period):
+    """Batched windows with the loads reduced inside the step launches
+    (psc_step_batched_loads: 256/512-thread workgroups, persistent grid,
+    placement periods) give the state and loads of the side-stream form, bit
+    for bit, over runs of 1, 2, 9, 17 and 3 steps (graph replays, direct
+    launches and the pending loads at the end of each run)."""
+    batch = load_package("batch")
+    print(f"[batched fused] launch {launch} period {period}", flush=True)
+    rng = np.random.default_rng(5)
+    wins = []
+    for L in (24, 40, 31):
+        t = np.zeros((7, 10))
+        t[:, 0:3] = rng.uniform(0, L, (7, 3))
+        t[:, 3] = rng.uniform(2.5, 6.0, 7)
+        wins.append((L, t))
+    kw = dict(tau=0.8, subsamples=3, body_force=(1e-6, 0, 2e-6), device="cuda:0", launch=launch, chunk_cells=256)
+    A = batch.BatchedPscD3Q19(wins, **kw)
+    B = batch.BatchedPscD3Q19(wins, **kw)
+    A.fuse_loads = False
+    B.fuse_period = period
+    for chunk in (1, 2, 9, 17, 3):
+        A.run(chunk)
+        B.run(chunk)
+        for w in range(len(wins)):
+            assert np.array_equal(A.window_state(w), B.window_state(w)), (chunk, w)
+        la, lb = A.loads(), B.loads()
+        for w in range(len(wins)):
+            assert np.array_equal(la[w], lb[w]), (chunk, w)
+    assert B.C > 3 * B.P  # several chunks per particle
+    assert max(np.abs(v).max() for v in B.loads()) > 0
+
+
 @pytest.mark.parametrize("launch", ["auto", (1, 0, "spans"), (1 | 0x40000, 0, "rows"), (0x200, 0, "rows"),
                                     (0x40000, 64, "spans")])
 def test_batched_windows_match_single_solvers(psc3d, launch):

@@ -56,6 +56,7 @@ extern "C" {
 #define PSC_STEP_BPC_MASK 0xff00    /* bits 8-15: cap on resident workgroups per CU */
 #define PSC_STEP_CPT_MASK 0x30000   /* bits 16-17: cells per thread along y (1, 2)  */
 #define PSC_BATCH_WG512 0x40000     /* psc_step_batched: 512-thread workgroups      */
+#define PSC_BATCH_IMB_NT 0x80000    /* psc_step_batched: non-temporal imb stores    */
 
 #define PSC_NSUM 13 /* number of FP64 sums produced by psc_moments           */
 
@@ -167,8 +168,8 @@ typedef struct PscWindow {
  * item).
  * Each window's step is collide_stream_psc (psc3d.py:208-246), the same
  * arithmetic as psc_step with PSC_F32_DEV.  N_w must stay below 2^22 cells.
- * flags: PSC_STEP_NONTEMPORAL, the workgroups-per-CU cap of psc_step and
- * PSC_BATCH_WG512. */
+ * flags: PSC_STEP_NONTEMPORAL, the workgroups-per-CU cap of psc_step,
+ * PSC_BATCH_WG512 and PSC_BATCH_IMB_NT. */
 int psc_step_batched(const float* d_in, float* d_out, const uint32_t* d_count_sum, float* d_imb,
                      int64_t imb_stride, const PscWindow* d_windows, const int32_t* d_items,
                      int32_t nitems, int32_t subsamples, double tau, int32_t flags,

@@ -209,16 +209,24 @@ class BatchedPscD3Q19:
         t.chunk_nnz, t.count_sum = self._nnz.data_ptr(), self._csum.data_ptr()
         return t
 
-    def _launch_fused(self, a, b, q, pc, pf, stream):
+    def _ensure_fused_buffers(self):
+        import torch
+        if self._imb_alt is None:
+            self._imb_alt = torch.zeros_like(self._imb)
+        if self._partials_alt is None:
+            self._partials_alt = torch.zeros_like(self._partials)
+
+    def _launch_fused(self, a, b, q, pc, pf, stream, flags=None, mode=None):
         """One step from a into b writing imb[q], plus the chunk sums of
         imb[1-q] (pc) and the per-particle sums of partials[q] (pf)."""
-        bs = 512 if (self._flags & _lib.PSC_BATCH_WG512) else 256
-        windows, items, nitems = self._items_for(bs, self._mode)
+        flags = self._flags if flags is None else flags
+        bs = 512 if (flags & _lib.PSC_BATCH_WG512) else 256
+        windows, items, nitems = self._items_for(bs, mode or self._mode)
         imbs = (self._imb, self._imb_alt)
         prt = (self._partials, self._partials_alt)
         t = self._loads_tables()
         _lib.call("psc_step_batched_loads", a.data_ptr(), b.data_ptr(), self._csum.data_ptr(), imbs[q].data_ptr(),
-                  self.Ntot, windows.data_ptr(), items.data_ptr(), nitems, self._s, self.tau, self._flags,
+                  self.Ntot, windows.data_ptr(), items.data_ptr(), nitems, self._s, self.tau, flags,
                   self._grid, ct.byref(t), imbs[1 - q].data_ptr() if pc else None, prt[1 - q].data_ptr(),
                   prt[q].data_ptr() if pf else None, self._loads.data_ptr(), max(1, int(self.fuse_period)),
                   stream)
@@ -234,31 +242,44 @@ class BatchedPscD3Q19:
 
     def _tune(self):
         """Pick the workgroup size (256 or 512), item mode (cell spans or
-        whole rows), non-temporal access and occupancy cap, timing each
-        candidate with HIP events on the solver's own buffers (the state is
-        not modified: A -> B)."""
+        whole rows), non-temporal access (state, and separately imb) and
+        occupancy cap, timing each candidate with HIP events on the solver's
+        own buffers (the state is not modified: A -> B).  The launch timed is
+        the one run() uses: with fused loads, the step that also carries the
+        load reduction of two earlier steps."""
         if self._tuned:
             return
         import torch
         from . import autotune
         s = self._stream()
+        fused = self.fuse_loads and self.P > 0
+        if fused:
+            self._ensure_fused_buffers()
         best = None
         for bs in (256, 512):
             for mode in ("rows", "spans"):
                 for nt in (0, 1):
                     for bpc in (0, 2, 4, 8):
-                        flags = autotune.flags_of(nt, bpc, 0) | (_lib.PSC_BATCH_WG512 if bs == 512 else 0)
-                        self._launch_step(self._a, self._b, flags, 0, s, mode)
-                        e0 = torch.cuda.Event(enable_timing=True)
-                        e1 = torch.cuda.Event(enable_timing=True)
-                        e0.record()
-                        for _ in range(3):
-                            self._launch_step(self._a, self._b, flags, 0, s, mode)
-                        e1.record()
-                        e1.synchronize()
-                        ms = e0.elapsed_time(e1) / 3
-                        if best is None or ms < best[0]:
-                            best = (ms, flags, mode)
+                        for inb in (0, 1):
+                            flags = (autotune.flags_of(nt, bpc, 0) | (_lib.PSC_BATCH_WG512 if bs == 512 else 0)
+                                     | (_lib.PSC_BATCH_IMB_NT if inb else 0))
+
+                            def launch():
+                                if fused:
+                                    self._launch_fused(self._a, self._b, 0, True, True, s, flags, mode)
+                                else:
+                                    self._launch_step(self._a, self._b, flags, 0, s, mode)
+                            launch()
+                            e0 = torch.cuda.Event(enable_timing=True)
+                            e1 = torch.cuda.Event(enable_timing=True)
+                            e0.record()
+                            for _ in range(3):
+                                launch()
+                            e1.record()
+                            e1.synchronize()
+                            ms = e0.elapsed_time(e1) / 3
+                            if best is None or ms < best[0]:
+                                best = (ms, flags, mode)
         self._flags, self._grid, self._mode = best[1], 0, best[2]
         self._tuned = True
 
@@ -321,10 +342,7 @@ class BatchedPscD3Q19:
 
     def _run_fused(self, steps, graph, G):
         import torch
-        if self._imb_alt is None:
-            self._imb_alt = torch.zeros_like(self._imb)
-        if self._partials_alt is None:
-            self._partials_alt = torch.zeros_like(self._partials)
+        self._ensure_fused_buffers()
         s = self._stream()
         if not hasattr(self, "_buf0"):
             self._buf0 = self._a.data_ptr()

@@ -260,7 +260,7 @@ int psc_step_batched(const float* d_in, float* d_out, const uint32_t* d_count_su
   if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
     return set_error(PSC_EINVAL, "psc_step_batched: subsamples %d", subsamples);
   if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_step_batched: tau %g must exceed 1/2", tau);
-  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK | PSC_BATCH_WG512)) != 0)
+  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK | PSC_BATCH_WG512 | PSC_BATCH_IMB_NT)) != 0)
     return set_error(PSC_EINVAL, "psc_step_batched: unknown flags 0x%x", flags);
   return launch_step_f32_batched(d_in, d_out, d_count_sum, d_imb, (size_t)imb_stride, d_windows, d_items, nitems,
                                  subsamples, tau, flags, grid_blocks, (hipStream_t)stream, nullptr, nullptr, nullptr,
@@ -281,7 +281,7 @@ int psc_step_batched_loads(const float* d_in, float* d_out, const uint32_t* d_co
   if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
     return set_error(PSC_EINVAL, "psc_step_batched_loads: subsamples %d", subsamples);
   if (!(tau > 0.5)) return set_error(PSC_EINVAL, "psc_step_batched_loads: tau %g must exceed 1/2", tau);
-  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK | PSC_BATCH_WG512)) != 0)
+  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK | PSC_BATCH_WG512 | PSC_BATCH_IMB_NT)) != 0)
     return set_error(PSC_EINVAL, "psc_step_batched_loads: unknown flags 0x%x", flags);
   if (t->nparticles < 0 || t->nchunks < 0 || t->chunk_cells < 1)
     return set_error(PSC_EINVAL, "psc_step_batched_loads: table sizes");

@@ -70,7 +70,7 @@ struct StepConsts {
   int field;  // 1: the 4-byte solid input holds alpha as an FP32 field instead of a sub-sample count
 };
 
-template <bool PULL, bool MOVING, bool IMB, bool NT>
+template <bool PULL, bool MOVING, bool IMB, bool NT, bool IMBNT = false>
 __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* __restrict__ gout,
                                           const uint32_t* __restrict__ csum, const float* __restrict__ usn,
                                           float* __restrict__ imb, size_t imb_stride, const Dims d,
@@ -196,9 +196,15 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
   // cache lines instead of leaving partial lines to merge.
   if (IMB && __any(cnt != 0)) {
     const uint32_t c = cell_id(d, ix, iy, iz);
-    imb[c] = mx;
-    imb[imb_stride + c] = my;
-    imb[2 * imb_stride + c] = mz;
+    if (IMBNT) {  // streaming stores: imb much larger than the caches
+      __builtin_nontemporal_store(mx, imb + c);
+      __builtin_nontemporal_store(my, imb + imb_stride + c);
+      __builtin_nontemporal_store(mz, imb + 2 * imb_stride + c);
+    } else {
+      imb[c] = mx;
+      imb[imb_stride + c] = my;
+      imb[2 * imb_stride + c] = mz;
+    }
   }
 #pragma unroll
   for (int i = 0; i < Q; ++i) st<NT>(go + (size_t)i * plane, dst(i), gn[i]);
@@ -397,7 +403,7 @@ __device__ __forceinline__ void divmod_f(uint32_t n, uint32_t d, float inv, uint
 // With LOADS, the item sequence also holds the load work of earlier steps
 // (fl: as for k_step_f32_loads, over the flattened cell index of all
 // windows): nchunk + nfinal extra items lead it, one in every fl.period.
-template <bool IMB, bool NT, int BS, bool LOADS>
+template <bool IMB, bool NT, int BS, bool LOADS, bool IMBNT>
 __global__ __launch_bounds__(BS) void k_step_f32_batched(const float* __restrict__ gin, float* __restrict__ gout,
                                                          const uint32_t* __restrict__ csum,
                                                          float* __restrict__ imb, size_t imb_stride,
@@ -441,7 +447,7 @@ __global__ __launch_bounds__(BS) void k_step_f32_batched(const float* __restrict
     k.Fy = w.fy;
     k.Fz = w.fz;
     const Dims d{w.nx, w.ny, w.nz, w.pz, (unsigned long long)w.ps};
-    step_cell<true, false, IMB, NT>(gin + w.state_off, gout + w.state_off, csum + w.cell_off, nullptr,
+    step_cell<true, false, IMB, NT, IMBNT>(gin + w.state_off, gout + w.state_off, csum + w.cell_off, nullptr,
                                     IMB ? imb + w.cell_off : nullptr, imb_stride, d, k, (int)ix, (int)iy, (int)iz);
   }
 }
@@ -654,22 +660,27 @@ int launch_step_f32_batched(const float* in, float* out, const uint32_t* csum, f
   const bool nt = (flags & 1) != 0;
   const bool wide = (flags & PSC_BATCH_WG512) != 0;
   const bool ld = la != nullptr;
-#define PSC_BATCH_CASE(I, T, L)                                                                                \
-  if ((imb != nullptr) == I && nt == T && ld == L) {                                                          \
+  const bool inb = imb != nullptr && (flags & PSC_BATCH_IMB_NT) != 0;
+#define PSC_BATCH_CASE(I, T, L, J)                                                                             \
+  if ((imb != nullptr) == I && nt == T && ld == L && inb == J) {                                              \
     if (wide)                                                                                                 \
-      k_step_f32_batched<I, T, 512, L><<<grid, 512, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, \
-                                                                   k, fl, tot);                               \
+      k_step_f32_batched<I, T, 512, L, J><<<grid, 512, lds, stream>>>(in, out, csum, imb, imb_stride, w, it,   \
+                                                                      nitems, k, fl, tot);                    \
     else                                                                                                      \
-      k_step_f32_batched<I, T, 256, L><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it, nitems, \
-                                                                   k, fl, tot);                               \
+      k_step_f32_batched<I, T, 256, L, J><<<grid, 256, lds, stream>>>(in, out, csum, imb, imb_stride, w, it,   \
+                                                                      nitems, k, fl, tot);                    \
     return check_launch("psc_step_batched");                                                                  \
   }
-  PSC_BATCH_CASE(true, true, false)
-  PSC_BATCH_CASE(true, false, false)
-  PSC_BATCH_CASE(false, true, false)
-  PSC_BATCH_CASE(false, false, false)
-  PSC_BATCH_CASE(true, true, true)
-  PSC_BATCH_CASE(true, false, true)
+  PSC_BATCH_CASE(true, true, false, false)
+  PSC_BATCH_CASE(true, false, false, false)
+  PSC_BATCH_CASE(false, true, false, false)
+  PSC_BATCH_CASE(false, false, false, false)
+  PSC_BATCH_CASE(true, true, true, false)
+  PSC_BATCH_CASE(true, false, true, false)
+  PSC_BATCH_CASE(true, true, false, true)
+  PSC_BATCH_CASE(true, false, false, true)
+  PSC_BATCH_CASE(true, true, true, true)
+  PSC_BATCH_CASE(true, false, true, true)
 #undef PSC_BATCH_CASE
   return set_error(PSC_EINVAL, "psc_step_batched: no kernel variant");
 }

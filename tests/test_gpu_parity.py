@@ -303,11 +303,12 @@ def test_moving_fp32_loads_track_fp64(psc3d):
 
 
 @pytest.mark.parametrize("launch,period", [("auto", 5), ((1 | 0x40000, 0, "rows"), 1), ((0x40000, 64, "spans"), 3),
-                                           ((1, 0, "spans"), 2), ((0x200, 0, "rows"), 10001)])
+                                           ((1, 0, "spans"), 2), ((0x200, 0, "rows"), 10001),
+                                           ((1 | 0x80000, 0, "spans"), 5), ((0x40000 | 0x80000, 0, "rows"), 3)])
 def test_batched_fused_loads_match_separate_reduction(launch, period):
     """Batched windows with the loads reduced inside the step launches
     (psc_step_batched_loads: 256/512-thread workgroups, persistent grid,
-    placement periods) give the state and loads of the side-stream form, bit
+    non-temporal imb stores, placement periods) give the state and loads of the side-stream form, bit
     for bit, over runs of 1, 2, 9, 17 and 3 steps (graph replays, direct
     launches and the pending loads at the end of each run)."""
     batch = load_package("batch")
@@ -337,7 +338,7 @@ def test_batched_fused_loads_match_separate_reduction(launch, period):
 
 
 @pytest.mark.parametrize("launch", ["auto", (1, 0, "spans"), (1 | 0x40000, 0, "rows"), (0x200, 0, "rows"),
-                                    (0x40000, 64, "spans")])
+                                    (0x40000, 64, "spans"), (1 | 0x80000, 0, "spans")])
 def test_batched_windows_match_single_solvers(psc3d, launch):
     """Every window of a batch (uneven sizes, every workgroup size and item
     mode, persistent grid, graph replays) equals a standalone FP32 solver of

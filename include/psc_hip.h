@@ -17,7 +17,9 @@
  *     asynchronous on `stream` (a hipStream_t, NULL = default stream).
  *   - Lattice fields are structure-of-arrays: direction-major planes of
  *     N = nx*ny*nz cells, cell id = (ix*ny + iy)*nz + iz (iz fastest), the
- *     C order of the reference's (19, n, n, n) arrays.
+ *     C order of the reference's (19, n, n, n) arrays.  The one exception is
+ *     imb, whose 3 components of a cell are consecutive (imb[3*cell + axis]):
+ *     it is written and read one cell at a time.
  *   - Distribution storage: PSC_F32_DEV stores g_i = f_i - w_i in FP32,
  *     PSC_F64_PLAIN stores f_i in FP64 with the reference's exact operation
  *     order.  Stream layout PSC_PUSH keeps the post-stream state (the
@@ -124,7 +126,8 @@ int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, in
  * fraction of a cell is min(count_sum/s^3, 1); B = a/(a + tau(1-a) + 1e-14);
  * the solid velocity is us_num/(count_sum/s^3) (zero if d_us_num is NULL).
  * The body force (fx, fy, fz) generalises body_force_z; (0, 0, Fz) is the
- * reference.  d_imb (3 planes, optional) receives sum_i (B Omega_s)_i c_i
+ * reference.  d_imb (optional; 3 values per cell, interleaved: d_imb[3*cell
+ * + axis]) receives sum_i (B Omega_s)_i c_i
  * (psc3d.py:234-236) in every cell with count_sum > 0; other cells are left
  * untouched or set to 0 (FP32 storage writes every lane of a wavefront that
  * holds a solid cell), so readers mask imb by count_sum > 0.  d_in and d_out
@@ -160,8 +163,8 @@ typedef struct PscWindow {
  * state is 19 planes of its N_w = nx*ny*pz cells (rows padded to the pitch
  * pz; padding cells are never read or written), each plane ps cells long,
  * at d_in + state_off; its
- * count_sum/imb cells start at cell_off (imb has 3 planes of imb_stride
- * cells).  d_items lists (window, first cell) pairs; an item covers the
+ * count_sum/imb cells start at cell_off (imb: 3 interleaved values per cell,
+ * imb_stride cells in all).  d_items lists (window, first cell) pairs; an item covers the
  * window's `span` cells from there (the workgroup size when span <= 0).  Each
  * workgroup (256 threads, 512 with PSC_BATCH_WG512) steps one item at a time,
  * looping over the items with a stride of grid_blocks (0 = one workgroup per
@@ -217,6 +220,7 @@ int psc_step_batched_loads(const float* d_in, float* d_out, const uint32_t* d_co
  * d_partials (nchunks*6 doubles); pass 2 adds the partials of chunks
  * d_chunk_first[p] .. d_chunk_first[p+1]-1 into d_loads (P*6 doubles: Fx Fy
  * Fz Tx Ty Tz).  Fixed thread assignment and reduction trees: deterministic.
+ * d_imb: 3 interleaved values per cell (d_imb[3*cell + axis]);
  * imb_precision: 1 = FP32 imb, 2 = FP64 imb. */
 int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
                        const int32_t* d_chunk_first, int32_t nparticles, int32_t nchunks,

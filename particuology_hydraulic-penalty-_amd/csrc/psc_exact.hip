@@ -200,10 +200,10 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
       const double coll = omb * obgk + bos[i];
       fnew[i] = FORCE ? f[i] + (coll + gterm[i]) : f[i] + coll;
     }
-    if (IMB) {
-      imb[cell] = moment<0>(bos);
-      imb[N + cell] = moment<1>(bos);
-      imb[2 * N + cell] = moment<2>(bos);
+    if (IMB) {  // interleaved: imb[3 cell + axis]
+      imb[3 * (size_t)cell] = moment<0>(bos);
+      imb[3 * (size_t)cell + 1] = moment<1>(bos);
+      imb[3 * (size_t)cell + 2] = moment<2>(bos);
     }
   }
   store_all<PULL>(Dirs{}, fout, N, d, ix, iy, iz, fnew);
@@ -573,9 +573,9 @@ __global__ __launch_bounds__(kMomThreads) void k_moments(const TS* __restrict__ 
     acc[0] += solid;
     acc[1] += fl;
     if (imb != nullptr && solid != 0.0) {
-      acc[8] += (double)imb[c];
-      acc[9] += (double)imb[N + c];
-      acc[10] += (double)imb[2 * N + c];
+      acc[8] += (double)imb[3 * c];
+      acc[9] += (double)imb[3 * c + 1];
+      acc[10] += (double)imb[3 * c + 2];
     }
     if (HAVE_STATE) {
       const int iz = (int)(c % d.pz);

@@ -70,11 +70,14 @@ struct StepConsts {
   int field;  // 1: the 4-byte solid input holds alpha as an FP32 field instead of a sub-sample count
 };
 
+// imb is interleaved: the three components of a cell are consecutive floats
+// (imb[3 c + axis]), so a cell's imb is one 12-byte store here and one
+// 12-byte read in the load reduction.
 template <bool PULL, bool MOVING, bool IMB, bool NT, bool IMBNT = false>
 __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* __restrict__ gout,
                                           const uint32_t* __restrict__ csum, const float* __restrict__ usn,
-                                          float* __restrict__ imb, size_t imb_stride, const Dims d,
-                                          const StepConsts& k, int ix, int iy, int iz) {
+                                          float* __restrict__ imb, const Dims d, const StepConsts& k, int ix,
+                                          int iy, int iz) {
   const size_t plane = (size_t)d.ps * sizeof(float);  // bytes per direction plane
   const int xm = wrap_step(ix, -1, d.nx), xp = wrap_step(ix, 1, d.nx);
   const int ym = wrap_step(iy, -1, d.ny), yp = wrap_step(iy, 1, d.ny);
@@ -195,15 +198,15 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
   // writes all its lanes (zeros at fluid cells) so the stores fill whole
   // cache lines instead of leaving partial lines to merge.
   if (IMB && __any(cnt != 0)) {
-    const uint32_t c = cell_id(d, ix, iy, iz);
+    float* q = imb + 3 * (size_t)cell_id(d, ix, iy, iz);
     if (IMBNT) {  // streaming stores: imb much larger than the caches
-      __builtin_nontemporal_store(mx, imb + c);
-      __builtin_nontemporal_store(my, imb + imb_stride + c);
-      __builtin_nontemporal_store(mz, imb + 2 * imb_stride + c);
+      __builtin_nontemporal_store(mx, q);
+      __builtin_nontemporal_store(my, q + 1);
+      __builtin_nontemporal_store(mz, q + 2);
     } else {
-      imb[c] = mx;
-      imb[imb_stride + c] = my;
-      imb[2 * imb_stride + c] = mz;
+      q[0] = mx;
+      q[1] = my;
+      q[2] = mz;
     }
   }
 #pragma unroll
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin,
   for (int c = 0; c < CPT; ++c) {
     const int iy = iy0 + c;
     if (iy < d.ny)
-      step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, usn, imb, (size_t)d.nx * d.ny * d.nz, d, k, ix, iy, iz);
+      step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, usn, imb, d, k, ix, iy, iz);
   }
 }
 
@@ -288,7 +291,7 @@ __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict_
   for (int c = 0; c < CPT; ++c) {
     const int iy = iy0 + c;
     if (iy < d.ny)
-      step_cell<true, false, true, NT>(gin, gout, csum, nullptr, imb, (size_t)d.nx * d.ny * d.nz, d, k, ix, iy, iz);
+      step_cell<true, false, true, NT>(gin, gout, csum, nullptr, imb, d, k, ix, iy, iz);
   }
 }
 
@@ -448,7 +451,7 @@ __global__ __launch_bounds__(BS) void k_step_f32_batched(const float* __restrict
     k.Fz = w.fz;
     const Dims d{w.nx, w.ny, w.nz, w.pz, (unsigned long long)w.ps};
     step_cell<true, false, IMB, NT, IMBNT>(gin + w.state_off, gout + w.state_off, csum + w.cell_off, nullptr,
-                                    IMB ? imb + w.cell_off : nullptr, imb_stride, d, k, (int)ix, (int)iy, (int)iz);
+                                           IMB ? imb + 3 * w.cell_off : nullptr, d, k, (int)ix, (int)iy, (int)iz);
   }
 }
 

@@ -43,7 +43,6 @@ __device__ __forceinline__ void loads_chunk_body(const LoadsArgs& a, const TI* _
   const int x0 = b[0], y0 = b[1], z0 = b[2];
   const double* q = a.parts + 10 * (size_t)pid;
   const double px = q[0], py = q[1], pz = q[2];
-  const size_t N = (size_t)nx * ny * nz;
   const size_t out0 = (size_t)c * a.chunk_cells;
   constexpr uint32_t kMask = (1u << kLocBits) - 1u;
   const int lane = tid & 63, pwave = tid >> 6, nwave = nt >> 6;
@@ -65,9 +64,9 @@ __device__ __forceinline__ void loads_chunk_body(const LoadsArgs& a, const TI* _
       for (int j = 0; j < KB; ++j) {
         const bool ok = cnt[j] != 0u;
         tot[j] = ok ? a.csum[cell[j]] : 1u;
-        ix_[j] = ok ? imb[cell[j]] : (TI)0;
-        iy_[j] = ok ? imb[N + cell[j]] : (TI)0;
-        iz_[j] = ok ? imb[2 * N + cell[j]] : (TI)0;
+        ix_[j] = ok ? imb[3 * (size_t)cell[j]] : (TI)0;
+        iy_[j] = ok ? imb[3 * (size_t)cell[j] + 1] : (TI)0;
+        iz_[j] = ok ? imb[3 * (size_t)cell[j] + 2] : (TI)0;
       }
 #pragma unroll
       for (int j = 0; j < KB; ++j) {

@@ -596,7 +596,8 @@ class PscD3Q19:
     def _imb_np(self):
         if "imb" not in self._cache:
             mask = self._np(self._csum).reshape(self._shape) != 0
-            imb = self._np(self._imb).astype(np.float64).reshape((3,) + self._shape)
+            # interleaved on the device: imb[3 cell + axis]
+            imb = np.moveaxis(self._np(self._imb).astype(np.float64).reshape(self._shape + (3,)), -1, 0)
             self._cache["imb"] = np.where(mask[None], imb, 0.0)
         return self._cache["imb"]
 

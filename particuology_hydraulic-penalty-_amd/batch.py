@@ -361,7 +361,7 @@ class BatchedPscD3Q19:
         direct(head)
         done += head
         if graph and steps - done >= G:
-            key = ("fused", q, G)
+            key = ("fused", q, G, self._flags, self._mode, self._grid, int(self.fuse_period))
             if getattr(self, "_fgraph", None) is None or self._fgraph[0] != key:
                 a, b = self._a, self._b
                 torch.cuda.synchronize(self.device)

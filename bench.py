@@ -163,6 +163,12 @@ def cpu_baseline(table, n, steps, quiet):
     return n ** 3 * steps / dt / 1e6, dt
 
 
+def imb_bytes(csum):
+    """Algorithmic bytes of the imb output: 12 per cell with solid (count_sum > 0)."""
+    n = int((csum != 0).sum().item())
+    return 12 * n, n
+
+
 def loads_bytes(sol):
     """Algorithmic bytes of one step's load reduction: per compacted entry
     4 (cell) + 4 (box coordinates) + 2 (count) + 4 (count_sum) + 12 (imb),
@@ -218,6 +224,7 @@ def bench_moving(args, psc3d, workloads, lib, torch, np):
     log(f"moving window {n}^3: {warm} warm-up + {steps} steps, FP32", args.quiet)
     s32, w32 = run("fp32", steps, True)
     kern = step_kernel_ms(s32, 20, torch, lib, moving=True)
+    nsolid = int((s32._csum != 0).sum().item())   # cells that read us_num and write imb
     loads32 = s32._loads_np().copy()
     mlups = N * steps / w32 / 1e6
     log(f"fp32: {mlups:.0f} MLUPS, step kernel {kern*1e3:.0f} us", args.quiet)
@@ -231,8 +238,10 @@ def bench_moving(args, psc3d, workloads, lib, torch, np):
     Ts = np.linalg.norm(loads64[:, 3:6], axis=1)
     eF = np.linalg.norm(loads32[:, 0:3] - loads64[:, 0:3], axis=1) / np.maximum(Fs, 1e-3 * Fs.max())
     eT = np.linalg.norm(loads32[:, 3:6] - loads64[:, 3:6], axis=1) / np.maximum(Ts, 1e-3 * Ts.max())
-    bytes_cell = 180
-    achieved = bytes_cell * N / (kern * 1e-3) / 1e9
+    # algorithmic bytes: 156 per cell plus, at the cells with solid, 12 of solid velocity read
+    # and 12 of imb written
+    launch_bytes = BYTES_F32_STATIC * N + 24 * nsolid
+    achieved = launch_bytes / (kern * 1e-3) / 1e9
     out = {
         "metric": METRIC + " (moving-window configuration)",
         "value": mlups, "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warm,
@@ -244,7 +253,8 @@ def bench_moving(args, psc3d, workloads, lib, torch, np):
                    "lattice": [n, n, n], "particles": int(base.shape[0])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_step_f32 (pull, moving)",
-                     "bytes_per_cell": bytes_cell, "kernel_us": kern * 1e3},
+                     "bytes_per_cell": BYTES_F32_STATIC, "solid_cells": nsolid, "bytes_per_launch": launch_bytes,
+                     "kernel_us": kern * 1e3},
         "fp64_MLUPS": N * steps / w64 / 1e6,
         "force_rel_err_fp32_vs_fp64": {"max": float(eF.max()), "median": float(np.median(eF))},
         "torque_rel_err_fp32_vs_fp64": {"max": float(eT.max()), "median": float(np.median(eT))},
@@ -310,16 +320,18 @@ def bench_windows(args, workloads, torch, np):
     cells = int(B.nlive.sum())  # live cells only: row and plane padding is neither read nor written
     nnz = int(B._nnz[:B.C].sum().item())
     lbytes = 26 * nnz + 96 * B.C + 48 * B.P
+    ibytes, nsolid = imb_bytes(B._csum)
+    step_bytes = BYTES_F32_STATIC * cells + ibytes
     if fused:
         # every timed step is one k_step_f32_batched<..., loads> launch: the HIP-event time of
         # the timed region over the step count is its average duration (plus the pending
         # loads once at the end of the run)
         kern_l = gpu_ms / steps
-        launch_bytes = BYTES_F32_STATIC * cells + lbytes
+        launch_bytes = step_bytes + lbytes
         achieved = launch_bytes / (kern_l * 1e-3) / 1e9
         kname, kus = "k_step_f32_batched (with the loads of the two previous steps)", kern_l * 1e3
     else:
-        launch_bytes = BYTES_F32_STATIC * cells
+        launch_bytes = step_bytes
         achieved = launch_bytes / (kern * 1e-3) / 1e9
         kname, kus = "k_step_f32_batched", kern * 1e3
     por = B.porosity()
@@ -340,9 +352,10 @@ def bench_windows(args, workloads, torch, np):
                                              else "k_step_f32_batched_windows_pmc.txt"),
                      "kernel": kname,
                      "bytes_per_cell": BYTES_F32_STATIC, "bytes_per_launch": launch_bytes,
+                     "solid_cells": nsolid, "imb_bytes": ibytes,
                      "loads_bytes_per_step": lbytes, "loads_entries": nnz, "kernel_us": kus,
                      "step_kernel_alone_us": kern * 1e3,
-                     "step_kernel_alone_GBps": BYTES_F32_STATIC * cells / (kern * 1e-3) / 1e9,
+                     "step_kernel_alone_GBps": step_bytes / (kern * 1e-3) / 1e9,
                      "launch": {"flags": B._flags, "grid": B._grid, "items": B._mode,
                                 "workgroup": 512 if B._flags & 0x40000 else 256,
                                 "fuse_period": B.fuse_period if fused else None}},
@@ -641,6 +654,7 @@ def main():
     fused = sol32.loads_mode == "fused"
     kern_ms = step_kernel_ms(sol32, args.kernel_repeats, torch, lib)
     lbytes, nnz = loads_bytes(sol32)
+    ibytes, nsolid = imb_bytes(sol32._csum)
     # fused: every timed step is one k_step_f32_loads launch, so the HIP-event time of the
     # timed region over the step count is that kernel's average launch duration (it also
     # holds the once-per-run pending loads and the k readout, ~0.1 ms in all)
@@ -678,14 +692,15 @@ def main():
                          f"one thread; host {platform.processor() or platform.machine()}, "
                          f"os.cpu_count()={os.cpu_count()}"}
 
+    step_bytes = BYTES_F32_STATIC * N + ibytes
     if fused:
         # the dominant kernel of the timed step: collide+stream plus the loads of earlier steps
-        launch_bytes = BYTES_F32_STATIC * N + lbytes
+        launch_bytes = step_bytes + lbytes
         achieved = launch_bytes / (fused_ms * 1e-3) / 1e9
         roof_kernel, roof_us, traffic = ("k_step_f32_loads (pull, static; step + loads of the two previous steps)",
                                          fused_ms * 1e3, read_traffic("k_step_f32_loads_128_pmc.txt"))
     else:
-        launch_bytes = BYTES_F32_STATIC * N
+        launch_bytes = step_bytes
         achieved = launch_bytes / (kern_ms * 1e-3) / 1e9
         roof_kernel, roof_us, traffic = ("k_step_f32 (pull, static)", kern_ms * 1e3,
                                          read_traffic("k_step_f32_128_pmc.txt"))
@@ -724,11 +739,13 @@ def main():
                        "fuse_period": sol32.fuse_period if fused else None},
             "bytes_per_launch": launch_bytes,
             "bytes_per_cell": BYTES_F32_STATIC,
+            "solid_cells": nsolid,
+            "imb_bytes": ibytes,
             "loads_bytes_per_step": lbytes,
             "loads_entries": nnz,
             "kernel_us": roof_us,
             "step_kernel_alone_us": kern_ms * 1e3,
-            "step_kernel_alone_GBps": BYTES_F32_STATIC * N / (kern_ms * 1e-3) / 1e9,
+            "step_kernel_alone_GBps": step_bytes / (kern_ms * 1e-3) / 1e9,
         },
         "cpu_baseline": cpu,
         "k_fp32": k32,

@@ -523,12 +523,18 @@ __global__ __launch_bounds__(256) void k_clear_entries(const uint32_t* __restric
 #define PSC_LOADS_GRID 0
 #endif
 
+// Entries per thread per round of the chunk pass (a register batch: the sums
+// and their bits do not depend on it).
+#ifndef PSC_LOADS_KB
+#define PSC_LOADS_KB kLoadsKB
+#endif
+
 template <typename TI>
 __global__ __launch_bounds__(PSC_LOADS_THREADS) void k_loads_chunk(LoadsArgs a, const TI* __restrict__ imb, Dims d,
                                                                    double* __restrict__ partials) {
   __shared__ double lds[6 * 4];
   for (int c = blockIdx.x; c < a.nchunks; c += gridDim.x)
-    loads_chunk_body<TI>(a, imb, d.nx, d.ny, d.nz, c, threadIdx.x, blockDim.x, partials, lds);
+    loads_chunk_body<TI, PSC_LOADS_KB>(a, imb, d.nx, d.ny, d.nz, c, threadIdx.x, blockDim.x, partials, lds);
 }
 
 inline int loads_grid(int nchunks) {

@@ -269,7 +269,7 @@ __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict_
         const int tid = threadIdx.y * blockDim.x + threadIdx.x;
         if (e < fl.nchunk) {
           __shared__ double lds[6 * 4];
-          loads_chunk_body<float, 4>(fl.a, fl.imb_prev, d.nx, d.ny, d.nz, e, tid, blockDim.x * blockDim.y,
+          loads_chunk_body<float>(fl.a, fl.imb_prev, d.nx, d.ny, d.nz, e, tid, blockDim.x * blockDim.y,
                                      fl.part_out, lds);
         } else {
           loads_final_body(fl.a.chunk_first, e - fl.nchunk, tid, fl.part_prev, fl.loads);
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(BS) void k_step_f32_batched(const float* __restrict
         if (r == fl.period - 1) {
           if (e < fl.nchunk) {
             __shared__ double lds[6 * 4];
-            loads_chunk_body<float, 4>(fl.a, fl.imb_prev, 1, 1, (int)imb_stride, e, tid, BS, fl.part_out, lds);
+            loads_chunk_body<float>(fl.a, fl.imb_prev, 1, 1, (int)imb_stride, e, tid, BS, fl.part_out, lds);
           } else {
             loads_final_body(fl.a.chunk_first, e - fl.nchunk, tid, fl.part_prev, fl.loads);
           }

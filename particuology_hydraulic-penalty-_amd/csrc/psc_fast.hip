@@ -3,8 +3,12 @@
 // Storage is g_i = f_i - w_i in FP32, structure of arrays.  One thread owns
 // one lattice cell; a workgroup is a (block_z x block_y) tile of one x-plane,
 // so a wavefront reads and writes 64 consecutive z cells of each direction
-// plane (coalesced).  No LDS and no MFMA: the step is HBM-bound at
-// 19*4 B read + 19*4 B written + 4 B of solid count per cell.
+// plane (coalesced).  No MFMA, and the cell update uses no LDS: the step is
+// HBM-bound at 19*4 B read + 19*4 B written + 4 B of solid count per cell
+// (+ 12 B of imb at cells with solid).  The step kernels that also carry
+// the particle-load reduction of earlier steps (k_step_f32_loads and the
+// LOADS form of k_step_f32_batched) use a few hundred bytes of LDS in those
+// load workgroups only.
 //
 // The arithmetic restates src/lbm_dem/psc3d.py:211-236 in deviation form:
 //   rho = 1 + sum g,  j = sum g c,  u_eff = (j + F)/rho   (both half shifts)

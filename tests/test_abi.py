@@ -73,6 +73,37 @@ def test_argument_errors_are_reported(lib):
     assert h.psc_version() >= 10000
 
 
+def test_fused_load_entry_points_check_arguments(lib):
+    """psc_step_batched_loads and the fused form of psc_run_steps reject bad
+    arguments before touching a GPU."""
+    h = lib.load()
+    t = lib.PscLoadsTables()
+    p16, p32, p48, p64 = (ct.c_void_p(v) for v in (16, 32, 48, 64))
+    args = [p16, p32, p64, p48, 100, p64, p64, 1, 3, 0.8, 0, 0]
+    rc = h.psc_step_batched_loads(*args, None, None, None, None, None, 1, None)
+    assert rc == -1 and b"null" in h.psc_last_error()  # no tables
+    rc = h.psc_step_batched_loads(*args, ct.byref(t), None, None, None, None, 0, None)
+    assert rc == -1 and b"sizes" in h.psc_last_error()  # period 0
+    t.nparticles, t.nchunks, t.chunk_cells = 2, 3, 2048
+    rc = h.psc_step_batched_loads(*args, ct.byref(t), p48, p16, None, None, 5, None)
+    assert rc == -1 and b"imb" in h.psc_last_error()  # imb_prev aliases imb
+    rc = h.psc_step_batched_loads(*args, ct.byref(t), p16, p16, None, None, 5, None)
+    assert rc == -1  # chunk tables missing
+    rc = h.psc_step_batched_loads(*args[:10], 0x100000, 0, ct.byref(t), None, None, None, None, 5, None)
+    assert rc == -1 and b"flags" in h.psc_last_error()
+    plan = lib.PscStepPlan()
+    plan.state_a, plan.state_b, plan.count_sum = 16, 32, 64
+    plan.imb, plan.imb_alt, plan.partials, plan.partials_alt = 128, 128, 256, 512
+    plan.storage, plan.layout, plan.nx, plan.ny, plan.nz, plan.subsamples = 1, 1, 8, 8, 8, 2
+    plan.tau, plan.nparticles, plan.fuse_loads = 0.8, 2, 5
+    par = ct.c_int32(0)
+    rc = h.psc_run_steps(ct.byref(plan), 4, 0, 16, ct.byref(par), None)
+    assert rc == -1 and b"alias" in h.psc_last_error()
+    plan.imb_alt, plan.tau = 192, 0.5
+    rc = h.psc_run_steps(ct.byref(plan), 4, 0, 16, ct.byref(par), None)
+    assert rc == -1 and b"tau" in h.psc_last_error()
+
+
 def test_solver_refuses_to_run_without_gpu():
     torch = pytest.importorskip("torch")
     if torch.cuda.is_available():

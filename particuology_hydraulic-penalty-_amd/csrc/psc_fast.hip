@@ -74,32 +74,16 @@ struct StepConsts {
   int field;  // 1: the 4-byte solid input holds alpha as an FP32 field instead of a sub-sample count
 };
 
-// imb is interleaved: the three components of a cell are consecutive floats
-// (imb[3 c + axis]), so a cell's imb is one 12-byte store here and one
-// 12-byte read in the load reduction.
-template <bool PULL, bool MOVING, bool IMB, bool NT, bool IMBNT = false>
-__device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* __restrict__ gout,
-                                          const uint32_t* __restrict__ csum, const float* __restrict__ usn,
-                                          float* __restrict__ imb, const Dims d, const StepConsts& k, int ix,
-                                          int iy, int iz) {
-  const size_t plane = (size_t)d.ps * sizeof(float);  // bytes per direction plane
-  const int xm = wrap_step(ix, -1, d.nx), xp = wrap_step(ix, 1, d.nx);
-  const int ym = wrap_step(iy, -1, d.ny), yp = wrap_step(iy, 1, d.ny);
-  const int zm = wrap_step(iz, -1, d.nz), zp = wrap_step(iz, 1, d.nz);
-  auto X = [&](int s) { return s < 0 ? xm : (s > 0 ? xp : ix); };
-  auto Y = [&](int s) { return s < 0 ? ym : (s > 0 ? yp : iy); };
-  auto Z = [&](int s) { return s < 0 ? zm : (s > 0 ? zp : iz); };
-  // byte offset of the cell displaced by (sx, sy, sz)
-  auto off = [&](int sx, int sy, int sz) -> uint32_t { return cell_id(d, X(sx), Y(sy), Z(sz)) << 2; };
-  const uint32_t here = cell_id(d, ix, iy, iz) << 2;
-  const char* gi = (const char*)gin;
-  char* go = (char*)gout;
-
-  float g[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i)
-    g[i] = ld<NT>(gi + (size_t)i * plane, PULL ? off(-cx_of(i), -cy_of(i), -cz_of(i)) : here);
-
+// The collision of one cell from its gathered populations g (deviation
+// form) and solid count cnt: post-collision values gn and, at a solid cell,
+// its imb (mx, my, mz; 0 in fluid).  usn/cell/N: the solid-velocity
+// numerator planes and the cell's index, read only by MOVING solid cells.
+// Every step kernel calls this one function, so every launch form computes
+// the same bits.
+template <bool MOVING>
+__device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, const StepConsts& k,
+                                             const float* __restrict__ usn, uint32_t cell, size_t N, float (&gn)[Q],
+                                             float& mx, float& my, float& mz) {
   const float drho = ((g[0] + (g[1] + g[2])) + ((g[3] + g[4]) + (g[5] + g[6]))) +
                      (((g[7] + g[8]) + (g[9] + g[10])) + ((g[11] + g[12]) + (g[13] + g[14]))) +
                      ((g[15] + g[16]) + (g[17] + g[18]));
@@ -120,14 +104,9 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
   auto wrh = [&](int i) { return (float)w_of(i) * rho; };
   auto pw = [&](int i) { return k.pref * (float)w_of(i); };
 
-  const uint32_t cnt = csum[cell_id(d, ix, iy, iz)];
-  // destination offset of direction i
-  auto dst = [&](int i) -> uint32_t { return PULL ? here : off(cx_of(i), cy_of(i), cz_of(i)); };
-
-  // Post-collision values; stored by one store per direction after the
-  // branch, so every store keeps the uniform-base + 32-bit-offset form.
-  float gn[Q];
-  float mx = 0.f, my = 0.f, mz = 0.f;  // imb of a solid cell (0 in fluid)
+  mx = 0.f;
+  my = 0.f;
+  mz = 0.f;
   if (cnt == 0) {
     // fluid cell: g' = (1 - 1/tau) g + geq/tau + Guo, pairwise even/odd parts
     {
@@ -156,8 +135,7 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
     const float omb = (1.0f - B) * k.itau;
     float sx = 0.f, sy = 0.f, sz = 0.f, ssq15 = 0.f;
     if (MOVING) {
-      const uint32_t c = cell_id(d, ix, iy, iz);
-      const size_t N = (size_t)d.ps;
+      const uint32_t c = cell;
       const float ia = 1.0f / ((float)cnt * k.inv_s3);
       sx = usn[c] * ia;
       sy = usn[N + c] * ia;
@@ -198,6 +176,43 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
       if (cz_of(i)) mz += (float)cz_of(i) * dm;
     }
   }
+}
+
+// One cell: gather, collide, store.  imb is interleaved: the three
+// components of a cell are consecutive floats (imb[3 c + axis]), so a cell's
+// imb is one 12-byte store here and one 12-byte read in the load reduction.
+template <bool PULL, bool MOVING, bool IMB, bool NT, bool IMBNT = false>
+__device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* __restrict__ gout,
+                                          const uint32_t* __restrict__ csum, const float* __restrict__ usn,
+                                          float* __restrict__ imb, const Dims d, const StepConsts& k, int ix,
+                                          int iy, int iz) {
+  const size_t plane = (size_t)d.ps * sizeof(float);  // bytes per direction plane
+  const int xm = wrap_step(ix, -1, d.nx), xp = wrap_step(ix, 1, d.nx);
+  const int ym = wrap_step(iy, -1, d.ny), yp = wrap_step(iy, 1, d.ny);
+  const int zm = wrap_step(iz, -1, d.nz), zp = wrap_step(iz, 1, d.nz);
+  auto X = [&](int s) { return s < 0 ? xm : (s > 0 ? xp : ix); };
+  auto Y = [&](int s) { return s < 0 ? ym : (s > 0 ? yp : iy); };
+  auto Z = [&](int s) { return s < 0 ? zm : (s > 0 ? zp : iz); };
+  // byte offset of the cell displaced by (sx, sy, sz)
+  auto off = [&](int sx, int sy, int sz) -> uint32_t { return cell_id(d, X(sx), Y(sy), Z(sz)) << 2; };
+  const uint32_t here = cell_id(d, ix, iy, iz) << 2;
+  const char* gi = (const char*)gin;
+  char* go = (char*)gout;
+
+  float g[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+    g[i] = ld<NT>(gi + (size_t)i * plane, PULL ? off(-cx_of(i), -cy_of(i), -cz_of(i)) : here);
+
+  const uint32_t cnt = csum[cell_id(d, ix, iy, iz)];
+  // destination offset of direction i
+  auto dst = [&](int i) -> uint32_t { return PULL ? here : off(cx_of(i), cy_of(i), cz_of(i)); };
+
+  // Post-collision values; stored by one store per direction after the
+  // collision, so every store keeps the uniform-base + 32-bit-offset form.
+  float gn[Q];
+  float mx, my, mz;  // imb of a solid cell (0 in fluid)
+  collide_cell<MOVING>(g, cnt, k, usn, cell_id(d, ix, iy, iz), (size_t)d.ps, gn, mx, my, mz);
   // imb is defined at solid cells only; a wavefront with any solid cell
   // writes all its lanes (zeros at fluid cells) so the stores fill whole
   // cache lines instead of leaving partial lines to merge.
@@ -573,10 +588,11 @@ int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, flo
   k.Fz = (float)fz;
   const dim3 block(bz, by, 1);
   const bool ntb = (nt & 1) != 0;
-  if (ntb && cpt == 2) k_step_f32_loads<true, 2><<<(unsigned)grid, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
-  else if (ntb) k_step_f32_loads<true, 1><<<(unsigned)grid, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
-  else if (cpt == 2) k_step_f32_loads<false, 2><<<(unsigned)grid, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
-  else k_step_f32_loads<false, 1><<<(unsigned)grid, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
+  const unsigned g = (unsigned)grid;
+  if (ntb && cpt == 2) k_step_f32_loads<true, 2><<<g, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
+  else if (ntb) k_step_f32_loads<true, 1><<<g, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
+  else if (cpt == 2) k_step_f32_loads<false, 2><<<g, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
+  else k_step_f32_loads<false, 1><<<g, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
   return check_launch("psc_step(f32+loads)");
 }
 

@@ -59,6 +59,11 @@ extern "C" {
 #define PSC_STEP_CPT_MASK 0x30000   /* bits 16-17: cells per thread along y (1, 2)  */
 #define PSC_BATCH_WG512 0x40000     /* psc_step_batched: 512-thread workgroups      */
 #define PSC_BATCH_IMB_NT 0x80000    /* psc_step_batched: non-temporal imb stores    */
+#define PSC_STEP_LINEAR 0x100000    /* bit 20: FP32 steps; a workgroup takes        */
+                                    /* block_z*block_y consecutive cells of an x    */
+                                    /* plane's flat (y, z) index instead of a       */
+                                    /* (z, y) tile; one cell per thread,            */
+                                    /* ny*nz < 2^24; no idle lanes at row ends      */
 
 #define PSC_NSUM 13 /* number of FP64 sums produced by psc_moments           */
 

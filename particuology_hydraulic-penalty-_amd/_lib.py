@@ -22,6 +22,7 @@ PSC_NSUM = 13
 PSC_ALPHA_FIELD = 0
 PSC_BATCH_WG512 = 0x40000
 PSC_BATCH_IMB_NT = 0x80000
+PSC_STEP_LINEAR = 0x100000
 
 # Names and argument types of every exported entry point, in header order.
 _P = ct.c_void_p

@@ -7,8 +7,11 @@ open at once).  The best point moves with the lattice shape, so it is
 measured, not guessed: every candidate is timed with HIP events on the
 solver's own buffers (state A -> scratch B; the state is not modified) and
 the fastest is kept.  Results are cached per (device, lattice, storage,
-stream layout, moving) in memory and in a JSON file
-($PSC_HIP_TUNE_CACHE, default ~/.cache/psc_hip/step_tuning.json).
+stream layout, moving, fused loads) in memory and in a JSON file
+($PSC_HIP_TUNE_CACHE, default ~/.cache/psc_hip/step_tuning.json).  When the
+native loop fuses the particle loads into the step kernels, candidates are
+timed in that loop (on scratch copies of the buffers): the extra load
+workgroups shift the best occupancy cap.
 
 All candidates compute bit-identical results; only the launch differs.
 """
@@ -24,8 +27,12 @@ from . import _lib
 _LOCK = threading.Lock()
 _MEM: dict = {}
 
-# (block_z, block_y, nontemporal, workgroups-per-CU cap; 0 = none)
+# (block_z, block_y, nontemporal, workgroups-per-CU cap; 0 = none, flat mapping)
 SHAPES = [(64, 4), (64, 2), (64, 1), (96, 2), (128, 2), (128, 1), (192, 1), (256, 1)]
+# Workgroup sizes of the flat mapping (PSC_STEP_LINEAR: consecutive cells of
+# an x plane's flat (y, z) index), tried when z rows are not a whole number of
+# wavefronts: a (z, y) tile then leaves the lanes past each row end idle.
+FLAT = (64, 128, 256)
 NT = (0, 1)
 BPC = (0, 2, 3, 4, 6, 8)
 
@@ -37,7 +44,12 @@ def candidates(nz: int):
             continue  # more than half the threads of a row would idle
         for nt in NT:
             for bpc in BPC:
-                out.append((bz, by, nt, bpc))
+                out.append((bz, by, nt, bpc, False))
+    if nz % 64:
+        for bz in FLAT:
+            for nt in NT:
+                for bpc in BPC:
+                    out.append((bz, 1, nt, bpc, True))
     return out
 
 
@@ -76,8 +88,9 @@ def _store_file(key, cfg):
         pass  # the cache is an optimisation only
 
 
-def flags_of(nt: int, bpc: int, cpt: int = 1) -> int:
-    return (_lib.PSC_STEP_NONTEMPORAL if nt else 0) | ((int(bpc) & 0xFF) << 8) | ((int(cpt) & 3) << 16)
+def flags_of(nt: int, bpc: int, cpt: int = 1, flat: bool = False) -> int:
+    return ((_lib.PSC_STEP_NONTEMPORAL if nt else 0) | ((int(bpc) & 0xFF) << 8) | ((int(cpt) & 3) << 16)
+            | (_lib.PSC_STEP_LINEAR if flat else 0))
 
 
 def time_config(solver, bz, by, flags, repeats):
@@ -103,9 +116,50 @@ def time_config(solver, bz, by, flags, repeats):
     return e0.elapsed_time(e1) / repeats
 
 
-def tune(solver, budget_ms: float = 400.0, use_cache: bool = True):
-    """Return (block_z, block_y, flags) of the fastest candidate for solver."""
-    key = _key(solver)
+def _fused_scratch(solver):
+    """Buffers of the fused timing loop, apart from the solver's own: it steps
+    copies, so the solver's state, imb and loads are not modified."""
+    import torch
+    a = solver._a.clone()
+    imb = solver._imb_bufs[0]
+    return (a, torch.empty_like(a), torch.zeros_like(imb), torch.zeros_like(imb),
+            torch.zeros_like(solver._partials), torch.zeros_like(solver._partials), torch.zeros_like(solver._loads))
+
+
+def time_fused_config(solver, scratch, bz, by, flags, repeats):
+    """Average step duration (ms) of `repeats` steps of the native loop
+    (psc_run_steps, direct launches) with the loads fused into the step
+    kernels, on the scratch buffers of _fused_scratch."""
+    import ctypes as ct
+    import torch
+    a, b, i0, i1, q0, q1, ld = scratch
+    p = solver._plan(False, True)
+    p.state_a, p.state_b = a.data_ptr(), b.data_ptr()
+    p.imb, p.imb_alt = i0.data_ptr(), i1.data_ptr()
+    p.partials, p.partials_alt, p.loads = q0.data_ptr(), q1.data_ptr(), ld.data_ptr()
+    p.block_z, p.block_y, p.flags = bz, by, flags
+    stream = torch.cuda.current_stream(solver.device)
+    h = stream.cuda_stream
+    lib = _lib.load()
+    parity = ct.c_int32(0)
+    _lib.check(lib.psc_run_steps(ct.byref(p), 4, 0, 16, ct.byref(parity), h), "psc_run_steps")
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    rc = lib.psc_run_steps(ct.byref(p), repeats, 0, 16, ct.byref(parity), h)
+    e1.record(stream)
+    _lib.check(rc, "psc_run_steps")
+    e1.synchronize()
+    return e0.elapsed_time(e1) / repeats
+
+
+def tune(solver, budget_ms: float = 400.0, use_cache: bool = True, fused: bool = False):
+    """Return (block_z, block_y, flags) of the fastest candidate for solver.
+
+    fused: time each candidate in the native step loop with the loads fused
+    into the step kernels (the solver's plan, on scratch copies), instead of
+    the plain step kernel."""
+    key = _key(solver) + ("|fused" if fused else "")
     with _LOCK:
         if use_cache:
             if key in _MEM:
@@ -120,10 +174,19 @@ def tune(solver, budget_ms: float = 400.0, use_cache: bool = True):
         # one probe launch sizes the repeat count to the time budget
         probe = time_config(solver, 64, 4, 0, 1)
         reps = max(2, min(50, int(budget_ms / max(probe, 1e-3) / len(cands))))
+        if fused:
+            scratch = _fused_scratch(solver)
+            reps = max(12, reps)  # the loop's two head steps and final flush are amortised
+
+            def timer(bz, by, flags):
+                return time_fused_config(solver, scratch, bz, by, flags, reps)
+        else:
+            def timer(bz, by, flags):
+                return time_config(solver, bz, by, flags, reps)
         best = None
-        for bz, by, nt, bpc in cands:
-            flags = flags_of(nt, bpc)
-            ms = time_config(solver, bz, by, flags, reps)
+        for bz, by, nt, bpc, flat in cands:
+            flags = flags_of(nt, bpc, 1, flat)
+            ms = timer(bz, by, flags)
             if best is None or ms < best[0]:
                 best = (ms, bz, by, flags)
         cfg = (best[1], best[2], best[3])

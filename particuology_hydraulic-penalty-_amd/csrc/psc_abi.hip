@@ -145,9 +145,11 @@ int step_args(const void* d_in, const void* d_out, const uint32_t* d_count_sum, 
   // default: 64-cell row tiles, 32 when rows are a multiple of 32 but not of 64
   if (*bz <= 0) *bz = (nz >= 64 && !(nz % 64 != 0 && nz % 32 == 0)) ? 64 : 32;
   if (*by <= 0) *by = 256 / *bz > 0 ? 256 / *bz : 1;
-  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK)) != 0)
+  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK | PSC_STEP_LINEAR)) != 0)
     return set_error(PSC_EINVAL, "psc_step: unknown flags 0x%x", flags);
   if (((flags & PSC_STEP_CPT_MASK) >> 16) > 2) return set_error(PSC_EINVAL, "psc_step: cells per thread > 2");
+  if ((flags & PSC_STEP_LINEAR) && ((flags & PSC_STEP_CPT_MASK) >> 16) > 1)
+    return set_error(PSC_EINVAL, "psc_step: the flat mapping takes one cell per thread");
   if (*bz * *by > 256 || *bz * *by < 1 || ((*bz * *by) % 64) != 0)
     return set_error(PSC_ELAUNCH, "psc_step: block %dx%d must hold a multiple of 64 threads, at most 256", *bz, *by);
   return PSC_OK;

@@ -232,16 +232,38 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
   for (int i = 0; i < Q; ++i) st<NT>(go + (size_t)i * plane, dst(i), gn[i]);
 }
 
+// q = n / d, r = n % d for n < 2^24 from an FP32 reciprocal, corrected.
+__device__ __forceinline__ void divmod_f(uint32_t n, uint32_t d, float inv, uint32_t& q, uint32_t& r) {
+  q = (uint32_t)((float)n * inv);
+  int rr = (int)n - (int)(q * d);
+  while (rr < 0) { --q; rr += (int)d; }
+  while (rr >= (int)d) { ++q; rr -= (int)d; }
+  r = (uint32_t)rr;
+}
+
 // Grid: (z tiles, y tiles, x).  A thread updates CPT cells at consecutive y.
 // The dynamic LDS allocation is unused storage that caps the number of
 // resident workgroups per CU (an occupancy limit chosen by the tuner); it
 // leaves kLdsHeadroom bytes free so the small load-reduction workgroups of
 // the previous step, running on another stream, can share the CU.
-template <bool PULL, bool MOVING, bool IMB, bool NT, int CPT>
+//
+// LIN (flat mapping; grid (workgroups per x plane, x)): a workgroup takes
+// blockDim.x*blockDim.y consecutive cells of an x plane's flat (y, z) index,
+// so rows that are not a whole number of wavefronts leave no idle lanes
+// (136-cell rows keep 136 of 192 lanes busy with one row per z tile).
+template <bool PULL, bool MOVING, bool IMB, bool NT, int CPT, bool LIN>
 __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin, float* __restrict__ gout,
                                                   const uint32_t* __restrict__ csum,
                                                   const float* __restrict__ usn, float* __restrict__ imb, Dims d,
-                                                  StepConsts k) {
+                                                  StepConsts k, float inv_nz) {
+  if (LIN) {
+    const uint32_t c = blockIdx.x * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
+    if (c >= (uint32_t)d.ny * (uint32_t)d.nz) return;
+    uint32_t iy, iz;
+    divmod_f(c, (uint32_t)d.nz, inv_nz, iy, iz);
+    step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, usn, imb, d, k, blockIdx.y, (int)iy, (int)iz);
+    return;
+  }
   const int iz = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy0 = (blockIdx.y * blockDim.y + threadIdx.y) * CPT;
   const int ix = blockIdx.z;
@@ -275,10 +297,13 @@ struct FusedLoads {
   int period;                // extra workgroups lead, one per period workgroups (>= 1)
 };
 
-template <bool NT, int CPT>
+// LIN: flat mapping as in k_step_f32; gz is then the number of workgroups per
+// x plane and gy is 1.
+template <bool NT, int CPT, bool LIN>
 __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict__ gin, float* __restrict__ gout,
                                                         const uint32_t* __restrict__ csum, float* __restrict__ imb,
-                                                        Dims d, StepConsts k, FusedLoads fl, int gz, int gy) {
+                                                        Dims d, StepConsts k, FusedLoads fl, int gz, int gy,
+                                                        float inv_nz) {
   int b = blockIdx.x;
   const int nextra = fl.nchunk + fl.nfinal;
   if (nextra > 0) {
@@ -299,6 +324,16 @@ __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict_
     } else {
       b -= nextra;
     }
+  }
+  if (LIN) {
+    const int ix = b / gz;
+    if (ix >= d.nx) return;
+    const uint32_t c = (uint32_t)(b - ix * gz) * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
+    if (c >= (uint32_t)d.ny * (uint32_t)d.nz) return;
+    uint32_t iy, iz;
+    divmod_f(c, (uint32_t)d.nz, inv_nz, iy, iz);
+    step_cell<true, false, true, NT>(gin, gout, csum, nullptr, imb, d, k, ix, (int)iy, (int)iz);
+    return;
   }
   const int t = b / gz;
   const int ix = t / gy;
@@ -413,15 +448,6 @@ struct BatchWindow {
   long long ps;         // cells per direction plane (multiple of 64: 256-byte aligned planes)
 };
 
-// q = n / d, r = n % d for n < 2^24 from an FP32 reciprocal, corrected.
-__device__ __forceinline__ void divmod_f(uint32_t n, uint32_t d, float inv, uint32_t& q, uint32_t& r) {
-  q = (uint32_t)((float)n * inv);
-  int rr = (int)n - (int)(q * d);
-  while (rr < 0) { --q; rr += (int)d; }
-  while (rr >= (int)d) { ++q; rr -= (int)d; }
-  r = (uint32_t)rr;
-}
-
 // With LOADS, the item sequence also holds the load work of earlier steps
 // (fl: as for k_step_f32_loads, over the flattened cell index of all
 // windows): nchunk + nfinal extra items lead it, one in every fl.period.
@@ -490,12 +516,17 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
     lds = (kLdsPerCU - kLdsHeadroom) / (size_t)bpc;
     if (lds > 65536) lds = 65536;
   }
+  const bool lin = (nt & PSC_STEP_LINEAR) != 0;
   const dim3 block(bz, by, 1);
   const int rows = (ny + cpt - 1) / cpt;
-  const dim3 grid((nz + bz - 1) / bz, (rows + by - 1) / by, nx);
+  const unsigned per_plane = (unsigned)(((size_t)ny * nz + (size_t)bz * by - 1) / ((size_t)bz * by));
+  const dim3 grid = lin ? dim3(per_plane, nx, 1) : dim3((nz + bz - 1) / bz, (rows + by - 1) / by, nx);
+  const float inv_nz = 1.0f / (float)nz;
   if (cpt != 1 && cpt != 2) return set_error(PSC_EINVAL, "psc_step(f32): cells per thread %d", cpt);
   if ((size_t)nx * ny * nz * sizeof(float) > 0xffffffffull)
     return set_error(PSC_EINVAL, "psc_step(f32): a direction plane must stay below 4 GiB (N < 2^30 cells)");
+  if (lin && (cpt != 1 || (size_t)ny * nz >= (1u << 24)))
+    return set_error(PSC_EINVAL, "psc_step(f32): the flat mapping needs one cell per thread and ny*nz < 2^24");
   StepConsts k;
   k.field = s == 0;
   k.inv_s3 = k.field ? 0.0f : (float)(1.0 / (double)(s * s * s));
@@ -512,10 +543,15 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
   const bool ntb = (nt & 1) != 0;
 #define PSC_F32_CASE(P, M, I, T)                                                                  \
   if (pull == P && moving == M && wimb == I && ntb == T) {                                       \
-    if (cpt == 2)                                                                                \
-      k_step_f32<P, M, I, T, 2><<<grid, block, lds, stream>>>(in, out, csum, usn, imb, d, k);    \
+    if (lin)                                                                                     \
+      k_step_f32<P, M, I, T, 1, true><<<grid, block, lds, stream>>>(in, out, csum, usn, imb, d, k, \
+                                                                    inv_nz);                     \
+    else if (cpt == 2)                                                                           \
+      k_step_f32<P, M, I, T, 2, false><<<grid, block, lds, stream>>>(in, out, csum, usn, imb, d, k, \
+                                                                     inv_nz);                    \
     else                                                                                         \
-      k_step_f32<P, M, I, T, 1><<<grid, block, lds, stream>>>(in, out, csum, usn, imb, d, k);    \
+      k_step_f32<P, M, I, T, 1, false><<<grid, block, lds, stream>>>(in, out, csum, usn, imb, d, k, \
+                                                                     inv_nz);                    \
     return check_launch("psc_step(f32)");                                                        \
   }
 #define PSC_F32_CASES(T)                 \
@@ -555,8 +591,13 @@ int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, flo
   if ((bz * by) % 64 != 0 || bz * by > 256) return set_error(PSC_ELAUNCH, "psc_step(f32+loads): block %dx%d", bz, by);
   if ((size_t)nx * ny * nz * sizeof(float) > 0xffffffffull)
     return set_error(PSC_EINVAL, "psc_step(f32+loads): a direction plane must stay below 4 GiB (N < 2^30 cells)");
+  const bool lin = (nt & PSC_STEP_LINEAR) != 0;
+  if (lin && (cpt != 1 || (size_t)ny * nz >= (1u << 24)))
+    return set_error(PSC_EINVAL, "psc_step(f32+loads): the flat mapping needs one cell per thread and ny*nz < 2^24");
   const int rows = (ny + cpt - 1) / cpt;
-  const int gz = (nz + bz - 1) / bz, gy = (rows + by - 1) / by;
+  const int gz = lin ? (int)(((size_t)ny * nz + (size_t)bz * by - 1) / ((size_t)bz * by)) : (nz + bz - 1) / bz;
+  const int gy = lin ? 1 : (rows + by - 1) / by;
+  const float inv_nz = 1.0f / (float)nz;
   const long long ncell = (long long)gz * gy * nx;
   FusedLoads fl{la, imb_prev, part_out, part_prev, loads, 0, 0, 1};
   if (imb_prev != nullptr) {
@@ -589,10 +630,14 @@ int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, flo
   const dim3 block(bz, by, 1);
   const bool ntb = (nt & 1) != 0;
   const unsigned g = (unsigned)grid;
-  if (ntb && cpt == 2) k_step_f32_loads<true, 2><<<g, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
-  else if (ntb) k_step_f32_loads<true, 1><<<g, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
-  else if (cpt == 2) k_step_f32_loads<false, 2><<<g, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
-  else k_step_f32_loads<false, 1><<<g, block, lds, stream>>>(in, out, csum, imb, d, k, fl, gz, gy);
+#define PSC_F32L_ARGS in, out, csum, imb, d, k, fl, gz, gy, inv_nz
+  if (ntb && lin) k_step_f32_loads<true, 1, true><<<g, block, lds, stream>>>(PSC_F32L_ARGS);
+  else if (lin) k_step_f32_loads<false, 1, true><<<g, block, lds, stream>>>(PSC_F32L_ARGS);
+  else if (ntb && cpt == 2) k_step_f32_loads<true, 2, false><<<g, block, lds, stream>>>(PSC_F32L_ARGS);
+  else if (ntb) k_step_f32_loads<true, 1, false><<<g, block, lds, stream>>>(PSC_F32L_ARGS);
+  else if (cpt == 2) k_step_f32_loads<false, 2, false><<<g, block, lds, stream>>>(PSC_F32L_ARGS);
+  else k_step_f32_loads<false, 1, false><<<g, block, lds, stream>>>(PSC_F32L_ARGS);
+#undef PSC_F32L_ARGS
   return check_launch("psc_step(f32+loads)");
 }
 

@@ -396,19 +396,7 @@ class PscD3Q19:
         if self._autotune:
             self._tune()
         import ctypes as ct
-        # FP32 pull lattices with stationary particles below FUSE_MAX_CELLS reduce the
-        # loads inside the next steps' kernels (measured, us/step, fused vs side stream:
-        # 64^3 13.0 vs 23.3, 96^3 30.8 vs 35.9, 128^3 62.9 vs 70.7, 256^3 517 vs 510).
-        # Otherwise the loads of step t overlap step t+1 on a side stream when the step
-        # is long enough to hide them (FP64, or FP32 from ~80^3 up); on small lattices
-        # the fork/join costs more than it hides and the loads run in line.
-        has_loads = self._tables is not None and self._tables.particles.shape[0] > 0
-        fusable = (has_loads and self._storage == _lib.PSC_F32_DEV and self._layout == _lib.PSC_PULL
-                   and not self._moving)
-        fused = fusable and (self._N < FUSE_MAX_CELLS if self.fuse_loads is None else bool(self.fuse_loads))
-        auto = self._storage == _lib.PSC_F64_PLAIN or self._N >= PIPELINE_MIN_CELLS
-        pipelined = (has_loads and not fused
-                     and (auto if self.pipeline_loads is None else bool(self.pipeline_loads)))
+        fused, pipelined = self._load_modes()
         self.loads_mode = "fused" if fused else ("pipelined" if pipelined else "in line")
         if pipelined or fused:
             self._ensure_pipeline_buffers()
@@ -422,6 +410,24 @@ class PscD3Q19:
         self._imb = self._imb_bufs[1 - parity.value] if (pipelined or fused) else self._imb_bufs[0]
         self._step_count += steps
         self._cache.clear()
+
+    def _load_modes(self):
+        """(fused, pipelined): how run() reduces the per-particle loads.
+
+        FP32 pull lattices with stationary particles below FUSE_MAX_CELLS reduce the
+        loads inside the next steps' kernels (measured, us/step, fused vs side stream:
+        64^3 13.0 vs 23.3, 96^3 30.8 vs 35.9, 128^3 62.9 vs 70.7, 256^3 517 vs 510).
+        Otherwise the loads of step t overlap step t+1 on a side stream when the step
+        is long enough to hide them (FP64, or FP32 from ~80^3 up); on small lattices
+        the fork/join costs more than it hides and the loads run in line."""
+        has_loads = self._tables is not None and self._tables.particles.shape[0] > 0
+        fusable = (has_loads and self._storage == _lib.PSC_F32_DEV and self._layout == _lib.PSC_PULL
+                   and not self._moving)
+        fused = fusable and (self._N < FUSE_MAX_CELLS if self.fuse_loads is None else bool(self.fuse_loads))
+        auto = self._storage == _lib.PSC_F64_PLAIN or self._N >= PIPELINE_MIN_CELLS
+        pipelined = (has_loads and not fused
+                     and (auto if self.pipeline_loads is None else bool(self.pipeline_loads)))
+        return fused, pipelined
 
     def _ensure_pipeline_buffers(self):
         torch = _torch()
@@ -462,10 +468,16 @@ class PscD3Q19:
         return p
 
     def _tune(self):
-        """Pick the psc_step launch configuration once (autotune.py)."""
+        """Pick the psc_step launch configuration once (autotune.py).  When run()
+        will fuse the loads into the step kernels (static geometry built), the
+        candidates are timed in that fused loop, whose best launch differs from
+        the plain step's."""
         if self._autotune and self._storage == _lib.PSC_F32_DEV:
             from . import autotune
-            bz, by, flags = autotune.tune(self)
+            fused = self.static_geometry and self._geometry_built and self._load_modes()[0]
+            if fused:
+                self._ensure_pipeline_buffers()
+            bz, by, flags = autotune.tune(self, fused=fused)
             self._block = (bz, by)
             self._flags = flags
         self._autotune = False

@@ -169,21 +169,24 @@ def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, bl
     assert a._step_count == b._step_count == 72
 
 
-@pytest.mark.parametrize("block,flags,period", [((128, 1), 0x10601, 1), ((192, 1), 0x10601, 5),
-                                                ((64, 4), 0x20001, 3), ((96, 2), 0x00301, 2),
-                                                ((128, 1), 0x10601, 1001)])
-def test_fused_loads_match_separate_reduction(psc3d, block, flags, period):
-    """A 64^3 lattice with 40 spheres (overlapping; 256-cell chunks, so
+@pytest.mark.parametrize("n,block,flags,period", [(64, (128, 1), 0x10601, 1), (64, (192, 1), 0x10601, 5),
+                                                  (64, (64, 4), 0x20001, 3), (64, (96, 2), 0x00301, 2),
+                                                  (64, (128, 1), 0x10601, 1001), (64, (256, 1), 0x100601, 5),
+                                                  (52, (128, 1), 0x100301, 3), (52, (64, 1), 0x100000, 1),
+                                                  (52, (64, 2), 0x10601, 5)])
+def test_fused_loads_match_separate_reduction(psc3d, n, block, flags, period):
+    """An n^3 lattice with 40 spheres (overlapping; 256-cell chunks, so
     hundreds of chunks and several per particle): the step kernels that also
     reduce earlier steps' loads (with the tuner's flag forms: non-temporal,
-    occupancy cap, two cells per thread; load workgroups all ahead of the
-    cells, interleaved, or spread over more than the grid) give the state, imb
-    and loads of separate launches, bit for bit."""
+    occupancy cap, two cells per thread, flat cell mapping, also on rows of 52
+    cells; load workgroups all ahead of the cells, interleaved, or spread over
+    more than the grid) give the state, imb and loads of separate launches,
+    bit for bit."""
     rng = np.random.default_rng(7)
-    parts = [psc3d.Particle3D(*rng.uniform(0, 64, 3), rng.uniform(3.0, 7.5)) for _ in range(40)]
+    parts = [psc3d.Particle3D(*rng.uniform(0, n, 3), rng.uniform(3.0, 7.5)) for _ in range(40)]
 
     def make():
-        s = psc3d.SparsePscD3Q19(64, [psc3d.Particle3D(p.x, p.y, p.z, p.r) for p in parts], tau=0.8, subsamples=3,
+        s = psc3d.SparsePscD3Q19(n, [psc3d.Particle3D(p.x, p.y, p.z, p.r) for p in parts], tau=0.8, subsamples=3,
                                  body_force=(1e-5, 0.0, 2e-5), static_geometry=True, precision="fp32",
                                  device="cuda:0", block=block, chunk_cells=256)
         s._flags = flags
@@ -191,7 +194,7 @@ def test_fused_loads_match_separate_reduction(psc3d, block, flags, period):
 
     a, b = make(), make()
     b.fuse_loads, b.fuse_period = True, period
-    print(f"[fused] block {block} flags {flags:#x} period {period}", flush=True)
+    print(f"[fused] n {n} block {block} flags {flags:#x} period {period}", flush=True)
     for chunk in (1, 3, 24, 9):
         for _ in range(chunk):
             a.collide_stream_psc()
@@ -202,6 +205,43 @@ def test_fused_loads_match_separate_reduction(psc3d, block, flags, period):
         assert np.array_equal(np.stack([a.imb_mx, a.imb_my, a.imb_mz]), np.stack([b.imb_mx, b.imb_my, b.imb_mz]))
     assert np.abs(a.hydro_forces).max() > 0
     assert int(a._tables.chunks.shape[0]) > 100 and int(np.diff(a._tables.chunk_first).max()) > 4
+
+
+def test_fused_tuning_leaves_the_solver_unchanged(psc3d, tmp_path, monkeypatch):
+    """Timing the launch candidates in the fused native loop steps scratch
+    copies: the solver's state and loads are untouched, the tuned run gives
+    the bits of a fixed-launch run, and the choice is cached under its own key."""
+    import torch
+    monkeypatch.setenv("PSC_HIP_TUNE_CACHE", str(tmp_path / "tune.json"))
+    at = load_package("autotune")
+    at._MEM.clear()
+    rng = np.random.default_rng(11)
+    parts = [(*rng.uniform(0, 40, 3), rng.uniform(3.0, 6.0)) for _ in range(12)]
+
+    def make(block):
+        return psc3d.SparsePscD3Q19(40, [psc3d.Particle3D(*p) for p in parts], tau=0.8, subsamples=3,
+                                    body_force=(1e-5, 0.0, 0.0), static_geometry=True, precision="fp32",
+                                    device="cuda:0", block=block)
+
+    a, b = make((64, 4)), make("auto")
+    a.run(3)  # builds the geometry, stepping in Python
+    b.run(3)  # the same; b tunes in the fused loop on its first step
+    assert b._load_modes()[0] and not b._autotune
+    assert any(k.endswith("|fused") for k in at._MEM)
+    print(f"[tune] fused choice {b._block} {b._flags:#x}", flush=True)
+    at._MEM.clear()
+    (tmp_path / "tune.json").unlink()
+    b._autotune = True
+    before = (b._a.clone(), b._b.clone(), b._loads.clone())
+    b._tune()  # both caches emptied: times every candidate again
+    torch.cuda.synchronize()
+    assert torch.equal(before[0], b._a) and torch.equal(before[2], b._loads)
+    a.run(40)
+    b.run(40)
+    assert b.loads_mode == "fused"
+    assert np.array_equal(a.f, b.f)
+    assert np.array_equal(a.hydro_forces, b.hydro_forces)
+    assert np.array_equal(a.hydro_torques, b.hydro_torques)
 
 
 def test_every_launch_configuration_gives_identical_results(psc3d):
@@ -215,15 +255,16 @@ def test_every_launch_configuration_gives_identical_results(psc3d):
     ref = None
     nx, ny, nz = s._shape
     h = torch.cuda.current_stream().cuda_stream
-    for bz, by, nt, bpc in at.candidates(nz) + [(32, 8, 1, 3), (16, 16, 0, 0)]:
-        for cpt in (1, 2):
+    flat = [(bz, 1, nt, 0, True) for bz in at.FLAT for nt in (0, 1)] + [(64, 4, 1, 3, True)]
+    for bz, by, nt, bpc, lin in at.candidates(nz) + [(32, 8, 1, 3, False), (16, 16, 0, 0, False)] + flat:
+        for cpt in ((1,) if lin else (1, 2)):
             lib.call("psc_step", s._a.data_ptr(), s._b.data_ptr(), s._csum.data_ptr(), None, s._imb.data_ptr(),
                      1, 1, nx, ny, nz, s._s, s.tau, 0.0, 0.0, float(g["force"][2]), bz, by,
-                     at.flags_of(nt, bpc, cpt), h)
+                     at.flags_of(nt, bpc, cpt, lin), h)
             out = s._b.cpu().numpy().copy()
             if ref is None:
                 ref = out
-            assert np.array_equal(out, ref), (bz, by, nt, bpc, cpt)
+            assert np.array_equal(out, ref), (bz, by, nt, bpc, cpt, lin)
 
 
 def _moving_table(rng, n, count, rmin, rmax, umax=2e-3):

@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="processes of the aggregate CPU baseline (capped at os.cpu_count(); 1: none)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--layout", default="pull", choices=["pull", "push"])
     ap.add_argument("--launch", default=None,
@@ -145,22 +147,60 @@ def copy_bandwidth(torch, lib, nbytes=2 << 30, repeats=20):
     return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline(table, n, steps, quiet):
-    """NumPy FP64 oracle (restatement of the reference) timed on one core
-    for ``steps`` static-geometry steps of an n^3 lattice."""
+def _oracle_solver(table, n):
     from oracle import psc_oracle as O
 
     parts = [O.Sphere(*row) for row in table]
-    s = O.OracleSolver(n, parts, tau=0.8, subsamples=4, body_force=(1.0e-6, 0.0, 0.0), static_geometry=True,
-                       layout="sparse")
+    return O.OracleSolver(n, parts, tau=0.8, subsamples=4, body_force=(1.0e-6, 0.0, 0.0), static_geometry=True,
+                          layout="sparse")
+
+
+def cpu_baseline(table, n, steps, quiet):
+    """NumPy FP64 oracle (restatement of the reference) timed on one core
+    for ``steps`` static-geometry steps of an n^3 lattice.  Returns (MLUPS,
+    seconds, voxelization seconds)."""
+    s = _oracle_solver(table, n)
+    t0 = time.perf_counter()
     s.rebuild_geometry()
+    tv = time.perf_counter() - t0
     s.collide_stream_psc()   # first step outside the timing (allocations settle)
     t0 = time.perf_counter()
     for _ in range(steps):
         s.collide_stream_psc()
     dt = time.perf_counter() - t0
-    log(f"cpu oracle: {steps} steps of {n}^3 in {dt:.2f}s", quiet)
-    return n ** 3 * steps / dt / 1e6, dt
+    log(f"cpu oracle: {steps} steps of {n}^3 in {dt:.2f}s (voxelization {tv:.2f}s)", quiet)
+    return n ** 3 * steps / dt / 1e6, dt, tv
+
+
+def _cpu_worker(table, n, steps, barrier, results):
+    """One process of the aggregate CPU baseline: an independent n^3 box."""
+    s = _oracle_solver(table, n)
+    s.rebuild_geometry()
+    s.collide_stream_psc()
+    barrier.wait()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.collide_stream_psc()
+    results.put(time.perf_counter() - t0)
+
+
+def cpu_baseline_aggregate(table, n, steps, procs, quiet):
+    """The reference's case-level parallelism: ``procs`` single-threaded
+    processes, each stepping its own n^3 box (fresh interpreters), started
+    together.  Aggregate MLUPS = procs * n^3 * steps / slowest process."""
+    import multiprocessing as mp
+
+    ctx = mp.get_context("spawn")
+    barrier, results = ctx.Barrier(procs), ctx.Queue()
+    ps = [ctx.Process(target=_cpu_worker, args=(table, n, steps, barrier, results)) for _ in range(procs)]
+    for p in ps:
+        p.start()
+    dts = [results.get(timeout=600) for _ in ps]
+    for p in ps:
+        p.join()
+    dt = max(dts)
+    log(f"cpu oracle aggregate: {procs} processes x {steps} steps of {n}^3, slowest {dt:.2f}s", quiet)
+    return procs * n ** 3 * steps / dt / 1e6, dt
 
 
 def imb_bytes(csum):
@@ -685,12 +725,19 @@ def main():
         cn = args.cpu_n
         count = max(1, int(round(table.shape[0] * (cn / n) ** 3)))
         ctable = table if cn == n else workloads.fixed_array(seed=0, n=cn, count=count)
-        cpu_mlups, cpu_dt = cpu_baseline(ctable, cn, args.cpu_steps, args.quiet)
+        cpu_mlups, cpu_dt, cpu_vox = cpu_baseline(ctable, cn, args.cpu_steps, args.quiet)
         cpu = {"value": cpu_mlups, "unit": "MLUPS", "cores": 1, "kind": "port",
                "sample": f"NumPy FP64 oracle (restatement of psc3d.py), {args.cpu_steps} static-geometry steps of a "
                          f"{cn}^3 periodic box with {count} RSA spheres r=12 (same volume fraction) in {cpu_dt:.1f} s, "
                          f"one thread; host {platform.processor() or platform.machine()}, "
-                         f"os.cpu_count()={os.cpu_count()}"}
+                         f"os.cpu_count()={os.cpu_count()}",
+               "voxelize_s": cpu_vox}
+        if args.cpu_procs > 1:
+            procs = min(args.cpu_procs, os.cpu_count() or 1)
+            agg, agg_dt = cpu_baseline_aggregate(ctable, cn, max(1, args.cpu_steps // 3), procs, args.quiet)
+            cpu["aggregate"] = {"value": agg, "unit": "MLUPS", "cores": procs,
+                                "sample": f"{procs} single-threaded processes, each {max(1, args.cpu_steps // 3)} "
+                                          f"steps of its own {cn}^3 box, slowest {agg_dt:.1f} s"}
 
     step_bytes = BYTES_F32_STATIC * N + ibytes
     if fused:

@@ -416,7 +416,7 @@ class PscD3Q19:
 
         FP32 pull lattices with stationary particles below FUSE_MAX_CELLS reduce the
         loads inside the next steps' kernels (measured, us/step, fused vs side stream:
-        64^3 13.0 vs 23.3, 96^3 30.8 vs 35.9, 128^3 62.9 vs 70.7, 256^3 517 vs 510).
+        64^3 11.7 vs 23.3, 96^3 25.5 vs 35.9, 128^3 63.7 vs 70.7, 256^3 517 vs 510).
         Otherwise the loads of step t overlap step t+1 on a side stream when the step
         is long enough to hide them (FP64, or FP32 from ~80^3 up); on small lattices
         the fork/join costs more than it hides and the loads run in line."""

@@ -59,6 +59,10 @@ extern "C" {
 #define PSC_STEP_CPT_MASK 0x30000   /* bits 16-17: cells per thread along y (1, 2)  */
 #define PSC_BATCH_WG512 0x40000     /* psc_step_batched: 512-thread workgroups      */
 #define PSC_BATCH_IMB_NT 0x80000    /* psc_step_batched: non-temporal imb stores    */
+#define PSC_RUN_INPLACE 0x200000    /* bit 21, PscStepPlan.flags of psc_run_steps   */
+                                    /* in the fused form only: steps go in pairs    */
+                                    /* updating one state buffer in place (AA       */
+                                    /* pattern), half the footprint, same bits      */
 #define PSC_STEP_LINEAR 0x100000    /* bit 20: FP32 steps; a workgroup takes        */
                                     /* block_z*block_y consecutive cells of an x    */
                                     /* plane's flat (y, z) index instead of a       */
@@ -318,7 +322,14 @@ typedef struct PscStepPlan {
  * workgroups (k = 1: all ahead of the cell workgroups).  The loads still
  * pending after the last step run after the loop.  Every step's loads are
  * computed, with the bits of the separate reduction, and d_loads ends with
- * the last step's.
+ * the last step's.  PSC_RUN_INPLACE in flags (fused form only) runs the steps
+ * in pairs on the state buffer of parity *parity, in place: a gather-scatter
+ * step that leaves the post-stream populations of direction i in plane
+ * opp(i), then a local step back to the pull form.  An odd last step is an
+ * ordinary step to the other buffer, so on return *parity and the state
+ * have their usual meaning; the bits equal those of ordinary steps.  At
+ * 128^3 one 160 MB state buffer stays resident in the 256 MB Infinity Cache
+ * where two do not.
  * use_graph = 1 captures `graph_steps` steps (even, >= 2) once per plan into
  * a hipGraph and replays it; the remainder runs as direct launches.  Graphs
  * are cached per plan contents until psc_graph_cache_clear(). */

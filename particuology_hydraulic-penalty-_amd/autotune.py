@@ -11,7 +11,9 @@ stream layout, moving, fused loads) in memory and in a JSON file
 ($PSC_HIP_TUNE_CACHE, default ~/.cache/psc_hip/step_tuning.json).  When the
 native loop fuses the particle loads into the step kernels, candidates are
 timed in that loop (on scratch copies of the buffers): the extra load
-workgroups shift the best occupancy cap.
+workgroups shift the best occupancy cap.  There each candidate is also timed
+with the steps in place in pairs (PSC_RUN_INPLACE, one state buffer): that
+pays where one state buffer fits the Infinity Cache and two do not.
 
 All candidates compute bit-identical results; only the launch differs.
 """
@@ -137,7 +139,7 @@ def time_fused_config(solver, scratch, bz, by, flags, repeats):
     p.state_a, p.state_b = a.data_ptr(), b.data_ptr()
     p.imb, p.imb_alt = i0.data_ptr(), i1.data_ptr()
     p.partials, p.partials_alt, p.loads = q0.data_ptr(), q1.data_ptr(), ld.data_ptr()
-    p.block_z, p.block_y, p.flags = bz, by, flags
+    p.block_z, p.block_y, p.flags = bz, by, flags  # flags may hold PSC_RUN_INPLACE
     stream = torch.cuda.current_stream(solver.device)
     h = stream.cuda_stream
     lib = _lib.load()
@@ -186,9 +188,10 @@ def tune(solver, budget_ms: float = 400.0, use_cache: bool = True, fused: bool =
         best = None
         for bz, by, nt, bpc, flat in cands:
             flags = flags_of(nt, bpc, 1, flat)
-            ms = timer(bz, by, flags)
-            if best is None or ms < best[0]:
-                best = (ms, bz, by, flags)
+            for f in ((flags, flags | _lib.PSC_RUN_INPLACE) if fused else (flags,)):
+                ms = timer(bz, by, f)
+                if best is None or ms < best[0]:
+                    best = (ms, bz, by, f)
         cfg = (best[1], best[2], best[3])
         _MEM[key] = cfg
         _store_file(key, list(cfg) + [best[0]])

@@ -57,7 +57,7 @@ int launch_loads_chunks(const LoadsArgs&, const void*, int, int, int, int, doubl
 int launch_loads_final(const int32_t*, int, const double*, double*, hipStream_t);
 int launch_step_f32_loads(const float*, float*, const uint32_t*, float*, const float*, const LoadsArgs&, double*,
                           const double*, double*, int, int, int, int, int, double, double, double, double, int, int,
-                          int, hipStream_t);
+                          int, hipStream_t, int);
 
 int launch_smooth_alpha(const double*, const int32_t*, const int32_t*, int, int, int, int, int, uint32_t*,
                         unsigned long long*, double*, float*, hipStream_t);
@@ -612,6 +612,11 @@ int side_resources(SideRes** out) {
 // into partials[q^1] and adds up partials[q] (the step before that) into
 // d_loads, so a step is one launch.  *pc / *pf say whether a previous step's
 // chunks / partials are pending; a sequence starts with neither.
+// With PSC_RUN_INPLACE in the plan's flags, steps go in pairs that update the
+// state buffer of parity *par in place (a gather-scatter step, then a local
+// step; see step_cell): the imb and partials buffers still alternate every
+// step, and after a pair both parities agree again.  An odd last step is an
+// ordinary pull step from one buffer to the other.
 int run_fused(const PscStepPlan& p, int n, int* par, bool* pc, bool* pf, hipStream_t s) {
   const LoadsArgs la{p.particles, p.boxes, p.chunks, p.chunk_first, p.nparticles, p.nchunks, p.chunk_cells,
                      p.ent_cell, p.ent_loc, p.ent_cnt, p.chunk_nnz, p.count_sum};
@@ -620,14 +625,19 @@ int run_fused(const PscStepPlan& p, int n, int* par, bool* pc, bool* pf, hipStre
   int bz = p.block_z, by = p.block_y;
   if (bz <= 0) bz = (p.nz >= 64 && !(p.nz % 64 != 0 && p.nz % 32 == 0)) ? 64 : 32;
   if (by <= 0) by = 256 / bz > 0 ? 256 / bz : 1;
+  const bool inplace = (p.flags & PSC_RUN_INPLACE) != 0;
+  const int flags = p.flags & ~PSC_RUN_INPLACE;
   for (int k = 0; k < n; ++k) {
     const int q = *par;
-    const int rc = launch_step_f32_loads((const float*)(q ? p.state_b : p.state_a),
-                                         (float*)(q ? p.state_a : p.state_b), p.count_sum, (float*)imbs[q],
+    // in-place pair: steps k and k+1 (k even within this call, k+1 < n)
+    const int aa = !inplace ? 0 : ((k & 1) ? 2 : (k + 1 < n ? 1 : 0));
+    const int sb = aa == 2 ? q ^ 1 : q;  // the pair's state buffer (the parity at its start)
+    float* st_in = (float*)(sb ? p.state_b : p.state_a);
+    float* st_out = aa ? st_in : (float*)(sb ? p.state_a : p.state_b);
+    const int rc = launch_step_f32_loads(st_in, st_out, p.count_sum, (float*)imbs[q],
                                          *pc ? (const float*)imbs[q ^ 1] : nullptr, la, prt[q ^ 1],
                                          *pf ? prt[q] : nullptr, p.loads, p.fuse_loads, p.nx, p.ny, p.nz,
-                                         p.subsamples, p.tau,
-                                         p.fx, p.fy, p.fz, bz, by, p.flags, s);
+                                         p.subsamples, p.tau, p.fx, p.fy, p.fz, bz, by, flags, s, aa);
     if (rc) return rc;
     *pf = *pc;
     *pc = true;
@@ -756,13 +766,18 @@ extern "C" int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t us
   const PscStepPlan key = normalized(*plan);
   int par = *parity;
   int done = 0;
+  if ((key.flags & PSC_RUN_INPLACE) && !fused_plan(key))
+    return set_error(PSC_EINVAL, "psc_run_steps: in-place steps need the fused form (FP32 pull, no solid velocity, "
+                     "fuse_loads > 0, particles, alternate imb and partials)");
   if (fused_plan(key)) {
     // two direct steps (after them both loads stages are pending), replays
     // of a graph of fused steps, the remainder, then the pending loads
     int bz = key.block_z, by = key.block_y;
     int rc = step_args(key.state_a, key.state_b, key.count_sum, key.us_num, key.layout, key.nx, key.ny, key.nz,
-                       key.subsamples, key.tau, key.flags, &bz, &by);
+                       key.subsamples, key.tau, key.flags & ~PSC_RUN_INPLACE, &bz, &by);
     if (rc) return rc;
+    if ((key.flags & PSC_RUN_INPLACE) && (key.flags & PSC_STEP_CPT_MASK) > (1 << 16))
+      return set_error(PSC_EINVAL, "psc_run_steps: in-place steps take one cell per thread");
     if (key.imb == key.imb_alt || key.partials == key.partials_alt)
       return set_error(PSC_EINVAL, "psc_run_steps: imb/partials buffers alias");
     bool pc = false, pf = false;

@@ -181,8 +181,20 @@ __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, 
 // One cell: gather, collide, store.  imb is interleaved: the three
 // components of a cell are consecutive floats (imb[3 c + axis]), so a cell's
 // imb is one 12-byte store here and one 12-byte read in the load reduction.
-template <bool PULL, bool MOVING, bool IMB, bool NT, bool IMBNT = false>
-__device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* __restrict__ gout,
+//
+// AA (in place, one state buffer; gin == gout), for a pair of steps that
+// starts and ends in the pull form S[i][x] = post-collision g_i(x):
+//   AA = 1 (gather-scatter): read S[i][x - c_i] (the pull gather), collide,
+//          write g_i(x) to S[opp(i)][x + c_i], the location this thread read
+//          as S[opp(i)][x - c_opp(i)].  Afterwards S[opp(i)][y] holds the
+//          post-stream population f_i(y).
+//   AA = 2 (local): read S[opp(i)][x] = f_i(x), collide, write S[i][x]: the
+//          pull form again.
+// Every location is read and written by one thread only, so the in-place
+// update has no races, and the collision sees the inputs of the pull step
+// in the same order: results are bit-identical to two pull steps.
+template <bool PULL, bool MOVING, bool IMB, bool NT, bool IMBNT = false, int AA = 0>
+__device__ __forceinline__ void step_cell(const float* gin, float* gout,
                                           const uint32_t* __restrict__ csum, const float* __restrict__ usn,
                                           float* __restrict__ imb, const Dims d, const StepConsts& k, int ix,
                                           int iy, int iz) {
@@ -201,8 +213,10 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
 
   float g[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i)
-    g[i] = ld<NT>(gi + (size_t)i * plane, PULL ? off(-cx_of(i), -cy_of(i), -cz_of(i)) : here);
+  for (int i = 0; i < Q; ++i) {
+    if (AA == 2) g[i] = ld<NT>(gi + (size_t)opp_of(i) * plane, here);
+    else g[i] = ld<NT>(gi + (size_t)i * plane, PULL ? off(-cx_of(i), -cy_of(i), -cz_of(i)) : here);
+  }
 
   const uint32_t cnt = csum[cell_id(d, ix, iy, iz)];
   // destination offset of direction i
@@ -229,7 +243,11 @@ __device__ __forceinline__ void step_cell(const float* __restrict__ gin, float* 
     }
   }
 #pragma unroll
-  for (int i = 0; i < Q; ++i) st<NT>(go + (size_t)i * plane, dst(i), gn[i]);
+  for (int i = 0; i < Q; ++i) {
+    if (AA == 1) st<NT>(go + (size_t)opp_of(i) * plane, off(cx_of(i), cy_of(i), cz_of(i)), gn[i]);
+    else if (AA == 2) st<NT>(go + (size_t)i * plane, here, gn[i]);
+    else st<NT>(go + (size_t)i * plane, dst(i), gn[i]);
+  }
 }
 
 // q = n / d, r = n % d for n < 2^24 from an FP32 reciprocal, corrected.
@@ -347,6 +365,55 @@ __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict_
     if (iy < d.ny)
       step_cell<true, false, true, NT>(gin, gout, csum, nullptr, imb, d, k, ix, iy, iz);
   }
+}
+
+// In-place (AA) form of k_step_f32_loads: one state buffer g, AA = 1 or 2 as
+// in step_cell, one cell per thread, the same load workgroups.  g carries no
+// __restrict__: it is read and written.
+template <bool NT, bool LIN, int AA>
+__global__ __launch_bounds__(256) void k_step_f32_aa(float* g, const uint32_t* __restrict__ csum,
+                                                     float* __restrict__ imb, Dims d, StepConsts k, FusedLoads fl,
+                                                     int gz, int gy, float inv_nz) {
+  int b = blockIdx.x;
+  const int nextra = fl.nchunk + fl.nfinal;
+  if (nextra > 0) {
+    if (b < nextra * fl.period) {
+      const int e = b / fl.period, r = b - e * fl.period;
+      if (r == fl.period - 1) {
+        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+        if (e < fl.nchunk) {
+          __shared__ double lds[6 * 4];
+          loads_chunk_body<float>(fl.a, fl.imb_prev, d.nx, d.ny, d.nz, e, tid, blockDim.x * blockDim.y,
+                                     fl.part_out, lds);
+        } else {
+          loads_final_body(fl.a.chunk_first, e - fl.nchunk, tid, fl.part_prev, fl.loads);
+        }
+        return;
+      }
+      b = e * (fl.period - 1) + r;
+    } else {
+      b -= nextra;
+    }
+  }
+  int ix, iy, iz;
+  if (LIN) {
+    ix = b / gz;
+    if (ix >= d.nx) return;
+    const uint32_t c = (uint32_t)(b - ix * gz) * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
+    if (c >= (uint32_t)d.ny * (uint32_t)d.nz) return;
+    uint32_t qy, qz;
+    divmod_f(c, (uint32_t)d.nz, inv_nz, qy, qz);
+    iy = (int)qy;
+    iz = (int)qz;
+  } else {
+    const int t = b / gz;
+    ix = t / gy;
+    if (ix >= d.nx) return;
+    iz = (b - t * gz) * blockDim.x + threadIdx.x;
+    iy = (t - ix * gy) * blockDim.y + threadIdx.y;
+    if (iz >= d.nz || iy >= d.ny) return;
+  }
+  step_cell<true, false, true, NT, false, AA>(g, g, csum, nullptr, imb, d, k, ix, iy, iz);
 }
 
 // ---------------------------------------------------------------------------
@@ -574,11 +641,12 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
 // the chunks of imb_prev (the previous step's imb) into part_out; with
 // part_prev set, it also adds up part_prev's per-particle partials into loads.
 // period: the extra workgroups lead, one in every `period` workgroups.  Same
-// flags as launch_step_f32.
+// flags as launch_step_f32.  aa: 0 = pull step from in to out; 1 or 2 = the
+// gather-scatter or local step of an in-place pair (step_cell), in == out.
 int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, float* imb, const float* imb_prev,
                           const LoadsArgs& la, double* part_out, const double* part_prev, double* loads, int period,
                           int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz, int bz, int by,
-                          int nt, hipStream_t stream) {
+                          int nt, hipStream_t stream, int aa) {
   const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
   const int cpt = (nt >> 16) & 3 ? (nt >> 16) & 3 : 1;
   const int bpc = (nt >> 8) & 0xff;
@@ -588,6 +656,9 @@ int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, flo
     if (lds > 65536) lds = 65536;
   }
   if (cpt != 1 && cpt != 2) return set_error(PSC_EINVAL, "psc_step(f32+loads): cells per thread %d", cpt);
+  if (aa < 0 || aa > 2 || (aa > 0 && (cpt != 1 || in != out)))
+    return set_error(PSC_EINVAL, "psc_step(f32+loads): in-place form %d needs one buffer and one cell per thread", aa);
+  if (aa == 0 && in == out) return set_error(PSC_EINVAL, "psc_step(f32+loads): input and output alias");
   if ((bz * by) % 64 != 0 || bz * by > 256) return set_error(PSC_ELAUNCH, "psc_step(f32+loads): block %dx%d", bz, by);
   if ((size_t)nx * ny * nz * sizeof(float) > 0xffffffffull)
     return set_error(PSC_EINVAL, "psc_step(f32+loads): a direction plane must stay below 4 GiB (N < 2^30 cells)");
@@ -630,6 +701,18 @@ int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, flo
   const dim3 block(bz, by, 1);
   const bool ntb = (nt & 1) != 0;
   const unsigned g = (unsigned)grid;
+  if (aa > 0) {
+#define PSC_F32AA_CASE(T, L, A)                                                                        \
+    if (ntb == T && lin == L && aa == A) {                                                             \
+      k_step_f32_aa<T, L, A><<<g, block, lds, stream>>>(out, csum, imb, d, k, fl, gz, gy, inv_nz);     \
+      return check_launch("psc_step(f32+loads, in place)");                                            \
+    }
+    PSC_F32AA_CASE(true, false, 1) PSC_F32AA_CASE(true, false, 2) PSC_F32AA_CASE(false, false, 1)
+    PSC_F32AA_CASE(false, false, 2) PSC_F32AA_CASE(true, true, 1) PSC_F32AA_CASE(true, true, 2)
+    PSC_F32AA_CASE(false, true, 1) PSC_F32AA_CASE(false, true, 2)
+#undef PSC_F32AA_CASE
+    return set_error(PSC_EINVAL, "psc_step(f32+loads): no in-place kernel variant");
+  }
 #define PSC_F32L_ARGS in, out, csum, imb, d, k, fl, gz, gy, inv_nz
   if (ntb && lin) k_step_f32_loads<true, 1, true><<<g, block, lds, stream>>>(PSC_F32L_ARGS);
   else if (lin) k_step_f32_loads<false, 1, true><<<g, block, lds, stream>>>(PSC_F32L_ARGS);

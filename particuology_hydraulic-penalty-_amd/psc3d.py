@@ -187,6 +187,10 @@ class PscD3Q19:
         # grids, one in every fuse_period (odd: even periods put them on half the XCDs)
         self.fuse_loads = None
         self.fuse_period = FUSE_PERIOD
+        # fused run(): step pairs in place on one state buffer (AA pattern: half the
+        # footprint, the same bits); None = as the launch tuner measured it, else forced
+        self.inplace = None
+        self._inplace_tuned = False
         self.loads_mode = None  # how the last native run() reduced the loads
         # run() without fused loads: overlap the loads of step t with step t+1 on a side
         # stream (None = by lattice size and precision)
@@ -451,7 +455,8 @@ class PscD3Q19:
         p.tau = self.tau
         p.fx, p.fy, p.fz = self._force
         p.block_z, p.block_y = self._block
-        p.flags = self._flags
+        inplace = self._inplace_tuned if self.inplace is None else bool(self.inplace)
+        p.flags = self._flags | (_lib.PSC_RUN_INPLACE if fused and inplace else 0)
         tabs = self._tables
         if tabs is not None and tabs.particles.shape[0] > 0:
             parts, boxes, chunks, cfirst = self._dev_tables
@@ -479,7 +484,8 @@ class PscD3Q19:
                 self._ensure_pipeline_buffers()
             bz, by, flags = autotune.tune(self, fused=fused)
             self._block = (bz, by)
-            self._flags = flags
+            self._flags = flags & ~_lib.PSC_RUN_INPLACE
+            self._inplace_tuned = bool(flags & _lib.PSC_RUN_INPLACE)
         self._autotune = False
 
     def _advance(self):

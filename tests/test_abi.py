@@ -74,8 +74,8 @@ def test_argument_errors_are_reported(lib):
 
 
 def test_fused_load_entry_points_check_arguments(lib):
-    """psc_step_batched_loads and the fused form of psc_run_steps reject bad
-    arguments before touching a GPU."""
+    """psc_step_batched_loads and the fused form of psc_run_steps (also with
+    in-place step pairs) reject bad arguments before touching a GPU."""
     h = lib.load()
     t = lib.PscLoadsTables()
     p16, p32, p48, p64 = (ct.c_void_p(v) for v in (16, 32, 48, 64))
@@ -102,6 +102,14 @@ def test_fused_load_entry_points_check_arguments(lib):
     plan.imb_alt, plan.tau = 192, 0.5
     rc = h.psc_run_steps(ct.byref(plan), 4, 0, 16, ct.byref(par), None)
     assert rc == -1 and b"tau" in h.psc_last_error()
+    plan.tau, plan.flags = 0.8, lib.PSC_RUN_INPLACE | 0x20000  # in place with two cells per thread
+    rc = h.psc_run_steps(ct.byref(plan), 4, 0, 16, ct.byref(par), None)
+    assert rc == -1 and b"one cell per thread" in h.psc_last_error()
+    plan.flags, plan.fuse_loads = lib.PSC_RUN_INPLACE, 0  # in place outside the fused form
+    rc = h.psc_run_steps(ct.byref(plan), 4, 0, 16, ct.byref(par), None)
+    assert rc == -1 and b"fused form" in h.psc_last_error()
+    rc = h.psc_step(p16, p32, p64, None, None, 1, 1, 8, 8, 8, 2, 0.8, 0.0, 0.0, 0.0, 64, 1, lib.PSC_RUN_INPLACE, None)
+    assert rc == -1 and b"flags" in h.psc_last_error()  # a plan flag, not a psc_step flag
 
 
 def test_solver_refuses_to_run_without_gpu():

@@ -136,22 +136,25 @@ def test_voxelizer_counts_random(psc3d):
         assert np.array_equal(got, ref_cnt)
 
 
-@pytest.mark.parametrize("precision,block,fuse,pipeline", [
-    ("fp64", (64, 4), None, None), ("fp64", (64, 4), None, False),
-    ("fp32", (64, 4), None, None), ("fp32", (64, 2), None, None), ("fp32", (64, 1), None, None),
-    ("fp32", (96, 2), None, None), ("fp32", (64, 4), False, True), ("fp32", (64, 2), False, True),
-    ("fp32", (64, 4), False, False)])
-def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, block, fuse, pipeline):
+@pytest.mark.parametrize("precision,block,fuse,pipeline,inplace", [
+    ("fp64", (64, 4), None, None, None), ("fp64", (64, 4), None, False, None),
+    ("fp32", (64, 4), None, None, None), ("fp32", (64, 2), None, None, None), ("fp32", (64, 1), None, None, None),
+    ("fp32", (96, 2), None, None, None), ("fp32", (64, 4), False, True, None), ("fp32", (64, 2), False, True, None),
+    ("fp32", (64, 4), False, False, None), ("fp32", (64, 4), True, None, True), ("fp32", (64, 1), True, None, True)])
+def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, block, fuse, pipeline, inplace):
     """psc_run_steps (direct launches and hipGraph replays, with eager reads in
     between the replays; load reduction fused into the next step's kernel, in
-    line, or pipelined on the side stream, with 64/128/192/256-thread
-    workgroups) gives the same state and loads as stepping one call at a time."""
+    line, or pipelined on the side stream; fused steps also in place in pairs
+    on one state buffer, with odd counts ending in an ordinary step; with
+    64/128/192/256-thread workgroups) gives the same state and loads as
+    stepping one call at a time."""
     g = G.load("perm24")
     a, _ = _solver(psc3d, g, precision, block=block)
     b, _ = _solver(psc3d, g, precision, block=block)
     c, _ = _solver(psc3d, g, precision, block=block)
     b.pipeline_loads = c.pipeline_loads = pipeline
     b.fuse_loads = c.fuse_loads = fuse
+    b.inplace = c.inplace = inplace
     for chunk in (16, 16, 5, 32, 1, 2):
         for _ in range(chunk):
             a.collide_stream_psc()
@@ -169,19 +172,20 @@ def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, bl
     assert a._step_count == b._step_count == 72
 
 
-@pytest.mark.parametrize("n,block,flags,period", [(64, (128, 1), 0x10601, 1), (64, (192, 1), 0x10601, 5),
-                                                  (64, (64, 4), 0x20001, 3), (64, (96, 2), 0x00301, 2),
-                                                  (64, (128, 1), 0x10601, 1001), (64, (256, 1), 0x100601, 5),
-                                                  (52, (128, 1), 0x100301, 3), (52, (64, 1), 0x100000, 1),
-                                                  (52, (64, 2), 0x10601, 5)])
-def test_fused_loads_match_separate_reduction(psc3d, n, block, flags, period):
+@pytest.mark.parametrize("n,block,flags,period,inplace", [
+    (64, (128, 1), 0x10601, 1, False), (64, (192, 1), 0x10601, 5, False), (64, (64, 4), 0x20001, 3, False),
+    (64, (96, 2), 0x00301, 2, False), (64, (128, 1), 0x10601, 1001, False), (64, (256, 1), 0x100601, 5, False),
+    (52, (128, 1), 0x100301, 3, False), (52, (64, 1), 0x100000, 1, False), (52, (64, 2), 0x10601, 5, False),
+    (64, (128, 1), 0x10601, 5, True), (64, (64, 4), 0x00000, 1, True), (52, (256, 1), 0x100601, 5, True),
+    (52, (64, 2), 0x10001, 3, True)])
+def test_fused_loads_match_separate_reduction(psc3d, n, block, flags, period, inplace):
     """An n^3 lattice with 40 spheres (overlapping; 256-cell chunks, so
     hundreds of chunks and several per particle): the step kernels that also
     reduce earlier steps' loads (with the tuner's flag forms: non-temporal,
     occupancy cap, two cells per thread, flat cell mapping, also on rows of 52
     cells; load workgroups all ahead of the cells, interleaved, or spread over
-    more than the grid) give the state, imb and loads of separate launches,
-    bit for bit."""
+    more than the grid; steps in place in pairs) give the state, imb and loads
+    of separate launches, bit for bit."""
     rng = np.random.default_rng(7)
     parts = [psc3d.Particle3D(*rng.uniform(0, n, 3), rng.uniform(3.0, 7.5)) for _ in range(40)]
 
@@ -193,8 +197,8 @@ def test_fused_loads_match_separate_reduction(psc3d, n, block, flags, period):
         return s
 
     a, b = make(), make()
-    b.fuse_loads, b.fuse_period = True, period
-    print(f"[fused] n {n} block {block} flags {flags:#x} period {period}", flush=True)
+    b.fuse_loads, b.fuse_period, b.inplace = True, period, inplace
+    print(f"[fused] n {n} block {block} flags {flags:#x} period {period} inplace {inplace}", flush=True)
     for chunk in (1, 3, 24, 9):
         for _ in range(chunk):
             a.collide_stream_psc()

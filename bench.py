@@ -54,7 +54,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--layout", default="pull", choices=["pull", "push"])
     ap.add_argument("--launch", default=None,
-                    help="fixed psc_step launch 'block_z,block_y,flags' instead of autotuning (for profiling)")
+                    help="fixed psc_step launch 'block_z,block_y,flags' instead of autotuning (for profiling); "
+                         "flags may add PSC_RUN_INPLACE (0x200000) for in-place step pairs")
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--workload", default="fixed_array",
                     choices=["fixed_array", "moving", "windows", "adjoint", "bounce_back", "coupling"],
@@ -70,6 +71,10 @@ def log(msg, quiet=False):
         print(msg, file=sys.stderr, flush=True)
 
 
+def lib_flag(name):
+    return getattr(importlib.import_module(PKG + "._lib"), name)
+
+
 def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=None):
     kw = {}
     if launch is not None:
@@ -78,7 +83,9 @@ def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=N
     sol = psc3d.PscD3Q19(n, table, tau=0.8, subsamples=4, body_force=(1.0e-6, 0.0, 0.0), static_geometry=True,
                          precision=precision, stream_layout=layout, device="cuda:0", **kw)
     if launch is not None:
-        sol._flags = launch[2]
+        # a fixed launch: flags as psc_step takes them, plus PSC_RUN_INPLACE for in-place step pairs
+        sol._flags = launch[2] & ~lib_flag("PSC_RUN_INPLACE")
+        sol.inplace = bool(launch[2] & lib_flag("PSC_RUN_INPLACE"))
     sol.rebuild_geometry()
     sol._tune()   # launch-shape autotuning happens before the warm-up
     sol.run(warmup)
@@ -692,15 +699,18 @@ def main():
                                              launch)
     porosity = 1.0 - float(sol32.solid.mean())
     fused = sol32.loads_mode == "fused"
+    inplace = fused and (sol32._inplace_tuned if sol32.inplace is None else bool(sol32.inplace))
     kern_ms = step_kernel_ms(sol32, args.kernel_repeats, torch, lib)
     lbytes, nnz = loads_bytes(sol32)
     ibytes, nsolid = imb_bytes(sol32._csum)
-    # fused: every timed step is one k_step_f32_loads launch, so the HIP-event time of the
+    # fused: every timed step is one k_step_f32_loads launch (in place: one k_step_f32_aa
+    # launch, the gather-scatter and local forms alternating), so the HIP-event time of the
     # timed region over the step count is that kernel's average launch duration (it also
     # holds the once-per-run pending loads and the k readout, ~0.1 ms in all)
     fused_ms = gpu_ms32 / args.steps if fused else None
     mlups = N * args.steps / wall32 / 1e6
-    log(f"fp32: {mlups:.0f} MLUPS (wall), loads {sol32.loads_mode}, step kernel alone {kern_ms*1e3:.1f} us, "
+    log(f"fp32: {mlups:.0f} MLUPS (wall), loads {sol32.loads_mode}{' (in-place step pairs)' if inplace else ''}, "
+        f"step kernel alone {kern_ms*1e3:.1f} us, "
         f"fused step kernel {fused_ms*1e3 if fused else float('nan'):.1f} us, k={k32:.9e}", args.quiet)
 
     fp64 = None
@@ -744,8 +754,13 @@ def main():
         # the dominant kernel of the timed step: collide+stream plus the loads of earlier steps
         launch_bytes = step_bytes + lbytes
         achieved = launch_bytes / (fused_ms * 1e-3) / 1e9
-        roof_kernel, roof_us, traffic = ("k_step_f32_loads (pull, static; step + loads of the two previous steps)",
-                                         fused_ms * 1e3, read_traffic("k_step_f32_loads_128_pmc.txt"))
+        if inplace:
+            roof_kernel, roof_us, traffic = ("k_step_f32_aa (static; step + loads of the two previous steps; "
+                                             "in-place pairs of a gather-scatter and a local step on one state "
+                                             "buffer)", fused_ms * 1e3, read_traffic("k_step_f32_aa_128_pmc.txt"))
+        else:
+            roof_kernel, roof_us, traffic = ("k_step_f32_loads (pull, static; step + loads of the two previous "
+                                             "steps)", fused_ms * 1e3, read_traffic("k_step_f32_loads_128_pmc.txt"))
     else:
         launch_bytes = step_bytes
         achieved = launch_bytes / (kern_ms * 1e-3) / 1e9
@@ -768,7 +783,7 @@ def main():
             "workload": f"fixed-array permeability: periodic {n}^3, 60 RSA spheres r=12 (seed 0), tau 0.8, "
                         "F=1e-6 along x, s=4, static geometry, FP32 deviation storage, "
                         f"{args.layout} streaming; step = collide+stream + particle loads "
-                        f"({sol32.loads_mode})",
+                        f"({sol32.loads_mode}{', in-place step pairs' if inplace else ''})",
             "lattice": [n, n, n],
             "particles": int(table.shape[0]),
             "porosity": porosity,
@@ -783,7 +798,7 @@ def main():
             "traffic": traffic,
             "kernel": roof_kernel,
             "launch": {"block": list(sol32._block), "flags": sol32._flags,
-                       "fuse_period": sol32.fuse_period if fused else None},
+                       "fuse_period": sol32.fuse_period if fused else None, "inplace_pairs": inplace},
             "bytes_per_launch": launch_bytes,
             "bytes_per_cell": BYTES_F32_STATIC,
             "solid_cells": nsolid,

@@ -104,19 +104,20 @@ def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=N
     return sol, k, wall, gpu_ms
 
 
-def step_kernel_ms(sol, repeats, torch, lib, moving=False):
+def step_kernel_ms(sol, repeats, torch, lib, moving=False, launch_cfg=None):
     """Average duration of the collide+stream kernel alone, HIP events on the
-    stream it is launched on."""
+    stream it is launched on; launch_cfg = (block_z, block_y, flags), default
+    the solver's own launch."""
     nx, ny, nz = sol._shape
     fx, fy, fz = sol.body_force
     stream = torch.cuda.current_stream()
     h = stream.cuda_stream
+    bz, by, flags = launch_cfg if launch_cfg is not None else (sol._block[0], sol._block[1], sol._flags)
 
     def launch(a, b):
         lib.call("psc_step", a.data_ptr(), b.data_ptr(), sol._csum.data_ptr(),
                  sol._us.data_ptr() if moving else None, sol._imb.data_ptr(),
-                 sol._storage, sol._layout, nx, ny, nz, sol._s, sol.tau, fx, fy, fz, sol._block[0], sol._block[1],
-                 sol._flags, h)
+                 sol._storage, sol._layout, nx, ny, nz, sol._s, sol.tau, fx, fy, fz, bz, by, flags, h)
 
     a, b = sol._a, sol._b
     for _ in range(20):
@@ -700,7 +701,12 @@ def main():
     porosity = 1.0 - float(sol32.solid.mean())
     fused = sol32.loads_mode == "fused"
     inplace = fused and (sol32._inplace_tuned if sol32.inplace is None else bool(sol32.inplace))
-    kern_ms = step_kernel_ms(sol32, args.kernel_repeats, torch, lib)
+    # the plain step kernel's own best launch (the timed loop's launch is tuned for the fused,
+    # possibly in-place, form); with --launch both use the given one
+    kern_launch = None
+    if launch is None:
+        kern_launch = importlib.import_module(PKG + ".autotune").tune(sol32)
+    kern_ms = step_kernel_ms(sol32, args.kernel_repeats, torch, lib, launch_cfg=kern_launch)
     lbytes, nnz = loads_bytes(sol32)
     ibytes, nsolid = imb_bytes(sol32._csum)
     # fused: every timed step is one k_step_f32_loads launch (in place: one k_step_f32_aa
@@ -807,6 +813,8 @@ def main():
             "loads_entries": nnz,
             "kernel_us": roof_us,
             "step_kernel_alone_us": kern_ms * 1e3,
+            "step_kernel_alone_launch": list(kern_launch) if kern_launch else [sol32._block[0], sol32._block[1],
+                                                                               sol32._flags],
             "step_kernel_alone_GBps": step_bytes / (kern_ms * 1e-3) / 1e9,
         },
         "cpu_baseline": cpu,

@@ -60,12 +60,17 @@ def _cache_path():
                           os.path.join(os.path.expanduser("~"), ".cache", "psc_hip", "step_tuning.json"))
 
 
+# Part of every cache key: entries written when the candidate set was smaller
+# (before the flat mapping and the in-place step pairs) are not reused.
+CANDIDATE_SET = 2
+
+
 def _key(solver):
     import torch
     name = torch.cuda.get_device_properties(solver.device).name
     arch = getattr(torch.cuda.get_device_properties(solver.device), "gcnArchName", "")
     return "|".join(str(v) for v in (name, arch, solver._shape, solver._storage, solver._layout,
-                                      bool(getattr(solver, "_moving", False))))
+                                      bool(getattr(solver, "_moving", False)), "c%d" % CANDIDATE_SET))
 
 
 def _load_file():

@@ -57,6 +57,8 @@ def parse():
                     help="fixed psc_step launch 'block_z,block_y,flags' instead of autotuning (for profiling); "
                          "flags may add PSC_RUN_INPLACE (0x200000) for in-place step pairs")
     ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--no-large-check", action="store_true",
+                    help="skip the 256^3 HBM-only rate of the plain step kernel")
     ap.add_argument("--workload", default="fixed_array",
                     choices=["fixed_array", "moving", "windows", "adjoint", "bounce_back", "coupling"],
                     help="fixed_array: 128^3 permeability (default); moving: 512^3 window, 2000 moving spheres; "
@@ -133,6 +135,29 @@ def step_kernel_ms(sol, repeats, torch, lib, moving=False, launch_cfg=None):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / repeats
+
+
+def hbm_only_check(psc3d, workloads, torch, lib, repeats, quiet, n=256):
+    """The plain FP32 step kernel (tuned launch) on an n^3 lattice of the same
+    workload (60 spheres per 128^3, the same volume fraction), whose 2.5 GB
+    two-buffer working set is ten times the 256 MB Infinity Cache: its rate is
+    an HBM rate."""
+    count = int(round(60 * (n / 128) ** 3))
+    sol = psc3d.PscD3Q19(n, workloads.fixed_array(seed=0, n=n, count=count), tau=0.8, subsamples=4,
+                         body_force=(1.0e-6, 0.0, 0.0), static_geometry=True, precision="fp32", device="cuda:0")
+    sol.rebuild_geometry()
+    cfg = importlib.import_module(PKG + ".autotune").tune(sol)
+    ms = step_kernel_ms(sol, max(20, repeats), torch, lib, launch_cfg=cfg)
+    ibytes, _ = imb_bytes(sol._csum)
+    nbytes = BYTES_F32_STATIC * n ** 3 + ibytes
+    gbps = nbytes / (ms * 1e-3) / 1e9
+    log(f"hbm-only check: {n}^3 step kernel {ms*1e3:.1f} us = {gbps:.0f} GB/s", quiet)
+    out = {"lattice": [n, n, n], "particles": count, "step_kernel_us": ms * 1e3, "bytes_per_launch": nbytes,
+           "achieved_GBps": gbps, "frac": gbps / HBM_PEAK_GBPS, "launch": list(cfg),
+           "note": "plain two-buffer step kernel on a 2.5 GB working set: an HBM-only rate"}
+    del sol
+    torch.cuda.empty_cache()
+    return out
 
 
 def copy_bandwidth(torch, lib, nbytes=2 << 30, repeats=20):
@@ -719,6 +744,10 @@ def main():
         f"step kernel alone {kern_ms*1e3:.1f} us, "
         f"fused step kernel {fused_ms*1e3 if fused else float('nan'):.1f} us, k={k32:.9e}", args.quiet)
 
+    hbm_check = None
+    if not args.no_large_check and n < 256 and launch is None:
+        hbm_check = hbm_only_check(psc3d, workloads, torch, lib, args.kernel_repeats // 4, args.quiet)
+
     fp64 = None
     if not args.no_fp64:
         sol64, k64, wall64, gpu_ms64 = timed_run(psc3d, table, n, "fp64", args.layout, args.steps, args.warmup,
@@ -818,6 +847,7 @@ def main():
             "step_kernel_alone_GBps": step_bytes / (kern_ms * 1e-3) / 1e9,
         },
         "cpu_baseline": cpu,
+        "hbm_only_check": hbm_check,
         "k_fp32": k32,
         "fp64": fp64,
         "k_rel_err_fp32_vs_fp64": (abs(k32 - fp64["k"]) / abs(fp64["k"])) if fp64 else None,

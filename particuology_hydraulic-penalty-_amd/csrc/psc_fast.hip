@@ -315,6 +315,34 @@ struct FusedLoads {
   int period;                // extra workgroups lead, one per period workgroups (>= 1)
 };
 
+// The load workgroups of a fused step launch: with nextra = nchunk + nfinal
+// of them, workgroup b < nextra * period with b % period == period - 1 runs
+// extra item b / period (a chunk pass on imb_prev, then per-particle final
+// passes) and returns true; every other b is mapped to its index among the
+// cell workgroups.
+__device__ __forceinline__ bool fused_loads_workgroup(const FusedLoads& fl, const Dims& d, int& b) {
+  const int nextra = fl.nchunk + fl.nfinal;
+  if (nextra == 0) return false;
+  if (b >= nextra * fl.period) {
+    b -= nextra;
+    return false;
+  }
+  const int e = b / fl.period, r = b - e * fl.period;
+  if (r != fl.period - 1) {
+    b = e * (fl.period - 1) + r;
+    return false;
+  }
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  if (e < fl.nchunk) {
+    __shared__ double lds[6 * 4];
+    loads_chunk_body<float>(fl.a, fl.imb_prev, d.nx, d.ny, d.nz, e, tid, blockDim.x * blockDim.y, fl.part_out,
+                               lds);
+  } else {
+    loads_final_body(fl.a.chunk_first, e - fl.nchunk, tid, fl.part_prev, fl.loads);
+  }
+  return true;
+}
+
 // LIN: flat mapping as in k_step_f32; gz is then the number of workgroups per
 // x plane and gy is 1.
 template <bool NT, int CPT, bool LIN>
@@ -323,26 +351,7 @@ __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict_
                                                         Dims d, StepConsts k, FusedLoads fl, int gz, int gy,
                                                         float inv_nz) {
   int b = blockIdx.x;
-  const int nextra = fl.nchunk + fl.nfinal;
-  if (nextra > 0) {
-    if (b < nextra * fl.period) {
-      const int e = b / fl.period, r = b - e * fl.period;
-      if (r == fl.period - 1) {
-        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-        if (e < fl.nchunk) {
-          __shared__ double lds[6 * 4];
-          loads_chunk_body<float>(fl.a, fl.imb_prev, d.nx, d.ny, d.nz, e, tid, blockDim.x * blockDim.y,
-                                     fl.part_out, lds);
-        } else {
-          loads_final_body(fl.a.chunk_first, e - fl.nchunk, tid, fl.part_prev, fl.loads);
-        }
-        return;
-      }
-      b = e * (fl.period - 1) + r;
-    } else {
-      b -= nextra;
-    }
-  }
+  if (fused_loads_workgroup(fl, d, b)) return;
   if (LIN) {
     const int ix = b / gz;
     if (ix >= d.nx) return;
@@ -375,26 +384,7 @@ __global__ __launch_bounds__(256) void k_step_f32_aa(float* g, const uint32_t* _
                                                      float* __restrict__ imb, Dims d, StepConsts k, FusedLoads fl,
                                                      int gz, int gy, float inv_nz) {
   int b = blockIdx.x;
-  const int nextra = fl.nchunk + fl.nfinal;
-  if (nextra > 0) {
-    if (b < nextra * fl.period) {
-      const int e = b / fl.period, r = b - e * fl.period;
-      if (r == fl.period - 1) {
-        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-        if (e < fl.nchunk) {
-          __shared__ double lds[6 * 4];
-          loads_chunk_body<float>(fl.a, fl.imb_prev, d.nx, d.ny, d.nz, e, tid, blockDim.x * blockDim.y,
-                                     fl.part_out, lds);
-        } else {
-          loads_final_body(fl.a.chunk_first, e - fl.nchunk, tid, fl.part_prev, fl.loads);
-        }
-        return;
-      }
-      b = e * (fl.period - 1) + r;
-    } else {
-      b -= nextra;
-    }
-  }
+  if (fused_loads_workgroup(fl, d, b)) return;
   int ix, iy, iz;
   if (LIN) {
     ix = b / gz;

@@ -833,7 +833,8 @@ def main():
             "traffic": traffic,
             "kernel": roof_kernel,
             "launch": {"block": list(sol32._block), "flags": sol32._flags,
-                       "fuse_period": sol32.fuse_period if fused else None, "inplace_pairs": inplace},
+                       "fuse_period": sol32.effective_fuse_period() if fused else None,
+                       "inplace_pairs": inplace},
             "bytes_per_launch": launch_bytes,
             "bytes_per_cell": BYTES_F32_STATIC,
             "solid_cells": nsolid,

@@ -13,7 +13,9 @@ native loop fuses the particle loads into the step kernels, candidates are
 timed in that loop (on scratch copies of the buffers): the extra load
 workgroups shift the best occupancy cap.  There each candidate is also timed
 with the steps in place in pairs (PSC_RUN_INPLACE, one state buffer): that
-pays where one state buffer fits the Infinity Cache and two do not.
+pays where one state buffer fits the Infinity Cache and two do not.  For the
+chosen launch, tune_period then times the placement period of the load
+workgroups in that loop.
 
 All candidates compute bit-identical results; only the launch differs.
 """
@@ -133,7 +135,7 @@ def _fused_scratch(solver):
             torch.zeros_like(solver._partials), torch.zeros_like(solver._partials), torch.zeros_like(solver._loads))
 
 
-def time_fused_config(solver, scratch, bz, by, flags, repeats):
+def time_fused_config(solver, scratch, bz, by, flags, repeats, period=None):
     """Average step duration (ms) of `repeats` steps of the native loop
     (psc_run_steps, direct launches) with the loads fused into the step
     kernels, on the scratch buffers of _fused_scratch."""
@@ -145,6 +147,8 @@ def time_fused_config(solver, scratch, bz, by, flags, repeats):
     p.imb, p.imb_alt = i0.data_ptr(), i1.data_ptr()
     p.partials, p.partials_alt, p.loads = q0.data_ptr(), q1.data_ptr(), ld.data_ptr()
     p.block_z, p.block_y, p.flags = bz, by, flags  # flags may hold PSC_RUN_INPLACE
+    if period is not None:
+        p.fuse_loads = int(period)
     stream = torch.cuda.current_stream(solver.device)
     h = stream.cuda_stream
     lib = _lib.load()
@@ -158,6 +162,32 @@ def time_fused_config(solver, scratch, bz, by, flags, repeats):
     _lib.check(rc, "psc_run_steps")
     e1.synchronize()
     return e0.elapsed_time(e1) / repeats
+
+
+PERIODS = (1, 3, 5, 7, 9)
+
+
+def tune_period(solver, bz, by, flags, repeats: int = 40, use_cache: bool = True):
+    """Placement period of the load workgroups in the fused loop (odd: the
+    workgroups go to the 8 XCDs round-robin), timed with the chosen launch."""
+    key = _key(solver) + "|period|%d,%d,%d" % (bz, by, flags)
+    with _LOCK:
+        if use_cache:
+            if key in _MEM:
+                return _MEM[key]
+            disk = _load_file().get(key)
+            if disk:
+                _MEM[key] = int(disk[0])
+                return _MEM[key]
+        scratch = _fused_scratch(solver)
+        best = None
+        for period in PERIODS:
+            ms = time_fused_config(solver, scratch, bz, by, flags, repeats, period)
+            if best is None or ms < best[0]:
+                best = (ms, period)
+        _MEM[key] = best[1]
+        _store_file(key, [best[1], best[0]])
+        return best[1]
 
 
 def tune(solver, budget_ms: float = 400.0, use_cache: bool = True, fused: bool = False):

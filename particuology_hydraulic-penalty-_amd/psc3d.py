@@ -184,9 +184,11 @@ class PscD3Q19:
         # run(): reduce the loads of step t inside the kernels of steps t+1 and t+2 (None =
         # where the kernel supports it -- FP32 pull storage, stationary particles -- on
         # lattices below FUSE_MAX_CELLS); fuse_period: the load workgroups lead those
-        # grids, one in every fuse_period (odd: even periods put them on half the XCDs)
+        # grids, one in every fuse_period (None = as the launch tuner measured it, else
+        # FUSE_PERIOD; odd: even periods put them on half the XCDs)
         self.fuse_loads = None
-        self.fuse_period = FUSE_PERIOD
+        self.fuse_period = None
+        self._period_tuned = FUSE_PERIOD
         # fused run(): step pairs in place on one state buffer (AA pattern: half the
         # footprint, the same bits); None = as the launch tuner measured it, else forced
         self.inplace = None
@@ -415,6 +417,10 @@ class PscD3Q19:
         self._step_count += steps
         self._cache.clear()
 
+    def effective_fuse_period(self) -> int:
+        """Placement period of the load workgroups in fused steps."""
+        return max(1, int(self._period_tuned if self.fuse_period is None else self.fuse_period))
+
     def _load_modes(self):
         """(fused, pipelined): how run() reduces the per-particle loads.
 
@@ -469,7 +475,7 @@ class PscD3Q19:
             p.chunk_nnz, p.partials = self._ptr(self._chunk_nnz), self._ptr(self._partials)
             p.partials_alt = self._ptr(self._partials_alt) if (pipelined or fused) else None
             p.loads = self._ptr(self._loads)
-            p.fuse_loads = max(1, int(self.fuse_period)) if fused else 0
+            p.fuse_loads = self.effective_fuse_period() if fused else 0
         return p
 
     def _tune(self):
@@ -486,6 +492,8 @@ class PscD3Q19:
             self._block = (bz, by)
             self._flags = flags & ~_lib.PSC_RUN_INPLACE
             self._inplace_tuned = bool(flags & _lib.PSC_RUN_INPLACE)
+            if fused:
+                self._period_tuned = autotune.tune_period(self, bz, by, flags)
         self._autotune = False
 
     def _advance(self):

@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--launch", default=None,
                     help="fixed psc_step launch 'block_z,block_y,flags' instead of autotuning (for profiling); "
                          "flags may add PSC_RUN_INPLACE (0x200000) for in-place step pairs")
+    ap.add_argument("--fuse-period", type=int, default=None,
+                    help="placement period of the fused load workgroups (default: tuned; with --launch: 5)")
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--no-large-check", action="store_true",
                     help="skip the 256^3 HBM-only rate of the plain step kernel")
@@ -77,7 +79,7 @@ def lib_flag(name):
     return getattr(importlib.import_module(PKG + "._lib"), name)
 
 
-def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=None):
+def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=None, period=None):
     kw = {}
     if launch is not None:
         bz, by, flags = launch
@@ -88,6 +90,8 @@ def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=N
         # a fixed launch: flags as psc_step takes them, plus PSC_RUN_INPLACE for in-place step pairs
         sol._flags = launch[2] & ~lib_flag("PSC_RUN_INPLACE")
         sol.inplace = bool(launch[2] & lib_flag("PSC_RUN_INPLACE"))
+    if period is not None:
+        sol.fuse_period = period
     sol.rebuild_geometry()
     sol._tune()   # launch-shape autotuning happens before the warm-up
     sol.run(warmup)
@@ -722,7 +726,7 @@ def main():
     log(f"fp32 run: {args.warmup} warmup + {args.steps} timed steps at {n}^3", args.quiet)
     launch = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
     sol32, k32, wall32, gpu_ms32 = timed_run(psc3d, table, n, "fp32", args.layout, args.steps, args.warmup, torch,
-                                             launch)
+                                             launch, args.fuse_period)
     porosity = 1.0 - float(sol32.solid.mean())
     fused = sol32.loads_mode == "fused"
     inplace = fused and (sol32._inplace_tuned if sol32.inplace is None else bool(sol32.inplace))

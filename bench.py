@@ -435,7 +435,7 @@ def bench_windows(args, workloads, torch, np):
                      "step_kernel_alone_GBps": step_bytes / (kern * 1e-3) / 1e9,
                      "launch": {"flags": B._flags, "grid": B._grid, "items": B._mode,
                                 "workgroup": 512 if B._flags & 0x40000 else 256,
-                                "fuse_period": B.fuse_period if fused else None}},
+                                "fuse_period": B.effective_fuse_period() if fused else None}},
         "permeability": {v: [float(x) for x, vv in zip(k, variants) if vv == v][:4] for v in ("skeleton", "passive",
                                                                                             "preloaded")},
         "porosity_range": [float(por.min()), float(por.max())],

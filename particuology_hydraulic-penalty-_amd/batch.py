@@ -120,7 +120,9 @@ class BatchedPscD3Q19:
         # run(): reduce each step's loads inside the next two step launches (False: on a
         # side stream, concurrently with the next step)
         self.fuse_loads = True
-        self.fuse_period = FUSE_PERIOD
+        # placement period of the load items (None = as the launch tuner measured it)
+        self.fuse_period = None
+        self._period_tuned = FUSE_PERIOD
         self._steps = 0
         self._voxelize(tabs)
 
@@ -216,10 +218,15 @@ class BatchedPscD3Q19:
         if self._partials_alt is None:
             self._partials_alt = torch.zeros_like(self._partials)
 
-    def _launch_fused(self, a, b, q, pc, pf, stream, flags=None, mode=None):
+    def effective_fuse_period(self) -> int:
+        """Placement period of the load items in fused step launches."""
+        return max(1, int(self._period_tuned if self.fuse_period is None else self.fuse_period))
+
+    def _launch_fused(self, a, b, q, pc, pf, stream, flags=None, mode=None, period=None):
         """One step from a into b writing imb[q], plus the chunk sums of
         imb[1-q] (pc) and the per-particle sums of partials[q] (pf)."""
         flags = self._flags if flags is None else flags
+        period = self.effective_fuse_period() if period is None else period
         bs = 512 if (flags & _lib.PSC_BATCH_WG512) else 256
         windows, items, nitems = self._items_for(bs, mode or self._mode)
         imbs = (self._imb, self._imb_alt)
@@ -228,8 +235,7 @@ class BatchedPscD3Q19:
         _lib.call("psc_step_batched_loads", a.data_ptr(), b.data_ptr(), self._csum.data_ptr(), imbs[q].data_ptr(),
                   self.Ntot, windows.data_ptr(), items.data_ptr(), nitems, self._s, self.tau, flags,
                   self._grid, ct.byref(t), imbs[1 - q].data_ptr() if pc else None, prt[1 - q].data_ptr(),
-                  prt[q].data_ptr() if pf else None, self._loads.data_ptr(), max(1, int(self.fuse_period)),
-                  stream)
+                  prt[q].data_ptr() if pf else None, self._loads.data_ptr(), max(1, int(period)), stream)
 
     def step(self):
         """One step of every window (collide_stream_psc) plus the loads."""
@@ -281,6 +287,21 @@ class BatchedPscD3Q19:
                             if best is None or ms < best[0]:
                                 best = (ms, flags, mode)
         self._flags, self._grid, self._mode = best[1], 0, best[2]
+        if fused:  # then the placement period of the load items, for that launch
+            bestp = None
+            for period in (1, 3, 5, 7, 9):
+                self._launch_fused(self._a, self._b, 0, True, True, s, period=period)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(3):
+                    self._launch_fused(self._a, self._b, 0, True, True, s, period=period)
+                e1.record()
+                e1.synchronize()
+                ms = e0.elapsed_time(e1) / 3
+                if bestp is None or ms < bestp[0]:
+                    bestp = (ms, period)
+            self._period_tuned = bestp[1]
         self._tuned = True
 
     def run(self, steps: int, graph: bool = True, graph_steps: int = 8) -> None:
@@ -361,7 +382,7 @@ class BatchedPscD3Q19:
         direct(head)
         done += head
         if graph and steps - done >= G:
-            key = ("fused", q, G, self._flags, self._mode, self._grid, int(self.fuse_period))
+            key = ("fused", q, G, self._flags, self._mode, self._grid, self.effective_fuse_period())
             if getattr(self, "_fgraph", None) is None or self._fgraph[0] != key:
                 a, b = self._a, self._b
                 torch.cuda.synchronize(self.device)

@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--kernel-repeats", type=int, default=400)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-big-steps", type=int, default=8,
+                    help="timed oracle steps of the benchmarked lattice itself (the reported cpu_baseline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=16,
                     help="processes of the aggregate CPU baseline (capped at os.cpu_count(); 1: none)")
@@ -770,23 +772,35 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        # bounded sample: same sphere size and volume fraction on a cpu_n^3 box
+        host = f"host {platform.processor() or platform.machine()}, os.cpu_count()={os.cpu_count()}"
+        sizes = []
+        # the workload itself: cpu_big_steps timed steps (after one untimed) of the n^3 lattice
+        bs = max(1, args.cpu_big_steps)
+        big_mlups, big_dt, big_vox = cpu_baseline(table, n, bs, args.quiet)
+        big_dt /= bs
+        sizes.append({"n": n, "particles": int(table.shape[0]), "steps": bs, "s_per_step": big_dt,
+                      "MLUPS": big_mlups, "voxelize_s": big_vox})
+        # smaller boxes: same sphere size and volume fraction (cpu_n^3), and the 20^3 box of the
+        # moving-sphere benchmark with one centred r=2.5 sphere
         cn = args.cpu_n
         count = max(1, int(round(table.shape[0] * (cn / n) ** 3)))
-        ctable = table if cn == n else workloads.fixed_array(seed=0, n=cn, count=count)
-        cpu_mlups, cpu_dt, cpu_vox = cpu_baseline(ctable, cn, args.cpu_steps, args.quiet)
-        cpu = {"value": cpu_mlups, "unit": "MLUPS", "cores": 1, "kind": "port",
-               "sample": f"NumPy FP64 oracle (restatement of psc3d.py), {args.cpu_steps} static-geometry steps of a "
-                         f"{cn}^3 periodic box with {count} RSA spheres r=12 (same volume fraction) in {cpu_dt:.1f} s, "
-                         f"one thread; host {platform.processor() or platform.machine()}, "
-                         f"os.cpu_count()={os.cpu_count()}",
-               "voxelize_s": cpu_vox}
+        ctable = workloads.fixed_array(seed=0, n=cn, count=count)
+        for bn, btab in ((cn, ctable), (20, np.array([[10.0, 10.0, 10.0, 2.5, 0, 0, 0, 0, 0, 0]]))):
+            m, dt, vx = cpu_baseline(btab, bn, args.cpu_steps, args.quiet)
+            sizes.append({"n": bn, "particles": int(btab.shape[0]), "steps": args.cpu_steps,
+                          "s_per_step": dt / args.cpu_steps, "MLUPS": m, "voxelize_s": vx})
+        cpu = {"value": big_mlups, "unit": "MLUPS", "cores": 1, "kind": "port",
+               "sample": f"NumPy FP64 oracle (restatement of psc3d.py), {bs} timed static-geometry steps (after "
+                         f"one untimed) of this workload ({n}^3, {table.shape[0]} spheres), {big_dt:.2f} s per "
+                         f"step, one thread; {host}",
+               "voxelize_s": big_vox, "sizes": sizes,
+               "extrapolated_512_s_per_step": big_dt * (512 / n) ** 3}
         if args.cpu_procs > 1:
             procs = min(args.cpu_procs, os.cpu_count() or 1)
             agg, agg_dt = cpu_baseline_aggregate(ctable, cn, max(1, args.cpu_steps // 3), procs, args.quiet)
             cpu["aggregate"] = {"value": agg, "unit": "MLUPS", "cores": procs,
                                 "sample": f"{procs} single-threaded processes, each {max(1, args.cpu_steps // 3)} "
-                                          f"steps of its own {cn}^3 box, slowest {agg_dt:.1f} s"}
+                                          f"steps of its own {cn}^3 box ({count} spheres), slowest {agg_dt:.1f} s"}
 
     step_bytes = BYTES_F32_STATIC * N + ibytes
     if fused:

@@ -217,7 +217,8 @@ def test_inplace_pairs_on_non_cubic_lattices(psc3d, shape, block, flags):
     """In-place step pairs index x, y and z separately (gather-scatter to
     x + c_i, periodic wrap, a 3-cell axis where z - 1 and z + 2 coincide):
     on non-cubic lattices the fused in-place loop gives the state, imb and
-    loads of Python stepping, bit for bit."""
+    loads of Python stepping, bit for bit, also with single steps taken
+    between native runs."""
     rng = np.random.default_rng(3)
     nx, ny, nz = shape
     parts = [psc3d.Particle3D(rng.uniform(0, nx), rng.uniform(0, ny), rng.uniform(0, nz), rng.uniform(2.0, 5.0))
@@ -241,6 +242,9 @@ def test_inplace_pairs_on_non_cubic_lattices(psc3d, shape, block, flags):
         assert np.array_equal(a.hydro_forces, b.hydro_forces)
         assert np.array_equal(a.hydro_torques, b.hydro_torques)
         assert np.array_equal(np.stack([a.imb_mx, a.imb_my, a.imb_mz]), np.stack([b.imb_mx, b.imb_my, b.imb_mz]))
+        # a single step between native runs continues from the pull form the loop left
+        da, db = a.collide_stream_psc(), b.collide_stream_psc()
+        assert np.array_equal(a.f, b.f) and da["mean_uz"] == db["mean_uz"]
     assert b.loads_mode == "fused" and np.abs(a.hydro_forces).max() > 0
 
 

@@ -386,7 +386,7 @@ class PscD3Q19:
             d._fill()
         return d
 
-    def run(self, steps: int, graph: bool = True, graph_steps: int = 16) -> None:
+    def run(self, steps: int, graph: bool = True, graph_steps: int = 32) -> None:
         """``steps`` calls of collide_stream_psc without building diag dicts.
 
         With static geometry the loop runs natively (psc_run_steps): direct

@@ -5,11 +5,11 @@ workgroup shape, on non-temporal (streaming) loads/stores and on how many
 workgroups are resident per CU (fewer resident waves keep fewer DRAM pages
 open at once).  The best point moves with the lattice shape, so it is
 measured, not guessed: every candidate is timed with HIP events on the
-solver's own buffers (state A -> scratch B; the state is not modified) and
-the six fastest are timed again, four times longer, before the fastest is kept
-(the leaders are often within 1% of each other).  Results are cached per (device, lattice, storage,
-stream layout, moving, fused loads) in memory and in a JSON file
-($PSC_HIP_TUNE_CACHE, default ~/.cache/psc_hip/step_tuning.json).  When the
+solver's own buffers (state A -> scratch B; the state is not modified); the
+six fastest are timed again, four times longer (the leaders are often within
+1% of each other), and the fastest is kept.  Results are cached per (device,
+lattice, storage, stream layout, moving, fused loads) in memory and in a JSON
+file ($PSC_HIP_TUNE_CACHE, default ~/.cache/psc_hip/step_tuning.json).  When the
 native loop fuses the particle loads into the step kernels, candidates are
 timed in that loop (on scratch copies of the buffers): the extra load
 workgroups shift the best occupancy cap.  There each candidate is also timed

@@ -261,7 +261,23 @@ class BatchedPscD3Q19:
         fused = self.fuse_loads and self.P > 0
         if fused:
             self._ensure_fused_buffers()
-        best = None
+        def timed(flags, mode, reps):
+            def launch():
+                if fused:
+                    self._launch_fused(self._a, self._b, 0, True, True, s, flags, mode)
+                else:
+                    self._launch_step(self._a, self._b, flags, 0, s, mode)
+            launch()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                launch()
+            e1.record()
+            e1.synchronize()
+            return e0.elapsed_time(e1) / reps
+
+        results = []
         for bs in (256, 512):
             for mode in ("rows", "spans"):
                 for nt in (0, 1):
@@ -269,23 +285,9 @@ class BatchedPscD3Q19:
                         for inb in (0, 1):
                             flags = (autotune.flags_of(nt, bpc, 0) | (_lib.PSC_BATCH_WG512 if bs == 512 else 0)
                                      | (_lib.PSC_BATCH_IMB_NT if inb else 0))
-
-                            def launch():
-                                if fused:
-                                    self._launch_fused(self._a, self._b, 0, True, True, s, flags, mode)
-                                else:
-                                    self._launch_step(self._a, self._b, flags, 0, s, mode)
-                            launch()
-                            e0 = torch.cuda.Event(enable_timing=True)
-                            e1 = torch.cuda.Event(enable_timing=True)
-                            e0.record()
-                            for _ in range(3):
-                                launch()
-                            e1.record()
-                            e1.synchronize()
-                            ms = e0.elapsed_time(e1) / 3
-                            if best is None or ms < best[0]:
-                                best = (ms, flags, mode)
+                            results.append((timed(flags, mode, 3), flags, mode))
+        # the leaders are often within 1% of each other: time them again, longer
+        best = min((timed(flags, mode, 12), flags, mode) for _, flags, mode in sorted(results)[:4])
         self._flags, self._grid, self._mode = best[1], 0, best[2]
         if fused:  # then the placement period of the load items, for that launch
             bestp = None
@@ -294,11 +296,11 @@ class BatchedPscD3Q19:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record()
-                for _ in range(3):
+                for _ in range(6):
                     self._launch_fused(self._a, self._b, 0, True, True, s, period=period)
                 e1.record()
                 e1.synchronize()
-                ms = e0.elapsed_time(e1) / 3
+                ms = e0.elapsed_time(e1) / 6
                 if bestp is None or ms < bestp[0]:
                     bestp = (ms, period)
             self._period_tuned = bestp[1]

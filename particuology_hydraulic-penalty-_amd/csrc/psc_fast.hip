@@ -1,14 +1,17 @@
 // FP32 deviation-form PSC step (the bandwidth-bound production kernel).
 //
 // Storage is g_i = f_i - w_i in FP32, structure of arrays.  One thread owns
-// one lattice cell; a workgroup is a (block_z x block_y) tile of one x-plane,
-// so a wavefront reads and writes 64 consecutive z cells of each direction
-// plane (coalesced).  No MFMA, and the cell update uses no LDS: the step is
+// one lattice cell; a workgroup is a (block_z x block_y) tile of one x-plane
+// or, with the flat mapping, a run of consecutive cells of an x plane, so a
+// wavefront reads and writes 64 consecutive cells of each direction plane
+// (coalesced).  No MFMA, and the cell update uses no LDS: the step is
 // HBM-bound at 19*4 B read + 19*4 B written + 4 B of solid count per cell
 // (+ 12 B of imb at cells with solid).  The step kernels that also carry
-// the particle-load reduction of earlier steps (k_step_f32_loads and the
-// LOADS form of k_step_f32_batched) use a few hundred bytes of LDS in those
-// load workgroups only.
+// the particle-load reduction of earlier steps (k_step_f32_loads, its
+// in-place pair form k_step_f32_aa and the LOADS form of k_step_f32_batched)
+// use a few hundred bytes of LDS in those load workgroups only.  The in-place
+// pairs (step_cell, AA = 1 and 2) update one state buffer, so a 128^3 state
+// (159 MB) stays largely in the 256 MB Infinity Cache.
 //
 // The arithmetic restates src/lbm_dem/psc3d.py:211-236 in deviation form:
 //   rho = 1 + sum g,  j = sum g c,  u_eff = (j + F)/rho   (both half shifts)

@@ -587,7 +587,7 @@ def bench_bounce_back(args, workloads, torch, np):
     res = {}
     for prec in ("fp32", "fp64"):
         s = lbm.BounceBackD3Q19(solid, tau=0.72, force_z=2.0e-7, precision=prec, device="cuda:0",
-                                block=(128, 2), nontemporal=(prec == "fp32"))
+                                block="auto" if prec == "fp32" else (128, 2))
         s.run(warm)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -606,7 +606,8 @@ def bench_bounce_back(args, workloads, torch, np):
         kern = e0.elapsed_time(e1) / reps
         out = s.result()
         res[prec] = {"MLUPS": N * steps / wall / 1e6, "kernel_us": kern * 1e3, "k": out["permeability_lu"],
-                     "bytes_per_cell": (19 * 4 * 2 + 1) if prec == "fp32" else (19 * 8 * 2 + 1)}
+                     "bytes_per_cell": (19 * 4 * 2 + 1) if prec == "fp32" else (19 * 8 * 2 + 1),
+                     "launch": [s.block[0], s.block[1], s.flags]}
         log(f"{prec}: {res[prec]['MLUPS']:.0f} MLUPS, kernel {kern*1e3:.1f} us, k={out['permeability_lu']:.9e}",
             args.quiet)
         del s
@@ -639,7 +640,8 @@ def bench_bounce_back(args, workloads, torch, np):
                    "lattice": [n, n, n], "porosity": float(1.0 - solid.mean())},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_bb_step_f32",
-                     "bytes_per_cell": r32["bytes_per_cell"], "kernel_us": r32["kernel_us"]},
+                     "bytes_per_cell": r32["bytes_per_cell"], "kernel_us": r32["kernel_us"],
+                     "launch": r32["launch"]},
         "fp64": res["fp64"], "k_rel_err_fp32_vs_fp64": abs(r32["k"] - res["fp64"]["k"]) / abs(res["fp64"]["k"]),
         "cpu_baseline": cpu,
     }

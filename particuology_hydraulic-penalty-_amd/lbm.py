@@ -112,7 +112,7 @@ class BounceBackD3Q19:
     solid mask, BGK + Guo + bounce-back, stepped on the GPU."""
 
     def __init__(self, solid, tau: float = 0.72, force_z: float = 0.0, body_force=None, precision: str = "fp64",
-                 device=None, block=(0, 0), nontemporal: bool = False):
+                 device=None, block="auto", nontemporal: bool = False):
         torch = _torch()
         self.device = _device(device)
         solid = np.asarray(solid, dtype=bool)
@@ -139,7 +139,10 @@ class BounceBackD3Q19:
         self._partials = torch.empty(int(_lib.load().psc_moments_blocks(self.N)) * _lib.PSC_NSUM,
                                      dtype=torch.float64, device=dev)
         self._sums = torch.empty(_lib.PSC_NSUM, dtype=torch.float64, device=dev)
-        self.block = tuple(int(v) for v in block)
+        # "auto": the FP32 launch (workgroup shape, non-temporal access, occupancy cap) is
+        # timed on the first step; FP64 and explicit blocks use the given launch
+        self._autotune = isinstance(block, str) and block == "auto" and precision == "fp32"
+        self.block = (0, 0) if isinstance(block, str) else tuple(int(v) for v in block)
         self.flags = _lib.PSC_STEP_NONTEMPORAL if (nontemporal and precision == "fp32") else 0
         self._graph = None
         self.steps = 0
@@ -150,7 +153,38 @@ class BounceBackD3Q19:
         _lib.call("psc_bb_step", a.data_ptr(), b.data_ptr(), self._solid.data_ptr(), self.storage, nx, ny, nz,
                   self.tau, fx, fy, fz, self.block[0], self.block[1], self.flags, s)
 
+    def _tune(self, budget_ms: float = 300.0):
+        """Time the FP32 launch candidates on A -> B (the state is not
+        modified), then the four fastest again, longer; keep the fastest.
+        Every candidate gives the same bits."""
+        self._autotune = False
+        torch = _torch()
+        from . import autotune
+        st = torch.cuda.current_stream(self.device)
+
+        def timed(bz, by, flags, reps):
+            self.block, self.flags = (bz, by), flags
+            self._launch(self._a, self._b, st.cuda_stream)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(reps):
+                self._launch(self._a, self._b, st.cuda_stream)
+            e1.record(st)
+            e1.synchronize()
+            return e0.elapsed_time(e1) / reps
+
+        cands = [(bz, by, nt, bpc) for bz, by, nt, bpc, flat in autotune.candidates(self.shape[2]) if not flat]
+        probe = timed(64, 4, 0, 1)
+        reps = max(2, min(20, int(budget_ms / max(probe, 1e-3) / len(cands))))
+        res = sorted((timed(bz, by, autotune.flags_of(nt, bpc, 0), reps), bz, by, autotune.flags_of(nt, bpc, 0))
+                     for bz, by, nt, bpc in cands)
+        best = min((timed(bz, by, f, 4 * reps), bz, by, f) for _, bz, by, f in res[:4])
+        self.block, self.flags = (best[1], best[2]), best[3]
+
     def step(self):
+        if self._autotune:
+            self._tune()
         self._launch(self._a, self._b, _stream(self.device))
         self._a, self._b = self._b, self._a
         self.steps += 1
@@ -159,6 +193,8 @@ class BounceBackD3Q19:
         """``steps`` steps; pairs of steps replay a captured hipGraph."""
         torch = _torch()
         steps = int(steps)
+        if self._autotune:
+            self._tune()
         if not graph or steps < 4:
             for _ in range(steps):
                 self.step()

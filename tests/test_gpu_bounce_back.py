@@ -62,3 +62,33 @@ def test_run_lbm_larger_window_vs_oracle_and_fp32():
     o32 = lbm.run_lbm(solid, 0.8, 1.0e-5, steps, return_fields=True, precision="fp32", device="cuda:0")
     assert abs(o32["permeability_lu"] - ref["permeability_lu"]) <= 1e-4 * abs(ref["permeability_lu"])
     assert np.max(np.abs(o32["uz"] - ref["uz"])) <= 1e-5 * np.max(np.abs(ref["uz"]))
+
+
+def test_every_bounce_back_launch_gives_identical_results():
+    """The FP32 bounce-back step's launch tuner may pick any candidate
+    (workgroup shape, non-temporal access, occupancy cap): all compute the
+    same bits, and tuning leaves the state unchanged."""
+    import torch
+    lbm = load_package("lbm")
+    at = load_package("autotune")
+    rng = np.random.default_rng(4)
+    n = 40
+    pts = rng.uniform(0.0, 1.0, (25, 3))
+    solid, _ = BB.voxelize(pts, np.array([0.5, 0.5, 0.5]), np.array([1.0, 1.0, 1.0]), 0.1, n)
+    s = lbm.BounceBackD3Q19(solid, tau=0.8, force_z=1.0e-5, precision="fp32", device="cuda:0")
+    s.run(5)  # tunes first
+    assert not s._autotune
+    before = s._a.clone()
+    s._tune()
+    assert torch.equal(before, s._a)
+    stream = torch.cuda.current_stream().cuda_stream
+    ref = None
+    for bz, by, nt, bpc, flat in at.candidates(n):
+        if flat:
+            continue
+        s.block, s.flags = (bz, by), at.flags_of(nt, bpc, 0)
+        s._launch(s._a, s._b, stream)
+        out = s._b.cpu().numpy().copy()
+        if ref is None:
+            ref = out
+        assert np.array_equal(out, ref), (bz, by, nt, bpc)

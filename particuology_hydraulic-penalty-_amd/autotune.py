@@ -34,6 +34,8 @@ _MEM: dict = {}
 
 # (block_z, block_y, nontemporal, workgroups-per-CU cap; 0 = none, flat mapping)
 SHAPES = [(64, 4), (64, 2), (64, 1), (96, 2), (128, 2), (128, 1), (192, 1), (256, 1)]
+# Extra workgroup shapes of the FP64 step (narrow z tiles keep more rows in flight)
+FP64_SHAPES = [(32, 4), (32, 8), (16, 16)]
 # Workgroup sizes of the flat mapping (PSC_STEP_LINEAR: consecutive cells of
 # an x plane's flat (y, z) index), tried when z rows are not a whole number of
 # wavefronts: a (z, y) tile then leaves the lanes past each row end idle.
@@ -208,7 +210,11 @@ def tune(solver, budget_ms: float = 400.0, use_cache: bool = True, fused: bool =
                 _MEM[key] = cfg
                 return cfg
         nz = solver._shape[2]
-        cands = candidates(nz)
+        if solver._storage == _lib.PSC_F64_PLAIN:
+            # the FP64 step takes no launch flags: workgroup shapes only
+            cands = [(bz, by, 0, 0, False) for bz, by in SHAPES + FP64_SHAPES if not (bz > 64 and nz <= bz // 2)]
+        else:
+            cands = candidates(nz)
         # one probe launch sizes the repeat count to the time budget
         probe = time_config(solver, 64, 4, 0, 1)
         reps = max(2, min(50, int(budget_ms / max(probe, 1e-3) / len(cands))))

@@ -483,7 +483,7 @@ class PscD3Q19:
         will fuse the loads into the step kernels (static geometry built), the
         candidates are timed in that fused loop, whose best launch differs from
         the plain step's."""
-        if self._autotune and self._storage == _lib.PSC_F32_DEV:
+        if self._autotune:
             from . import autotune
             fused = self.static_geometry and self._geometry_built and self._load_modes()[0]
             if fused:

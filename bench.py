@@ -586,8 +586,7 @@ def bench_bounce_back(args, workloads, torch, np):
     N = n ** 3
     res = {}
     for prec in ("fp32", "fp64"):
-        s = lbm.BounceBackD3Q19(solid, tau=0.72, force_z=2.0e-7, precision=prec, device="cuda:0",
-                                block="auto" if prec == "fp32" else (128, 2))
+        s = lbm.BounceBackD3Q19(solid, tau=0.72, force_z=2.0e-7, precision=prec, device="cuda:0")
         s.run(warm)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -763,7 +762,7 @@ def main():
         kern64 = step_kernel_ms(sol64, max(50, args.kernel_repeats // 4), torch, lib)
         mlups64 = N * args.steps / wall64 / 1e6
         fp64 = {"MLUPS": mlups64, "ms_per_step": wall64 * 1e3 / args.steps, "k": k64,
-                "step_kernel_us": kern64 * 1e3,
+                "step_kernel_us": kern64 * 1e3, "launch": [sol64._block[0], sol64._block[1]],
                 "step_kernel_GBps": BYTES_F64_STATIC * N / (kern64 * 1e-3) / 1e9}
         log(f"fp64: {mlups64:.0f} MLUPS, step kernel {kern64*1e3:.1f} us, k={k64:.9e}", args.quiet)
         del sol64

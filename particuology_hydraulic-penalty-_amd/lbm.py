@@ -139,9 +139,9 @@ class BounceBackD3Q19:
         self._partials = torch.empty(int(_lib.load().psc_moments_blocks(self.N)) * _lib.PSC_NSUM,
                                      dtype=torch.float64, device=dev)
         self._sums = torch.empty(_lib.PSC_NSUM, dtype=torch.float64, device=dev)
-        # "auto": the FP32 launch (workgroup shape, non-temporal access, occupancy cap) is
-        # timed on the first step; FP64 and explicit blocks use the given launch
-        self._autotune = isinstance(block, str) and block == "auto" and precision == "fp32"
+        # "auto": the launch (workgroup shape; for FP32 also non-temporal access and the
+        # occupancy cap) is timed on the first step; explicit blocks are used as given
+        self._autotune = isinstance(block, str) and block == "auto"
         self.block = (0, 0) if isinstance(block, str) else tuple(int(v) for v in block)
         self.flags = _lib.PSC_STEP_NONTEMPORAL if (nontemporal and precision == "fp32") else 0
         self._graph = None
@@ -154,9 +154,9 @@ class BounceBackD3Q19:
                   self.tau, fx, fy, fz, self.block[0], self.block[1], self.flags, s)
 
     def _tune(self, budget_ms: float = 300.0):
-        """Time the FP32 launch candidates on A -> B (the state is not
-        modified), then the four fastest again, longer; keep the fastest.
-        Every candidate gives the same bits."""
+        """Time the launch candidates on A -> B (the state is not modified),
+        then the four fastest again, longer; keep the fastest.  Every
+        candidate gives the same bits."""
         self._autotune = False
         torch = _torch()
         from . import autotune
@@ -174,7 +174,12 @@ class BounceBackD3Q19:
             e1.synchronize()
             return e0.elapsed_time(e1) / reps
 
-        cands = [(bz, by, nt, bpc) for bz, by, nt, bpc, flat in autotune.candidates(self.shape[2]) if not flat]
+        if self.storage == _lib.PSC_F64_PLAIN:  # the FP64 step takes no launch flags
+            nz = self.shape[2]
+            cands = [(bz, by, 0, 0) for bz, by in autotune.SHAPES + autotune.FP64_SHAPES
+                     if not (bz > 64 and nz <= bz // 2)]
+        else:
+            cands = [(bz, by, nt, bpc) for bz, by, nt, bpc, flat in autotune.candidates(self.shape[2]) if not flat]
         probe = timed(64, 4, 0, 1)
         reps = max(2, min(20, int(budget_ms / max(probe, 1e-3) / len(cands))))
         res = sorted((timed(bz, by, autotune.flags_of(nt, bpc, 0), reps), bz, by, autotune.flags_of(nt, bpc, 0))

@@ -67,7 +67,7 @@ def _cache_path():
 
 # Part of every cache key: entries written when the candidate set was smaller
 # (before the flat mapping and the in-place step pairs) are not reused.
-CANDIDATE_SET = 3
+CANDIDATE_SET = 4
 
 
 def _key(solver):
@@ -138,10 +138,11 @@ def _fused_scratch(solver):
             torch.zeros_like(solver._partials), torch.zeros_like(solver._partials), torch.zeros_like(solver._loads))
 
 
-def time_fused_config(solver, scratch, bz, by, flags, repeats, period=None):
+def time_fused_config(solver, scratch, bz, by, flags, repeats, period=None, graph=False):
     """Average step duration (ms) of `repeats` steps of the native loop
-    (psc_run_steps, direct launches) with the loads fused into the step
-    kernels, on the scratch buffers of _fused_scratch."""
+    (psc_run_steps: direct launches, or graph replays as run() does) with the
+    loads fused into the step kernels, on the scratch buffers of
+    _fused_scratch."""
     import ctypes as ct
     import torch
     a, b, i0, i1, q0, q1, ld = scratch
@@ -159,8 +160,11 @@ def time_fused_config(solver, scratch, bz, by, flags, repeats, period=None):
     _lib.check(lib.psc_run_steps(ct.byref(p), 4, 0, 16, ct.byref(parity), h), "psc_run_steps")
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
+    if graph:  # capture the graph outside the timed region
+        _lib.check(lib.psc_run_steps(ct.byref(p), 2 + GRAPH_STEPS, 1, GRAPH_STEPS, ct.byref(parity), h),
+                   "psc_run_steps")
     e0.record(stream)
-    rc = lib.psc_run_steps(ct.byref(p), repeats, 0, 16, ct.byref(parity), h)
+    rc = lib.psc_run_steps(ct.byref(p), repeats, 1 if graph else 0, GRAPH_STEPS, ct.byref(parity), h)
     e1.record(stream)
     _lib.check(rc, "psc_run_steps")
     e1.synchronize()
@@ -168,6 +172,7 @@ def time_fused_config(solver, scratch, bz, by, flags, repeats, period=None):
 
 
 PERIODS = (1, 3, 5, 7, 9)
+GRAPH_STEPS = 32  # steps per captured graph, as run() replays them
 
 
 def tune_period(solver, bz, by, flags, repeats: int = 120, use_cache: bool = True):
@@ -185,7 +190,7 @@ def tune_period(solver, bz, by, flags, repeats: int = 120, use_cache: bool = Tru
         scratch = _fused_scratch(solver)
         best = None
         for period in PERIODS:
-            ms = time_fused_config(solver, scratch, bz, by, flags, repeats, period)
+            ms = time_fused_config(solver, scratch, bz, by, flags, repeats, period, graph=True)
             if best is None or ms < best[0]:
                 best = (ms, period)
         _MEM[key] = best[1]
@@ -232,8 +237,12 @@ def tune(solver, budget_ms: float = 400.0, use_cache: bool = True, fused: bool =
             flags = flags_of(nt, bpc, 1, flat)
             for f in ((flags, flags | _lib.PSC_RUN_INPLACE) if fused else (flags,)):
                 results.append((timer(bz, by, f), bz, by, f))
-        # the leaders are often within 1% of each other: time them again, longer
-        reps *= 4
+        # the leaders are often within 1% of each other: time them again, longer (the
+        # fused loop as run() drives it, through graph replays)
+        reps = max(4 * reps, 2 * GRAPH_STEPS + 2)
+        if fused:
+            def timer(bz, by, flags):
+                return time_fused_config(solver, scratch, bz, by, flags, reps, graph=True)
         best = min((timer(bz, by, f), bz, by, f) for _, bz, by, f in sorted(results)[:6])
         cfg = (best[1], best[2], best[3])
         _MEM[key] = cfg

@@ -308,6 +308,28 @@ def test_every_launch_configuration_gives_identical_results(psc3d):
             assert np.array_equal(out, ref), (bz, by, nt, bpc, cpt, lin)
 
 
+def test_every_fp64_launch_gives_identical_results(psc3d):
+    """The tuner also picks the FP64 step's workgroup shape: every shape it
+    may pick computes the same bits (the reference's operation order per
+    cell, no cross-cell arithmetic)."""
+    at = load_package("autotune")
+    lib = load_package("_lib")
+    import torch
+    g = G.load("perm24")
+    s, _ = _solver(psc3d, g, "fp64", block=(64, 4))
+    s.run(7)
+    nx, ny, nz = s._shape
+    h = torch.cuda.current_stream().cuda_stream
+    ref = None
+    for bz, by in at.SHAPES + at.FP64_SHAPES:
+        lib.call("psc_step", s._a.data_ptr(), s._b.data_ptr(), s._csum.data_ptr(), None, s._imb.data_ptr(),
+                 lib.PSC_F64_PLAIN, 1, nx, ny, nz, s._s, s.tau, 0.0, 0.0, float(g["force"][2]), bz, by, 0, h)
+        out = s._b.cpu().numpy().copy()
+        if ref is None:
+            ref = out
+        assert np.array_equal(out, ref), (bz, by)
+
+
 def _moving_table(rng, n, count, rmin, rmax, umax=2e-3):
     t = np.zeros((count, 10))
     t[:, 0:3] = rng.uniform(0, n, (count, 3))

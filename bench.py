@@ -97,6 +97,7 @@ def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=N
     sol.rebuild_geometry()
     sol._tune()   # launch-shape autotuning happens before the warm-up
     sol.run(warmup)
+    sol.prepare_run(steps)   # capture the graphs the timed run replays (no steps taken)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -104,12 +105,15 @@ def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=N
     t0 = time.perf_counter()
     e0.record(stream)
     sol.run(steps)
-    k = sol.permeability(axis=0)   # FP64 flux reduction of the final state (part of the job)
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = e0.elapsed_time(e1)
-    return sol, k, wall, gpu_ms
+    # the permeability readout (FP64 flux reduction of the final state), timed on its own
+    r0 = time.perf_counter()
+    k = sol.permeability(axis=0)
+    readout_ms = (time.perf_counter() - r0) * 1e3
+    return sol, k, wall, gpu_ms, readout_ms
 
 
 def step_kernel_ms(sol, repeats, torch, lib, moving=False, launch_cfg=None):
@@ -258,16 +262,22 @@ def loads_bytes(sol):
     return 26 * nnz + 96 * C + 48 * P, nnz
 
 
-def read_traffic(name="k_step_f32_loads_128_pmc.txt"):
-    """HBM bytes per launch of the FP32 step from the committed rocprofv3
-    counter summary, if present."""
+def read_traffic(name, launch):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary ``profiles/<name>`` (FETCH_SIZE + WRITE_SIZE, calibrated on the
+    copy kernel), or None when the summary is missing or was measured on
+    another launch configuration than ``launch`` (a tuple of ints that the
+    summary's ``launch:`` line must repeat)."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
+    traffic, measured = None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+            traffic = float(line.split(":", 1)[1])
+        elif line.startswith("launch:"):
+            measured = tuple(int(v, 0) for v in line.split(":", 1)[1].split(","))
+    return traffic if measured == tuple(int(v) for v in launch) else None
 
 
 def bench_moving(args, psc3d, workloads, lib, torch, np):
@@ -428,7 +438,10 @@ def bench_windows(args, workloads, torch, np):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": read_traffic("k_step_f32_batched_loads_windows_pmc.txt" if fused
-                                             else "k_step_f32_batched_windows_pmc.txt"),
+                                             else "k_step_f32_batched_windows_pmc.txt",
+                                             (B._flags, B._grid, 512 if B._flags & 0x40000 else 256,
+                                              int(B._mode == "rows"),
+                                              B.effective_fuse_period() if fused else 0)),
                      "kernel": kname,
                      "bytes_per_cell": BYTES_F32_STATIC, "bytes_per_launch": launch_bytes,
                      "solid_cells": nsolid, "imb_bytes": ibytes,
@@ -728,8 +741,11 @@ def main():
 
     log(f"fp32 run: {args.warmup} warmup + {args.steps} timed steps at {n}^3", args.quiet)
     launch = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
-    sol32, k32, wall32, gpu_ms32 = timed_run(psc3d, table, n, "fp32", args.layout, args.steps, args.warmup, torch,
-                                             launch, args.fuse_period)
+    sol32, k32, wall32, gpu_ms32, readout_ms = timed_run(psc3d, table, n, "fp32", args.layout, args.steps,
+                                                         args.warmup, torch, launch, args.fuse_period)
+    nonfinite = sol32.readout_sums()["nonfinite"]
+    if nonfinite:
+        raise RuntimeError(f"the FP32 state holds {nonfinite} non-finite cells")
     porosity = 1.0 - float(sol32.solid.mean())
     fused = sol32.loads_mode == "fused"
     inplace = fused and (sol32._inplace_tuned if sol32.inplace is None else bool(sol32.inplace))
@@ -744,7 +760,7 @@ def main():
     # fused: every timed step is one k_step_f32_loads launch (in place: one k_step_f32_aa
     # launch, the gather-scatter and local forms alternating), so the HIP-event time of the
     # timed region over the step count is that kernel's average launch duration (it also
-    # holds the once-per-run pending loads and the k readout, ~0.1 ms in all)
+    # holds the once-per-run pending loads of the last two steps, ~15 us)
     fused_ms = gpu_ms32 / args.steps if fused else None
     mlups = N * args.steps / wall32 / 1e6
     log(f"fp32: {mlups:.0f} MLUPS (wall), loads {sol32.loads_mode}{' (in-place step pairs)' if inplace else ''}, "
@@ -757,8 +773,8 @@ def main():
 
     fp64 = None
     if not args.no_fp64:
-        sol64, k64, wall64, gpu_ms64 = timed_run(psc3d, table, n, "fp64", args.layout, args.steps, args.warmup,
-                                                 torch)
+        sol64, k64, wall64, gpu_ms64, _ = timed_run(psc3d, table, n, "fp64", args.layout, args.steps, args.warmup,
+                                                    torch)
         kern64 = step_kernel_ms(sol64, max(50, args.kernel_repeats // 4), torch, lib)
         mlups64 = N * args.steps / wall64 / 1e6
         fp64 = {"MLUPS": mlups64, "ms_per_step": wall64 * 1e3 / args.steps, "k": k64,
@@ -808,18 +824,23 @@ def main():
         # the dominant kernel of the timed step: collide+stream plus the loads of earlier steps
         launch_bytes = step_bytes + lbytes
         achieved = launch_bytes / (fused_ms * 1e-3) / 1e9
+        run_launch = (sol32._block[0], sol32._block[1],
+                      sol32._flags | (lib_flag("PSC_RUN_INPLACE") if inplace else 0), sol32.effective_fuse_period())
         if inplace:
             roof_kernel, roof_us, traffic = ("k_step_f32_aa (static; step + loads of the two previous steps; "
                                              "in-place pairs of a gather-scatter and a local step on one state "
-                                             "buffer)", fused_ms * 1e3, read_traffic("k_step_f32_aa_128_pmc.txt"))
+                                             "buffer)", fused_ms * 1e3,
+                                             read_traffic(f"k_step_f32_aa_{n}_pmc.txt", run_launch))
         else:
             roof_kernel, roof_us, traffic = ("k_step_f32_loads (pull, static; step + loads of the two previous "
-                                             "steps)", fused_ms * 1e3, read_traffic("k_step_f32_loads_128_pmc.txt"))
+                                             "steps)", fused_ms * 1e3,
+                                             read_traffic(f"k_step_f32_loads_{n}_pmc.txt", run_launch))
     else:
         launch_bytes = step_bytes
         achieved = launch_bytes / (kern_ms * 1e-3) / 1e9
+        run_launch = (sol32._block[0], sol32._block[1], sol32._flags)
         roof_kernel, roof_us, traffic = ("k_step_f32 (pull, static)", kern_ms * 1e3,
-                                         read_traffic("k_step_f32_128_pmc.txt"))
+                                         read_traffic(f"k_step_f32_{n}_pmc.txt", run_launch))
     out = {
         "metric": METRIC,
         "value": mlups,
@@ -874,6 +895,9 @@ def main():
         "fp32_over_fp64_MLUPS": (mlups / fp64["MLUPS"]) if fp64 else None,
         "copy_GBps": copy_gbps,
         "gpu_ms_timed_region": gpu_ms32,
+        "gpu_us_per_step": gpu_ms32 * 1e3 / args.steps,
+        "k_readout_ms": readout_ms,
+        "nonfinite_cells": int(nonfinite),
     }
     print(json.dumps(out))
 

@@ -94,6 +94,18 @@ int psc_clear(void* d_ptr, int64_t nbytes, void* stream);
  * PSC_F64_PLAIN, g = 0 in PSC_F32_DEV.  ncell = N. */
 int psc_init_state(void* d_state, int32_t storage, int64_t ncell, void* stream);
 
+/* Module function equilibrium(rho, ux, uy, uz) (psc3d.py:97-103) over n
+ * points, FP64, in the reference's operation order: d_out[i*n + c] for
+ * direction i (the reference's (19, *shape) array). */
+int psc_equilibrium(const double* d_rho, const double* d_ux, const double* d_uy, const double* d_uz, int64_t n,
+                    double* d_out, void* stream);
+
+/* Module function guo_force(fx, fy, fz, ux, uy, uz, tau) (psc3d.py:106-114)
+ * over n points, FP64, in the reference's operation order; layout as
+ * psc_equilibrium. */
+int psc_guo_force(double fx, double fy, double fz, const double* d_ux, const double* d_uy, const double* d_uz,
+                  int64_t n, double tau, double* d_out, void* stream);
+
 /* K1 -- sphere-to-lattice solid fraction by s^3 sub-sample counting.
  * Replaces _sparse_particle_alpha_task (psc3d.py:60-94) with the sum of
  * SparsePscD3Q19.rebuild_geometry (psc3d.py:328-345) / the dense
@@ -331,10 +343,20 @@ typedef struct PscStepPlan {
  * 128^3 one 160 MB state buffer stays resident in the 256 MB Infinity Cache
  * where two do not.
  * use_graph = 1 captures `graph_steps` steps (even, >= 2) once per plan into
- * a hipGraph and replays it; the remainder runs as direct launches.  Graphs
- * are cached per plan contents until psc_graph_cache_clear(). */
+ * a hipGraph and replays it as often as it fits, then one graph of the
+ * largest even step count of the remainder (a run shorter than graph_steps
+ * is that graph alone); at most one step runs as a direct launch besides the
+ * two that start the fused form.  Graphs are cached per plan contents,
+ * step count and starting parity until psc_graph_cache_clear(). */
 int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t use_graph, int32_t graph_steps,
                   int32_t* parity, void* stream);
+
+/* Capture, without launching anything, the graphs that psc_run_steps(plan,
+ * nsteps, 1, graph_steps, parity, ...) replays, so that the run itself
+ * starts with no capture (e.g. a timed run after a warm-up of another
+ * length).  Same checks and errors as psc_run_steps; the state is not
+ * touched.  No counterpart in the reference, whose loop has no launches. */
+int psc_prepare_steps(const PscStepPlan* plan, int32_t nsteps, int32_t graph_steps, const int32_t* parity);
 
 /* Destroy every cached hipGraph of psc_run_steps. */
 int psc_graph_cache_clear(void);

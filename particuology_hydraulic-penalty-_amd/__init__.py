@@ -9,10 +9,11 @@ it with ``importlib.import_module("particuology_hydraulic-penalty-_amd")``
 """
 
 from . import _lib
-from .psc3d import C, CS2, OPP, W, LazyDiag, Particle3D, PscD3Q19, SparsePscD3Q19, StaleDiagnostics
+from .psc3d import (C, CS2, OPP, W, LazyDiag, Particle3D, PscD3Q19, SparsePscD3Q19, StaleDiagnostics, equilibrium,
+                    guo_force)
 
 __all__ = ["C", "CS2", "OPP", "W", "Particle3D", "PscD3Q19", "SparsePscD3Q19", "LazyDiag", "StaleDiagnostics",
-           "library_path", "load_library"]
+           "equilibrium", "guo_force", "library_path", "load_library"]
 
 
 def library_path() -> str:

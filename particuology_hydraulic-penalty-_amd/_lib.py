@@ -36,6 +36,8 @@ SIGNATURES = {
     "psc_lattice_constants": ([_P, _P, _P], ct.c_int),
     "psc_clear": ([_P, _I64, _P], ct.c_int),
     "psc_init_state": ([_P, _I32, _I64, _P], ct.c_int),
+    "psc_equilibrium": ([_P, _P, _P, _P, _I64, _P, _P], ct.c_int),
+    "psc_guo_force": ([_D, _D, _D, _P, _P, _P, _I64, _D, _P, _P], ct.c_int),
     "psc_voxelize": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, ct.c_uint32,
                       _P], ct.c_int),
     "psc_step_batched": ([_P, _P, _P, _P, _I64, _P, _P, _I32, _I32, _D, _I32, _I32, _P], ct.c_int),
@@ -51,6 +53,7 @@ SIGNATURES = {
     "psc_export_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_import_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_run_steps": ([_P, _I32, _I32, _I32, _P, _P], ct.c_int),
+    "psc_prepare_steps": ([_P, _I32, _I32, _P], ct.c_int),
     "psc_graph_cache_clear": ([], ct.c_int),
     "psc_copy_f32": ([_P, _P, _I64, _P], ct.c_int),
     "psc_voxelize_binary": ([_P, _P, _I32, _P, _I32, _I32, _I32, _P, _P], ct.c_int),

@@ -78,6 +78,9 @@ int launch_voxelize_binary(const double*, const int32_t*, int, const double*, in
 int launch_export(const void*, int, int, int, int, int, double*, hipStream_t);
 int launch_import(const double*, int, int, int, int, int, void*, hipStream_t);
 int launch_init_plain(double*, size_t, hipStream_t);
+int launch_equilibrium_f64(const double*, const double*, const double*, const double*, size_t, double*, hipStream_t);
+int launch_guo_f64(double, double, double, const double*, const double*, const double*, size_t, double, double*,
+                   hipStream_t);
 
 namespace {
 
@@ -118,6 +121,10 @@ int grid_cap(size_t n, int threads) {
   return g < 1 ? 1 : (int)g;
 }
 
+// Largest FP32 lattice: the kernels address a direction plane with 32-bit
+// byte offsets (the last cell of a 2^30-cell plane is at 2^32 - 4).
+constexpr unsigned long long kMaxF32Cells = 1ull << 30;
+
 bool dims_ok(int nx, int ny, int nz) {
   if (nx < 1 || ny < 1 || nz < 1) return false;
   const unsigned long long n = (unsigned long long)nx * ny * nz;
@@ -131,9 +138,13 @@ using namespace psc;
 
 namespace psc {
 // Argument checks of psc_step; resolves the default block shape into *bz, *by.
-int step_args(const void* d_in, const void* d_out, const uint32_t* d_count_sum, const void* d_us_num, int layout,
-              int nx, int ny, int nz, int subsamples, double tau, int flags, int* bz, int* by) {
+int step_args(const void* d_in, const void* d_out, const uint32_t* d_count_sum, const void* d_us_num, int storage,
+              int layout, int nx, int ny, int nz, int subsamples, double tau, int flags, int* bz, int* by) {
   if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_step: bad lattice %dx%dx%d", nx, ny, nz);
+  // the FP32 kernels address a direction plane with 32-bit byte offsets
+  if (storage == PSC_F32_DEV && (unsigned long long)nx * ny * nz > kMaxF32Cells)
+    return set_error(PSC_EINVAL, "psc_step: FP32 lattice %dx%dx%d exceeds 2^30 cells (32-bit byte offsets in a plane)", nx, ny,
+                     nz);
   if (d_in == nullptr || d_out == nullptr || d_count_sum == nullptr) return set_error(PSC_EINVAL, "psc_step: null");
   if (d_in == d_out) return set_error(PSC_EINVAL, "psc_step: input and output alias");
   if (layout != PSC_PUSH && layout != PSC_PULL) return set_error(PSC_EINVAL, "psc_step: layout %d", layout);
@@ -198,6 +209,25 @@ int psc_init_state(void* d_state, int32_t storage, int64_t ncell, void* stream) 
   return set_error(PSC_EINVAL, "psc_init_state: storage %d", storage);
 }
 
+int psc_equilibrium(const double* d_rho, const double* d_ux, const double* d_uy, const double* d_uz, int64_t n,
+                    double* d_out, void* stream) {
+  if (n < 0) return set_error(PSC_EINVAL, "psc_equilibrium: negative size");
+  if (n == 0) return PSC_OK;
+  if (d_rho == nullptr || d_ux == nullptr || d_uy == nullptr || d_uz == nullptr || d_out == nullptr)
+    return set_error(PSC_EINVAL, "psc_equilibrium: null pointer");
+  return launch_equilibrium_f64(d_rho, d_ux, d_uy, d_uz, (size_t)n, d_out, (hipStream_t)stream);
+}
+
+int psc_guo_force(double fx, double fy, double fz, const double* d_ux, const double* d_uy, const double* d_uz,
+                  int64_t n, double tau, double* d_out, void* stream) {
+  if (n < 0) return set_error(PSC_EINVAL, "psc_guo_force: negative size");
+  if (!(tau > 0.0)) return set_error(PSC_EINVAL, "psc_guo_force: tau %g must be positive", tau);
+  if (n == 0) return PSC_OK;
+  if (d_ux == nullptr || d_uy == nullptr || d_uz == nullptr || d_out == nullptr)
+    return set_error(PSC_EINVAL, "psc_guo_force: null pointer");
+  return launch_guo_f64(fx, fy, fz, d_ux, d_uy, d_uz, (size_t)n, tau, d_out, (hipStream_t)stream);
+}
+
 int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks, int32_t nchunks,
                  int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, int32_t pitch_z, int32_t subsamples,
                  uint32_t* d_ent_cell,
@@ -237,8 +267,8 @@ int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const v
              int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double tau,
              double fx, double fy, double fz, int32_t block_z, int32_t block_y, int32_t flags, void* stream) {
   int bz = block_z, by = block_y;
-  const int rc = psc::step_args(d_in, d_out, d_count_sum, d_us_num, layout, nx, ny, nz, subsamples, tau, flags, &bz,
-                                &by);
+  const int rc = psc::step_args(d_in, d_out, d_count_sum, d_us_num, storage, layout, nx, ny, nz, subsamples, tau,
+                                flags, &bz, &by);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (storage == PSC_F32_DEV)
@@ -756,67 +786,133 @@ int graph_for(const PscStepPlan& p, int steps, int parity, int kind, hipGraphExe
 }  // namespace
 }  // namespace psc
 
-extern "C" int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t use_graph, int32_t graph_steps,
-                             int32_t* parity, void* stream) {
-  if (plan == nullptr || parity == nullptr || nsteps < 0) return set_error(PSC_EINVAL, "psc_run_steps: arguments");
-  if (*parity != 0 && *parity != 1) return set_error(PSC_EINVAL, "psc_run_steps: parity %d", *parity);
-  if (use_graph && (graph_steps < 2 || (graph_steps & 1)))
-    return set_error(PSC_EINVAL, "psc_run_steps: graph_steps %d must be even and >= 2", graph_steps);
-  hipStream_t s = (hipStream_t)stream;
-  const PscStepPlan key = normalized(*plan);
-  int par = *parity;
-  int done = 0;
+namespace psc {
+namespace {
+
+// Steps per replayed graph for `avail` steps: graph_steps, or for a shorter
+// run the largest even count that fits (0: too short for a graph).
+int graph_size(int avail, int graph_steps) {
+  if (avail >= graph_steps) return graph_steps;
+  return avail >= 2 ? (avail & ~1) : 0;
+}
+
+// The graphs a run of nsteps replays, in order: `reps` replays of `g1` steps
+// starting at parity `par1`, then one replay of `g2` steps (0: none) at
+// parity `par2`.  A fused run first takes two direct steps (after them both
+// loads stages are pending); even graphs leave the parity unchanged.
+struct RunShape {
+  int head = 0, g1 = 0, reps = 0, g2 = 0, kind = 0;
+};
+
+RunShape run_shape(const PscStepPlan& key, int nsteps, bool use_graph, int graph_steps) {
+  RunShape r;
+  const bool fused = fused_plan(key);
+  r.kind = fused ? 1 : 0;
+  r.head = fused ? (nsteps < 2 ? nsteps : 2) : 0;
+  if (!use_graph) return r;
+  const int avail = nsteps - r.head;
+  r.g1 = graph_size(avail, graph_steps);
+  if (r.g1 == 0) return r;
+  r.reps = avail / r.g1;
+  r.g2 = graph_size(avail - r.reps * r.g1, graph_steps);
+  return r;
+}
+
+int check_run_plan(const PscStepPlan& key) {
   if ((key.flags & PSC_RUN_INPLACE) && !fused_plan(key))
     return set_error(PSC_EINVAL, "psc_run_steps: in-place steps need the fused form (FP32 pull, no solid velocity, "
                      "fuse_loads > 0, particles, alternate imb and partials)");
   if (fused_plan(key)) {
-    // two direct steps (after them both loads stages are pending), replays
-    // of a graph of fused steps, the remainder, then the pending loads
     int bz = key.block_z, by = key.block_y;
-    int rc = step_args(key.state_a, key.state_b, key.count_sum, key.us_num, key.layout, key.nx, key.ny, key.nz,
-                       key.subsamples, key.tau, key.flags & ~PSC_RUN_INPLACE, &bz, &by);
+    int rc = step_args(key.state_a, key.state_b, key.count_sum, key.us_num, key.storage, key.layout, key.nx, key.ny,
+                       key.nz, key.subsamples, key.tau, key.flags & ~PSC_RUN_INPLACE, &bz, &by);
     if (rc) return rc;
     if ((key.flags & PSC_RUN_INPLACE) && (key.flags & PSC_STEP_CPT_MASK) > (1 << 16))
       return set_error(PSC_EINVAL, "psc_run_steps: in-place steps take one cell per thread");
     if (key.imb == key.imb_alt || key.partials == key.partials_alt)
       return set_error(PSC_EINVAL, "psc_run_steps: imb/partials buffers alias");
-    bool pc = false, pf = false;
-    const int head = nsteps < 2 ? nsteps : 2;
-    rc = run_fused(key, head, &par, &pc, &pf, s);
+  }
+  return PSC_OK;
+}
+
+int check_run_args(const PscStepPlan* plan, int32_t nsteps, int32_t use_graph, int32_t graph_steps,
+                   const int32_t* parity) {
+  if (plan == nullptr || parity == nullptr || nsteps < 0) return set_error(PSC_EINVAL, "psc_run_steps: arguments");
+  if (*parity != 0 && *parity != 1) return set_error(PSC_EINVAL, "psc_run_steps: parity %d", *parity);
+  if (use_graph && (graph_steps < 2 || (graph_steps & 1)))
+    return set_error(PSC_EINVAL, "psc_run_steps: graph_steps %d must be even and >= 2", graph_steps);
+  return PSC_OK;
+}
+
+int replay(const PscStepPlan& key, int steps, int parity, int kind, int times, hipStream_t s) {
+  hipGraphExec_t exec = nullptr;
+  const int rc = graph_for(key, steps, parity, kind, &exec);
+  if (rc) return rc;
+  for (int t = 0; t < times; ++t) {
+    const hipError_t e = hipGraphLaunch(exec, s);
+    if (e != hipSuccess) return set_error((int)e, "psc_run_steps: graph launch: %s", hipGetErrorString(e));
+  }
+  return PSC_OK;
+}
+
+}  // namespace
+}  // namespace psc
+
+extern "C" int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t use_graph, int32_t graph_steps,
+                             int32_t* parity, void* stream) {
+  int rc = check_run_args(plan, nsteps, use_graph, graph_steps, parity);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const PscStepPlan key = normalized(*plan);
+  rc = check_run_plan(key);
+  if (rc) return rc;
+  const RunShape r = run_shape(key, nsteps, use_graph != 0, graph_steps);
+  int par = *parity;
+  bool pc = false, pf = false;
+  if (r.kind == 1) {
+    // two direct steps (after them both loads stages are pending), the graph
+    // replays, the remainder, then the pending loads
+    rc = run_fused(key, r.head, &par, &pc, &pf, s);
     if (rc) return rc;
-    done = head;
-    if (use_graph && nsteps - done >= graph_steps) {
-      hipGraphExec_t exec = nullptr;
-      rc = graph_for(key, graph_steps, par, 1, &exec);
-      if (rc) return rc;
-      while (nsteps - done >= graph_steps) {
-        const hipError_t e = hipGraphLaunch(exec, s);
-        if (e != hipSuccess) return set_error((int)e, "psc_run_steps: graph launch: %s", hipGetErrorString(e));
-        done += graph_steps;
-      }
-    }
+  }
+  int done = r.head;
+  if (r.reps > 0) {
+    rc = replay(key, r.g1, par, r.kind, r.reps, s);
+    if (rc) return rc;
+    done += r.reps * r.g1;
+  }
+  if (r.g2 > 0) {
+    rc = replay(key, r.g2, par, r.kind, 1, s);
+    if (rc) return rc;
+    done += r.g2;
+  }
+  if (r.kind == 1) {
     rc = run_fused(key, nsteps - done, &par, &pc, &pf, s);
     if (rc == PSC_OK) rc = flush_fused(key, par, pc, pf, s);
-    if (rc) return rc;
-    *parity = par;
-    return PSC_OK;
+  } else if (done < nsteps) {
+    rc = run_sequence(key, nsteps - done, &par, s);
   }
-  if (use_graph && nsteps >= graph_steps) {
-    hipGraphExec_t exec = nullptr;
-    int rc = graph_for(key, graph_steps, par, 0, &exec);
-    if (rc) return rc;
-    while (nsteps - done >= graph_steps) {
-      const hipError_t e = hipGraphLaunch(exec, s);
-      if (e != hipSuccess) return set_error((int)e, "psc_run_steps: graph launch: %s", hipGetErrorString(e));
-      done += graph_steps;  // even: parity unchanged
-    }
-  }
-  if (done < nsteps) {
-    const int rc = run_sequence(key, nsteps - done, &par, s);
-    if (rc) return rc;
-  }
+  if (rc) return rc;
   *parity = par;
   return PSC_OK;
+}
+
+extern "C" int psc_prepare_steps(const PscStepPlan* plan, int32_t nsteps, int32_t graph_steps,
+                                 const int32_t* parity) {
+  int rc = check_run_args(plan, nsteps, 1, graph_steps, parity);
+  if (rc) return rc;
+  const PscStepPlan key = normalized(*plan);
+  rc = check_run_plan(key);
+  if (rc) return rc;
+  const RunShape r = run_shape(key, nsteps, true, graph_steps);
+  // the head is even (or no graph follows it), so every graph starts at *parity
+  hipGraphExec_t exec = nullptr;
+  if (r.reps > 0) {
+    rc = graph_for(key, r.g1, *parity, r.kind, &exec);
+    if (rc) return rc;
+  }
+  if (r.g2 > 0) rc = graph_for(key, r.g2, *parity, r.kind, &exec);
+  return rc;
 }
 
 extern "C" int psc_graph_cache_clear(void) {

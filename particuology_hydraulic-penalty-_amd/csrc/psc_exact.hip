@@ -700,6 +700,30 @@ __global__ void k_init_plain(double* __restrict__ f, size_t N) {
   }
 }
 
+// Module functions equilibrium / guo_force (psc3d.py:97-114) over n points:
+// out[i * n + c], the reference's (19, *shape) layout, in its operation order.
+__global__ void k_equilibrium_f64(const double* __restrict__ rho, const double* __restrict__ ux,
+                                  const double* __restrict__ uy, const double* __restrict__ uz, size_t n,
+                                  double* __restrict__ out) {
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+    double f[Q];
+    equilibrium_all(Dirs{}, rho[c], ux[c], uy[c], uz[c], f);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) out[(size_t)i * n + c] = f[i];
+  }
+}
+
+__global__ void k_guo_f64(double fx, double fy, double fz, const double* __restrict__ ux,
+                          const double* __restrict__ uy, const double* __restrict__ uz, size_t n, double pref,
+                          double* __restrict__ out) {
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+    double g[Q];
+    guo_all(Dirs{}, pref, ux[c], uy[c], uz[c], fx, fy, fz, g);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) out[(size_t)i * n + c] = g[i];
+  }
+}
+
 int grid_for(size_t n, int threads, int cap = 8192) {
   size_t g = (n + threads - 1) / threads;
   if (g > (size_t)cap) g = cap;
@@ -910,6 +934,19 @@ int launch_import(const double* f, int storage, int layout, int nx, int ny, int 
     else k_import<double, false, false><<<g, 256, 0, stream>>>(f, d, (double*)st);
   }
   return check_launch("psc_import_f64");
+}
+
+int launch_equilibrium_f64(const double* rho, const double* ux, const double* uy, const double* uz, size_t n,
+                           double* out, hipStream_t stream) {
+  k_equilibrium_f64<<<grid_for(n, 256), 256, 0, stream>>>(rho, ux, uy, uz, n, out);
+  return check_launch("psc_equilibrium");
+}
+
+int launch_guo_f64(double fx, double fy, double fz, const double* ux, const double* uy, const double* uz, size_t n,
+                   double tau, double* out, hipStream_t stream) {
+  // prefactor 1 - 0.5/tau as psc3d.py:107 forms it
+  k_guo_f64<<<grid_for(n, 256), 256, 0, stream>>>(fx, fy, fz, ux, uy, uz, n, 1.0 - 0.5 / tau, out);
+  return check_launch("psc_guo_force");
 }
 
 int launch_init_plain(double* f, size_t N, hipStream_t stream) {

@@ -583,8 +583,8 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
   const dim3 grid = lin ? dim3(per_plane, nx, 1) : dim3((nz + bz - 1) / bz, (rows + by - 1) / by, nx);
   const float inv_nz = 1.0f / (float)nz;
   if (cpt != 1 && cpt != 2) return set_error(PSC_EINVAL, "psc_step(f32): cells per thread %d", cpt);
-  if ((size_t)nx * ny * nz * sizeof(float) > 0xffffffffull)
-    return set_error(PSC_EINVAL, "psc_step(f32): a direction plane must stay below 4 GiB (N < 2^30 cells)");
+  if ((size_t)nx * ny * nz > ((size_t)1 << 30))
+    return set_error(PSC_EINVAL, "psc_step(f32): at most 2^30 cells (32-bit byte offsets in a plane)");
   if (lin && (cpt != 1 || (size_t)ny * nz >= (1u << 24)))
     return set_error(PSC_EINVAL, "psc_step(f32): the flat mapping needs one cell per thread and ny*nz < 2^24");
   StepConsts k;
@@ -653,8 +653,8 @@ int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, flo
     return set_error(PSC_EINVAL, "psc_step(f32+loads): in-place form %d needs one buffer and one cell per thread", aa);
   if (aa == 0 && in == out) return set_error(PSC_EINVAL, "psc_step(f32+loads): input and output alias");
   if ((bz * by) % 64 != 0 || bz * by > 256) return set_error(PSC_ELAUNCH, "psc_step(f32+loads): block %dx%d", bz, by);
-  if ((size_t)nx * ny * nz * sizeof(float) > 0xffffffffull)
-    return set_error(PSC_EINVAL, "psc_step(f32+loads): a direction plane must stay below 4 GiB (N < 2^30 cells)");
+  if ((size_t)nx * ny * nz > ((size_t)1 << 30))
+    return set_error(PSC_EINVAL, "psc_step(f32+loads): at most 2^30 cells (32-bit byte offsets in a plane)");
   const bool lin = (nt & PSC_STEP_LINEAR) != 0;
   if (lin && (cpt != 1 || (size_t)ny * nz >= (1u << 24)))
     return set_error(PSC_EINVAL, "psc_step(f32+loads): the flat mapping needs one cell per thread and ny*nz < 2^24");
@@ -726,8 +726,8 @@ int launch_bb_step_f32(const float* in, float* out, const uint8_t* solid, int nx
     lds = (kLdsPerCU - kLdsHeadroom) / (size_t)bpc;
     if (lds > 65536) lds = 65536;
   }
-  if ((size_t)nx * ny * nz * sizeof(float) > 0xffffffffull)
-    return set_error(PSC_EINVAL, "psc_bb_step(f32): a direction plane must stay below 4 GiB");
+  if ((size_t)nx * ny * nz > ((size_t)1 << 30))
+    return set_error(PSC_EINVAL, "psc_bb_step(f32): at most 2^30 cells (32-bit byte offsets in a plane)");
   StepConsts k;
   k.field = 0;
   k.inv_s3 = 0.0f;

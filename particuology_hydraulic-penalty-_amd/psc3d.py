@@ -1,8 +1,8 @@
 """Drop-in MI355X implementation of ``src/lbm_dem/psc3d.py``.
 
 Same public names as the reference module (``C, W, OPP, CS2, Particle3D,
-PscD3Q19, SparsePscD3Q19``), same constructor signatures, methods, attributes
-and ``diag`` keys (psc3d.py:17-407).  The lattice lives in HBM as torch-ROCm
+equilibrium, guo_force, PscD3Q19, SparsePscD3Q19``), same constructor
+signatures, methods, attributes and ``diag`` keys (psc3d.py:17-407).  The lattice lives in HBM as torch-ROCm
 tensors; every step runs the hand-written gfx950 kernels of libpsc_hip.so
 (voxelize, fused collide+stream, particle loads, moments).  NumPy arrays are
 produced only when an attribute is read.
@@ -35,6 +35,8 @@ from .geometry import build_tables, pack_particles
 PIPELINE_MIN_CELLS = 1 << 19  # unfused FP32 lattices from this size overlap the loads with the next step
 FUSE_MAX_CELLS = 1 << 23      # FP32 lattices below this size fuse the loads into the step kernels
 FUSE_PERIOD = 5               # fused loads: one load workgroup in every 5 at the head of the grid
+MAX_F32_CELLS = 1 << 30        # FP32 kernels address a direction plane with 32-bit byte offsets
+MAX_CELLS = (1 << 31) - 1      # cell ids are 32-bit
 
 # Lattice constants (psc3d.py:17-43): identical values and order.
 C = np.array(
@@ -70,6 +72,49 @@ class Particle3D:
 def _torch():
     import torch
     return torch
+
+
+def _on_device(shape, *arrays):
+    """FP64 device copies of ``arrays`` broadcast to ``shape`` (flattened),
+    on the current GPU; raises PscError without one (no host path)."""
+    torch = _torch()
+    _lib.load()
+    if not torch.cuda.is_available():
+        raise _lib.PscError("equilibrium/guo_force run on a ROCm GPU (gfx950); torch.cuda.is_available() is False")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    out = []
+    for a in arrays:
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), shape)).reshape(-1)
+        out.append(torch.from_numpy(b).to(dev))
+    return dev, out
+
+
+def equilibrium(rho, ux, uy, uz) -> np.ndarray:
+    """feq_i = w_i rho (1 + c_i.u/cs2 + (c_i.u)^2/(2 cs2^2) - u.u/(2 cs2)) as
+    the (19, *rho.shape) float64 array of psc3d.py:97-103, computed on the GPU
+    (psc_equilibrium, FP64, the reference's operation order)."""
+    torch = _torch()
+    shape = np.shape(rho)
+    dev, (r, x, y, z) = _on_device(shape, rho, ux, uy, uz)
+    n = r.numel()
+    out = torch.empty(19 * n, dtype=torch.float64, device=dev)
+    _lib.call("psc_equilibrium", r.data_ptr(), x.data_ptr(), y.data_ptr(), z.data_ptr(), n, out.data_ptr(),
+              torch.cuda.current_stream(dev).cuda_stream)
+    return out.cpu().numpy().reshape((19,) + tuple(shape))
+
+
+def guo_force(fx, fy, fz, ux, uy, uz, tau) -> np.ndarray:
+    """Guo forcing term (1 - 1/(2 tau)) w_i ((c_i.F - u.F)/cs2 + (c_i.u)(c_i.F)/cs2^2)
+    as the (19, *ux.shape) float64 array of psc3d.py:106-114, computed on the
+    GPU (psc_guo_force, FP64, the reference's operation order)."""
+    torch = _torch()
+    shape = np.shape(ux)
+    dev, (x, y, z) = _on_device(shape, ux, uy, uz)
+    n = x.numel()
+    out = torch.empty(19 * n, dtype=torch.float64, device=dev)
+    _lib.call("psc_guo_force", float(fx), float(fy), float(fz), x.data_ptr(), y.data_ptr(), z.data_ptr(), n,
+              float(tau), out.data_ptr(), torch.cuda.current_stream(dev).cuda_stream)
+    return out.cpu().numpy().reshape((19,) + tuple(shape))
 
 
 class StaleDiagnostics(RuntimeError):
@@ -155,19 +200,26 @@ class PscD3Q19:
                  static_geometry: bool = False, *, body_force=None, precision: str = "fp64",
                  stream_layout: str = "pull", periodic_geometry: bool = False, device=None,
                  chunk_cells: int = 2048, block="auto", nontemporal: bool = False, diag: str = "lazy"):
-        torch = _torch()
-        self._lib = _lib.load()
-        if not torch.cuda.is_available():
-            raise _lib.PscError("PscD3Q19 needs a ROCm GPU (gfx950); torch.cuda.is_available() is False")
         if isinstance(n, (tuple, list)):
             shape = tuple(int(v) for v in n)
             if len(shape) != 3:
                 raise ValueError("n must be an int or a 3-tuple")
         else:
             shape = (int(n),) * 3
+        if min(shape) < 1:
+            raise ValueError(f"lattice {shape} has an empty axis")
         self._shape = shape
         self.n = shape[0]
         self._N = shape[0] * shape[1] * shape[2]
+        if precision == "fp32" and self._N > MAX_F32_CELLS:
+            raise ValueError(f"an FP32 lattice holds at most {MAX_F32_CELLS} cells (32-bit byte offsets in a plane); "
+                             f"{shape} has {self._N}")
+        if self._N > MAX_CELLS:
+            raise ValueError(f"a lattice holds at most {MAX_CELLS} cells; {shape} has {self._N}")
+        torch = _torch()
+        self._lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise _lib.PscError("PscD3Q19 needs a ROCm GPU (gfx950); torch.cuda.is_available() is False")
         self._particles = particles
         self.tau = float(tau)
         if not self.tau > 0.5:
@@ -390,8 +442,9 @@ class PscD3Q19:
         """``steps`` calls of collide_stream_psc without building diag dicts.
 
         With static geometry the loop runs natively (psc_run_steps): direct
-        launches, or a captured hipGraph of ``graph_steps`` steps replayed.
-        With dynamic geometry every step re-voxelizes from Python objects."""
+        launches, or captured hipGraphs of ``graph_steps`` steps (and of the
+        even part of the remainder, or of a shorter run) replayed.  With
+        dynamic geometry every step re-voxelizes from Python objects."""
         steps = int(steps)
         if steps <= 0:
             return
@@ -399,14 +452,9 @@ class PscD3Q19:
             for _ in range(steps):
                 self._advance()
             return
-        if self._autotune:
-            self._tune()
         import ctypes as ct
-        fused, pipelined = self._load_modes()
+        plan, fused, pipelined = self._run_plan()
         self.loads_mode = "fused" if fused else ("pipelined" if pipelined else "in line")
-        if pipelined or fused:
-            self._ensure_pipeline_buffers()
-        plan = self._plan(pipelined, fused)
         parity = ct.c_int32(0)
         _lib.call("psc_run_steps", ct.byref(plan), steps, 1 if graph else 0, int(graph_steps), ct.byref(parity),
                   self._stream())
@@ -416,6 +464,29 @@ class PscD3Q19:
         self._imb = self._imb_bufs[1 - parity.value] if (pipelined or fused) else self._imb_bufs[0]
         self._step_count += steps
         self._cache.clear()
+
+    def _run_plan(self):
+        """(plan, fused, pipelined) of the native loop, tuned and with its
+        buffers allocated."""
+        if self._autotune:
+            self._tune()
+        fused, pipelined = self._load_modes()
+        if pipelined or fused:
+            self._ensure_pipeline_buffers()
+        return self._plan(pipelined, fused), fused, pipelined
+
+    def prepare_run(self, steps: int, graph_steps: int = 32) -> None:
+        """Build the geometry, tune the launch and capture the hipGraphs that
+        ``run(steps, graph_steps=graph_steps)`` replays, without stepping
+        (psc_prepare_steps), so that run() itself starts at once."""
+        import ctypes as ct
+        if not self._geometry_built:
+            self.rebuild_geometry()
+        if not self.static_geometry or int(steps) <= 0:
+            return
+        plan, _, _ = self._run_plan()
+        parity = ct.c_int32(0)
+        _lib.call("psc_prepare_steps", ct.byref(plan), int(steps), int(graph_steps), ct.byref(parity))
 
     def effective_fuse_period(self) -> int:
         """Placement period of the load workgroups in fused steps."""

@@ -11,14 +11,15 @@ It imports ``scripts/run_3d_window_lbm.py`` (``voxelize``, ``run_lbm``) and
 FP64), drives them on small seeded cases and stores inputs and outputs as
 ``tests/golden/bb_*.npz``.  The step's in-place bounce-back loop and the
 exact rounding of the voxel tests are what closed forms cannot pin.
-Matplotlib, which those scripts import, is pointed at a scratch config
-directory under the repository's ignored work area.
+Matplotlib, which those scripts import, is pointed at a temporary config
+directory.
 """
 
 from __future__ import annotations
 
 import os
 import sys
+import tempfile
 from pathlib import Path
 
 import numpy as np
@@ -29,7 +30,7 @@ REF = Path(sys.argv[1]) if len(sys.argv) > 1 else Path("/root/reference")
 
 def load_reference():
     os.environ.setdefault("MPLBACKEND", "Agg")
-    os.environ.setdefault("MPLCONFIGDIR", str(HERE.parents[1] / "_work" / "mplconfig"))
+    os.environ.setdefault("MPLCONFIGDIR", tempfile.mkdtemp(prefix="mplconfig_"))
     sys.path.insert(0, str(REF))
     import scripts.run_3d_window_lbm as w  # noqa: E402  (reference module, read-only)
     import scripts.run_local_fines_lbm_permeability as f  # noqa: E402

@@ -120,3 +120,29 @@ def test_solver_refuses_to_run_without_gpu():
     lib = load_package("_lib")
     with pytest.raises(lib.PscError):
         psc3d.PscD3Q19(8, [psc3d.Particle3D(4, 4, 4, 2)])
+
+
+def test_fp32_lattices_beyond_2_30_cells_are_rejected(lib):
+    """The FP32 kernels address a direction plane with 32-bit byte offsets:
+    psc_step and the native loop refuse FP32 lattices above 2^30 cells
+    (1025 x 1024 x 1024 here) before any launch, and the facade refuses them
+    before allocating; FP64 keeps element offsets (cell ids up to 2^31 - 1)."""
+    h = lib.load()
+    p16, p32, p64 = (ct.c_void_p(v) for v in (16, 32, 64))
+    rc = h.psc_step(p16, p32, p64, None, None, lib.PSC_F32_DEV, 1, 1025, 1024, 1024, 2, 0.8, 0, 0, 0, 0, 0, 0, None)
+    assert rc == -1 and b"2^30" in h.psc_last_error()
+    rc = h.psc_step(p16, p32, p64, None, None, lib.PSC_F32_DEV, 1, 1024, 1024, 1040, 2, 0.8, 0, 0, 0, 0, 0, 0, None)
+    assert rc == -1 and b"2^30" in h.psc_last_error()
+    plan = lib.PscStepPlan()
+    plan.state_a, plan.state_b, plan.count_sum = 16, 32, 64
+    plan.imb, plan.imb_alt, plan.partials, plan.partials_alt = 128, 192, 256, 512
+    plan.storage, plan.layout, plan.nx, plan.ny, plan.nz, plan.subsamples = lib.PSC_F32_DEV, 1, 1025, 1024, 1024, 2
+    plan.tau, plan.nparticles, plan.fuse_loads = 0.8, 2, 5
+    par = ct.c_int32(0)
+    rc = h.psc_run_steps(ct.byref(plan), 4, 0, 16, ct.byref(par), None)
+    assert rc == -1 and b"2^30" in h.psc_last_error()
+    psc3d = load_package("psc3d")
+    with pytest.raises(ValueError, match="FP32 lattice"):
+        psc3d.PscD3Q19((1025, 1024, 1024), [], precision="fp32")
+    with pytest.raises(ValueError, match="at most"):
+        psc3d.PscD3Q19((2048, 1024, 1024), [], precision="fp64")

@@ -1,10 +1,13 @@
 """HIP path against the oracle and the reference's golden fixtures (GPU).
 
 Tolerances: the FP64 plain kernel follows the reference's operation order,
-so f is bit-identical to the reference's; per-particle loads and global sums use tree reductions and agree to
-1e-10 relative.  The FP32 deviation kernel agrees to 5e-6 absolute in f
-(deviations ~ 1e-4 stored with a 2^-24 relative step) and 1e-3 relative in
-loads over <= 100 steps.
+so f is bit-identical to the reference's; per-particle loads and global sums
+use tree reductions and agree to 1e-10 relative.  The FP32 deviation kernel
+stores g = f - w with a 2^-24 relative step, so its error in f is measured
+against the signal itself, max|f_ref - w| (2.8e-8 to 4.3e-5 on the golden
+cases): it must stay below 1e-4 of it (a kernel that left every deviation at
+zero would miss by the whole signal), and the loads agree to 1e-3 relative
+over <= 100 steps.
 """
 
 import numpy as np
@@ -78,14 +81,20 @@ def test_fp32_deviation_close_to_reference(psc3d, name):
     s, parts = _solver(psc3d, g, "fp32")
     print(f"[{name}] running fp32", flush=True)
     F, T, D = _run(s, parts, name, int(g["steps"]))
+    n = int(g["n"])
     if "f" in g:
         err = np.max(np.abs(s.f - g["f"]))
+        signal = np.max(np.abs(g["f"] - O.W[:, None, None, None]))
     else:
         err = np.max(np.abs(s.f.ravel()[g["f_idx"]] - g["f_val"]))
+        signal = np.max(np.abs(g["f_val"] - O.W[np.asarray(g["f_idx"]) // n ** 3]))
     scale = max(np.max(np.abs(g["forces"])), 1e-30)
     ef = np.max(np.abs(F[-1] - g["forces"][-1])) / scale
-    print(f"  fp32 max|f - f_ref| = {err:.3e}, rel force err {ef:.3e}")
-    assert err <= 5e-6
+    tol = 1e-4 * signal
+    print(f"  fp32 max|f - f_ref| = {err:.3e} = {err / signal:.2e} of the signal max|f_ref - w| = {signal:.3e}, "
+          f"rel force err {ef:.3e}")
+    assert signal > tol  # the rest state f = w misses by the whole signal
+    assert err <= tol
     assert ef <= 1e-3
 
 
@@ -471,3 +480,55 @@ def test_batched_windows_match_single_solvers(psc3d, launch):
         assert np.array_equal(B.window_state(w), s._a.cpu().numpy().reshape(B.window_state(w).shape)), w
         assert np.array_equal(loads[w], s._loads_np()), w
         assert k[w] == pytest.approx(s.permeability(axis=2), rel=1e-12)
+
+
+def test_module_equilibrium_and_guo_force_match_oracle(psc3d):
+    """The module functions equilibrium / guo_force (psc3d.py:97-114) run on
+    the GPU in the reference's operation order: bit-identical to the oracle's
+    restatement (itself pinned by the golden trajectories and the moment
+    identities), also for broadcast scalars and a z-only force."""
+    rng = np.random.default_rng(41)
+    shape = (5, 6, 7)
+    rho = 1.0 + 0.01 * rng.standard_normal(shape)
+    u = 0.03 * rng.standard_normal((3,) + shape)
+    got = psc3d.equilibrium(rho, *u)
+    assert got.shape == (19,) + shape
+    assert np.array_equal(got, O.equilibrium(rho, *u))
+    assert np.array_equal(psc3d.equilibrium(rho, 0.0, 0.0, u[2]), O.equilibrium(rho, 0.0 * rho, 0.0 * rho, u[2]))
+    for F in ((3e-6, -1e-6, 2e-6), (0.0, 0.0, 1e-7)):
+        g = psc3d.guo_force(*F, *u, 0.72)
+        assert np.array_equal(g, O.guo_force(*F, *u, 0.72)), F
+    # the facade is what the reference's init uses: f = equilibrium(1, 0, 0, 0) = W
+    w = psc3d.equilibrium(np.ones((2, 2, 2)), 0.0, 0.0, 0.0)
+    assert np.array_equal(w[:, 0, 0, 0], psc3d.W)
+
+
+def test_fluid_statistics_readout_matches_oracle(psc3d):
+    """PscD3Q19.fluid_statistics(axis) -- (intrinsic, superficial, porosity)
+    of run_precision_validation.py:142-147 from FP64 GPU sums -- against the
+    oracle's readout of the oracle's own perm24 run, on every axis (the z
+    readout carries the flow, x and y are round-off), and the permeability
+    and mean-u_z readouts of the same state."""
+    g = G.load("perm24")
+    s, _ = _solver(psc3d, g, "fp64")
+    parts = [O.Sphere(*row) for row in g["particles"]]
+    n = int(g["n"])
+    o = O.OracleSolver(n, parts, tau=float(g["tau"]), subsamples=int(g["subsamples"]),
+                       body_force=tuple(float(v) for v in g["force"]), static_geometry=True, layout="sparse")
+    steps = 120
+    s.run(steps)
+    for _ in range(steps):
+        o.collide_stream_psc()
+    fields = (o.ux, o.uy, o.uz)
+    for axis in range(3):
+        got = s.fluid_statistics(axis=axis)
+        ref = O.fluid_statistics(o.solid, fields[axis], n)
+        print(f"  axis {axis}: gpu {got} oracle {ref}", flush=True)
+        scale = abs(O.fluid_statistics(o.solid, o.uz, n)[0])
+        for a, b in zip(got[:2], ref[:2]):
+            assert abs(a - b) <= 1e-12 * scale
+        assert got[2] == pytest.approx(ref[2], rel=1e-14)
+    assert s.fluid_statistics(axis=2)[0] == pytest.approx(O.fluid_weighted_mean(o.solid, o.uz), rel=1e-12)
+    assert s.permeability(axis=2) == pytest.approx(O.permeability(o.solid, o.uz, n, o.tau, o.body_force[2]),
+                                                   rel=1e-12)
+    assert s.mean_uz() == pytest.approx(float(np.mean(o.uz)), rel=1e-12)

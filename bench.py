@@ -96,8 +96,10 @@ def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=N
         sol.fuse_period = period
     sol.rebuild_geometry()
     sol._tune()   # launch-shape autotuning happens before the warm-up
+    # capture the graphs the timed run replays (no steps taken) before the warm-up, so that
+    # the timed run follows the warm-up with the GPU idle only for the synchronisation
+    sol.prepare_run(steps)
     sol.run(warmup)
-    sol.prepare_run(steps)   # capture the graphs the timed run replays (no steps taken)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)

@@ -14,9 +14,12 @@ native loop fuses the particle loads into the step kernels, candidates are
 timed in that loop (on scratch copies of the buffers): the extra load
 workgroups shift the best occupancy cap.  There each candidate is also timed
 with the steps in place in pairs (PSC_RUN_INPLACE, one state buffer): that
-pays where one state buffer fits the Infinity Cache and two do not.  For the
-chosen launch, tune_period then times the placement period of the load
-workgroups in that loop.
+pays where one state buffer fits the Infinity Cache and two do not.  The
+leading launches of that quick pass are then timed together with the
+placement period of the load workgroups, each pair through graph replays of
+LONG_STEPS steps as run() drives them, twice in interleaved rounds (the
+fastest time counts): runs that long repeat to about 0.05 %, while the
+leaders differ by 0.5-2 %.
 
 All candidates compute bit-identical results; only the launch differs.
 """
@@ -47,7 +50,7 @@ BPC = (0, 2, 3, 4, 6, 8)
 def candidates(nz: int):
     out = []
     for bz, by in SHAPES:
-        if bz > 64 and nz <= bz // 2:
+        if bz > 64 and nz < bz // 2:
             continue  # more than half the threads of a row would idle
         for nt in NT:
             for bpc in BPC:
@@ -65,9 +68,9 @@ def _cache_path():
                           os.path.join(os.path.expanduser("~"), ".cache", "psc_hip", "step_tuning.json"))
 
 
-# Part of every cache key: entries written when the candidate set was smaller
-# (before the flat mapping and the in-place step pairs) are not reused.
-CANDIDATE_SET = 4
+# Part of every cache key: entries chosen from an earlier candidate set or by an
+# earlier selection rule are not reused.
+CANDIDATE_SET = 6
 
 
 def _key(solver):
@@ -173,6 +176,8 @@ def time_fused_config(solver, scratch, bz, by, flags, repeats, period=None, grap
 
 PERIODS = (1, 3, 5, 7, 9)
 GRAPH_STEPS = 32  # steps per captured graph, as run() replays them
+LONG_STEPS = 8 * GRAPH_STEPS + 2  # steps of a leader's timed run (two head steps, then eight replays)
+LEADERS = 6  # launches of the quick pass timed again with every placement period
 
 
 def tune_period(solver, bz, by, flags, repeats: int = 120, use_cache: bool = True):
@@ -217,7 +222,7 @@ def tune(solver, budget_ms: float = 400.0, use_cache: bool = True, fused: bool =
         nz = solver._shape[2]
         if solver._storage == _lib.PSC_F64_PLAIN:
             # the FP64 step takes no launch flags: workgroup shapes only
-            cands = [(bz, by, 0, 0, False) for bz, by in SHAPES + FP64_SHAPES if not (bz > 64 and nz <= bz // 2)]
+            cands = [(bz, by, 0, 0, False) for bz, by in SHAPES + FP64_SHAPES if not (bz > 64 and nz < bz // 2)]
         else:
             cands = candidates(nz)
         # one probe launch sizes the repeat count to the time budget
@@ -228,7 +233,7 @@ def tune(solver, budget_ms: float = 400.0, use_cache: bool = True, fused: bool =
             reps = max(12, reps)  # the loop's two head steps and final flush are amortised
 
             def timer(bz, by, flags):
-                return time_fused_config(solver, scratch, bz, by, flags, reps)
+                return time_fused_config(solver, scratch, bz, by, flags, reps, period=1)
         else:
             def timer(bz, by, flags):
                 return time_config(solver, bz, by, flags, reps)
@@ -237,14 +242,26 @@ def tune(solver, budget_ms: float = 400.0, use_cache: bool = True, fused: bool =
             flags = flags_of(nt, bpc, 1, flat)
             for f in ((flags, flags | _lib.PSC_RUN_INPLACE) if fused else (flags,)):
                 results.append((timer(bz, by, f), bz, by, f))
-        # the leaders are often within 1% of each other: time them again, longer (the
-        # fused loop as run() drives it, through graph replays)
-        reps = max(4 * reps, 2 * GRAPH_STEPS + 2)
+        leaders = [(bz, by, f) for _, bz, by, f in sorted(results)[:LEADERS]]
         if fused:
-            def timer(bz, by, flags):
-                return time_fused_config(solver, scratch, bz, by, flags, reps, graph=True)
-        best = min((timer(bz, by, f), bz, by, f) for _, bz, by, f in sorted(results)[:6])
-        cfg = (best[1], best[2], best[3])
+            # each leader with every placement period, through long graph-replayed runs
+            best_t = {}
+            for _ in range(2):
+                for bz, by, f in leaders:
+                    for period in PERIODS:
+                        ms = time_fused_config(solver, scratch, bz, by, f, LONG_STEPS, period, graph=True)
+                        k = (bz, by, f, period)
+                        best_t[k] = min(best_t.get(k, ms), ms)
+            (bz, by, f, period), ms = min(best_t.items(), key=lambda kv: kv[1])
+            cfg = (bz, by, f)
+            pkey = _key(solver) + "|period|%d,%d,%d" % cfg
+            _MEM[pkey] = period
+            _store_file(pkey, [period, ms])
+        else:
+            # the leaders are often within 1% of each other: time them again, longer
+            reps = max(4 * reps, 2 * GRAPH_STEPS + 2)
+            ms, bz, by, f = min((time_config(solver, bz, by, f, reps), bz, by, f) for bz, by, f in leaders)
+            cfg = (bz, by, f)
         _MEM[key] = cfg
-        _store_file(key, list(cfg) + [best[0]])
+        _store_file(key, list(cfg) + [ms])
         return cfg

@@ -55,6 +55,8 @@ int64_t moments_blocks(int64_t);
 int launch_clear_entries(const uint32_t*, const int32_t*, int, int, int, int, int, uint32_t*, void*, int, hipStream_t);
 int launch_loads_chunks(const LoadsArgs&, const void*, int, int, int, int, double*, hipStream_t);
 int launch_loads_final(const int32_t*, int, const double*, double*, hipStream_t);
+int launch_loads_flush(const LoadsArgs&, const void*, int, int, int, int, double*, const double*, double*,
+                       hipStream_t);
 int launch_step_f32_loads(const float*, float*, const uint32_t*, float*, const float*, const LoadsArgs&, double*,
                           const double*, double*, int, int, int, int, int, double, double, double, double, int, int,
                           int, hipStream_t, int);
@@ -684,6 +686,12 @@ int flush_fused(const PscStepPlan& p, int par, bool pc, bool pf, hipStream_t s) 
   void* imbs[2] = {p.imb, p.imb_alt};
   double* prt[2] = {p.partials, p.partials_alt};
   int rc = PSC_OK;
+  if (pf && pc) {
+    // the previous step's per-particle pass and the last step's chunk pass
+    // share a launch, then the last step's per-particle pass
+    rc = launch_loads_flush(la, imbs[par ^ 1], 1, p.nx, p.ny, p.nz, prt[par ^ 1], prt[par], p.loads, s);
+    return rc ? rc : launch_loads_final(p.chunk_first, p.nparticles, prt[par ^ 1], p.loads, s);
+  }
   if (pf) rc = launch_loads_final(p.chunk_first, p.nparticles, prt[par], p.loads, s);
   if (rc == PSC_OK && pc) {
     rc = launch_loads_chunks(la, imbs[par ^ 1], 1, p.nx, p.ny, p.nz, prt[par ^ 1], s);

@@ -537,6 +537,24 @@ __global__ __launch_bounds__(PSC_LOADS_THREADS) void k_loads_chunk(LoadsArgs a, 
     loads_chunk_body<TI, PSC_LOADS_KB>(a, imb, d.nx, d.ny, d.nz, c, threadIdx.x, blockDim.x, partials, lds);
 }
 
+// The loads still pending at the end of a fused run, in one launch:
+// workgroup b < nchunks sums chunk b of the last step's imb into part_last,
+// workgroup nchunks + p adds up particle p's partials of the step before
+// (part_prev) into loads.  The two halves touch different buffers; the last
+// step's per-particle pass follows as k_loads_final.
+template <typename TI>
+__global__ __launch_bounds__(PSC_LOADS_THREADS) void k_loads_flush(LoadsArgs a, const TI* __restrict__ imb, Dims d,
+                                                                   double* __restrict__ part_last,
+                                                                   const double* __restrict__ part_prev,
+                                                                   double* __restrict__ loads) {
+  __shared__ double lds[6 * 4];
+  const int b = blockIdx.x;
+  if (b < a.nchunks)
+    loads_chunk_body<TI, PSC_LOADS_KB>(a, imb, d.nx, d.ny, d.nz, b, threadIdx.x, blockDim.x, part_last, lds);
+  else
+    loads_final_body(a.chunk_first, b - a.nchunks, threadIdx.x, part_prev, loads);
+}
+
 inline int loads_grid(int nchunks) {
   return (PSC_LOADS_GRID > 0 && PSC_LOADS_GRID < nchunks) ? PSC_LOADS_GRID : nchunks;
 }
@@ -836,6 +854,21 @@ int launch_loads_chunks(const LoadsArgs& la, const void* imb, int imbp, int nx, 
   if (imbp == 2) k_loads_chunk<double><<<loads_grid(la.nchunks), PSC_LOADS_THREADS, 0, stream>>>(la, (const double*)imb, d, partials);
   else k_loads_chunk<float><<<loads_grid(la.nchunks), PSC_LOADS_THREADS, 0, stream>>>(la, (const float*)imb, d, partials);
   return check_launch("psc_particle_loads(chunks)");
+}
+
+int launch_loads_flush(const LoadsArgs& la, const void* imb, int imbp, int nx, int ny, int nz, double* part_last,
+                       const double* part_prev, double* loads, hipStream_t stream) {
+  const long long nb = (long long)la.nchunks + la.np;
+  if (nb == 0) return PSC_OK;
+  if (nb > 0x7fffffffll) return set_error(PSC_EINVAL, "psc_run_steps: too many chunks and particles");
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
+  if (imbp == 2)
+    k_loads_flush<double><<<(int)nb, PSC_LOADS_THREADS, 0, stream>>>(la, (const double*)imb, d, part_last, part_prev,
+                                                                     loads);
+  else
+    k_loads_flush<float><<<(int)nb, PSC_LOADS_THREADS, 0, stream>>>(la, (const float*)imb, d, part_last, part_prev,
+                                                                    loads);
+  return check_launch("psc_run_steps(pending loads)");
 }
 
 int launch_loads_final(const int32_t* chunk_first, int np, const double* partials, double* loads, hipStream_t stream) {

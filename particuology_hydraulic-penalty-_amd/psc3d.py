@@ -34,7 +34,7 @@ from .geometry import build_tables, pack_particles
 
 PIPELINE_MIN_CELLS = 1 << 19  # unfused FP32 lattices from this size overlap the loads with the next step
 FUSE_MAX_CELLS = 1 << 23      # FP32 lattices below this size fuse the loads into the step kernels
-FUSE_PERIOD = 5               # fused loads: one load workgroup in every 5 at the head of the grid
+FUSE_PERIOD = 1               # fused loads (untuned): every load workgroup ahead of the cell workgroups
 MAX_F32_CELLS = 1 << 30        # FP32 kernels address a direction plane with 32-bit byte offsets
 MAX_CELLS = (1 << 31) - 1      # cell ids are 32-bit
 
@@ -295,6 +295,8 @@ class PscD3Q19:
         self._geometry_built = False
         self._step_count = 0
         self._cache = {}
+        self._plan_epoch = 0      # bumped whenever a buffer that run()'s plan holds is replaced
+        self._plan_cache = None
         _lib.call("psc_init_state", self._ptr(self._a), self._storage, N, self._stream())
         self.geometry_workers = 1
 
@@ -427,6 +429,7 @@ class PscD3Q19:
         self._geometry_built = True
         self._us_clean = True
         self._old_ent_cell = None
+        self._plan_epoch += 1
         self._cache.clear()
 
     # ------------------------------------------------------------------- step
@@ -467,13 +470,19 @@ class PscD3Q19:
 
     def _run_plan(self):
         """(plan, fused, pipelined) of the native loop, tuned and with its
-        buffers allocated."""
+        buffers allocated.  The ctypes plan is kept while nothing it holds
+        changed (buffers, launch, load settings), so a run starts with no
+        host work beyond the call."""
         if self._autotune:
             self._tune()
         fused, pipelined = self._load_modes()
         if pipelined or fused:
             self._ensure_pipeline_buffers()
-        return self._plan(pipelined, fused), fused, pipelined
+        key = (self._plan_epoch, id(self._a), fused, pipelined, self._block, self._flags, self.inplace,
+               self._inplace_tuned, self.effective_fuse_period())
+        if self._plan_cache is None or self._plan_cache[0] != key:
+            self._plan_cache = (key, self._plan(pipelined, fused))
+        return self._plan_cache[1], fused, pipelined
 
     def prepare_run(self, steps: int, graph_steps: int = 32) -> None:
         """Build the geometry, tune the launch and capture the hipGraphs that
@@ -514,9 +523,11 @@ class PscD3Q19:
         torch = _torch()
         if self._imb_bufs[1] is None:
             self._imb_bufs[1] = torch.zeros(3 * self._N, dtype=self._fdt, device=self.device)
+            self._plan_epoch += 1
         need = self._partials.numel()
         if self._partials_alt is None or self._partials_alt.numel() < need:
             self._partials_alt = torch.empty(need, dtype=torch.float64, device=self.device)
+            self._plan_epoch += 1
 
     def _plan(self, pipelined: bool = False, fused: bool = False):
         """PscStepPlan of the current buffers (state in self._a)."""

@@ -1,22 +1,34 @@
-"""Permeability grid-convergence post-processing (host side).
+"""Grid-convergence study of the permeability probes (host side).
 
-The reference's convergence campaign writes one ``<case>_nXXX.csv`` row per
-(case, resolution) permeability probe and merges them
-(scripts/merge_permeability_convergence.py:93-146): a raw table with the
-permeability of each case over the skeleton's at the same resolution, and a
-Richardson extrapolation per case from its three finest resolutions, with the
-observed order found by bisection of (h1^p - h2^p)/(h2^p - h3^p) = |e12/e23|
-on [0.1, 8] (80 halvings; order 2 when the differences vanish or change sign).
-This module produces the same two tables, with the same columns, from the
-same files (``permeability_probe`` in ``geometry_io`` writes them).
+A permeability probe at one resolution leaves one CSV file,
+``<case>_n<NNN>.csv``, holding a single data row (``geometry_io.write_probe``
+writes them; the reference's campaign writes the same files).  A
+``ConvergenceStudy`` loads a directory of such files and derives
+
+* the raw table: every probe row, ordered by case then resolution, with the
+  permeability over that of the skeleton case at the same resolution;
+* the Richardson table: per case, the extrapolation h -> 0 (h = 1/n) from its
+  three finest resolutions, with the observed order p solving
+  (h1^p - h2^p) / (h2^p - h3^p) = |(k1 - k2) / (k2 - k3)|,
+  and the extrapolated permeability over the skeleton's.
+
+Specification followed (scripts/merge_permeability_convergence.py:93-146):
+the two file names and their columns; the ratio is written as ``repr`` of
+the float; p is bracketed in [0.1, 8] and halved 80 times, and taken as 2
+when the two differences vanish or change sign; k_inf = (k3 h2^p - k2 h3^p)
+/ (h2^p - h3^p), or k3 when the denominator vanishes; a ratio that cannot be
+formed (no skeleton row, an unreadable or zero value) is left blank.  On the
+reference's own probe rows the tables come out identical to its merged
+tables (tests/test_io_convergence_cpu.py).
 """
 
 from __future__ import annotations
 
 import csv
-import re
+from dataclasses import dataclass, field
 from pathlib import Path
-from typing import Iterable
+import re
+from typing import Callable, Iterable, Sequence
 
 SUMMARY_FIELDS = [
     "case", "resolution", "steps", "tau", "force_z", "subsamples", "geometry_workers", "source_dump",
@@ -31,105 +43,166 @@ EXTRAP_FIELDS = [
     "permeability_extrapolated_lu", "fine_grid_permeability_lu", "relative_fine_to_extrapolated",
     "K_over_skeleton_extrapolated",
 ]
-_NAME = re.compile(r"^(?P<case>.+)_n(?P<n>\d{3})\.csv$")
+RAW_NAME = "permeability_convergence_raw.csv"
+RICHARDSON_NAME = "permeability_richardson.csv"
+REFERENCE_CASE = "skeleton"
+PROBE_FILE = re.compile(r"^(?P<case>.+)_n(?P<n>\d{3})\.csv$")
+
+ORDER_BRACKET = (0.1, 8.0)
+ORDER_HALVINGS = 80
+ORDER_FALLBACK = 2.0
+TINY = 1.0e-300
 
 
-def write_rows(path, fieldnames: Iterable[str], rows) -> None:
+def write_csv(path, columns: Iterable[str], rows: Iterable[dict]) -> None:
+    """A CSV table of ``rows`` restricted to ``columns`` (missing ones blank)."""
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
     with path.open("w", newline="") as fh:
-        w = csv.DictWriter(fh, fieldnames=list(fieldnames), extrasaction="ignore")
-        w.writeheader()
-        for r in rows:
-            w.writerow(r)
+        out = csv.DictWriter(fh, fieldnames=list(columns), extrasaction="ignore")
+        out.writeheader()
+        out.writerows(rows)
 
 
-def collect_rows(input_dir) -> list[dict]:
-    """The single data row of every ``<case>_nXXX.csv`` in ``input_dir``,
-    sorted by (case, resolution)."""
-    rows = []
-    for path in sorted(Path(input_dir).glob("*_n*.csv")):
-        if not _NAME.match(path.name):
-            continue
-        with path.open(newline="") as fh:
-            data = list(csv.DictReader(fh))
-        if len(data) != 1:
-            raise ValueError(f"{path} should contain exactly one data row")
-        rows.append(data[0])
-    return sorted(rows, key=lambda r: (r["case"], int(r["resolution"])))
+# Name kept for geometry_io, which writes single probe rows with it.
+write_rows = write_csv
 
 
-def add_raw_ratios(rows) -> None:
-    """K_over_skeleton_same_resolution = k / k(skeleton, same resolution),
-    blank where there is no skeleton row (or it cannot be formed)."""
-    skel = {r["resolution"]: float(r["permeability_lu"]) for r in rows if r.get("case") == "skeleton"}
-    for r in rows:
-        try:
-            r["K_over_skeleton_same_resolution"] = repr(float(r["permeability_lu"]) / skel[r["resolution"]])
-        except (KeyError, ValueError, ZeroDivisionError):
-            r["K_over_skeleton_same_resolution"] = ""
-
-
-def observed_order(h1, h2, h3, k1, k2, k3) -> float:
-    e12, e23 = k1 - k2, k2 - k3
-    if e12 == 0.0 or e23 == 0.0 or e12 * e23 <= 0.0:
-        return 2.0
-    target = abs(e12 / e23)
-    lo, hi = 0.1, 8.0
-    for _ in range(80):
-        p = 0.5 * (lo + hi)
-        if (h1 ** p - h2 ** p) / max(h2 ** p - h3 ** p, 1.0e-300) < target:
-            lo = p
+def bisect_rising(fn: Callable[[float], float], target: float, lo: float, hi: float, halvings: int) -> float:
+    """Midpoint of the bracket [lo, hi] after ``halvings`` halvings towards
+    fn(x) = target, for fn rising on the bracket (the lower half is kept
+    while fn(mid) >= target)."""
+    for _ in range(halvings):
+        mid = 0.5 * (lo + hi)
+        if fn(mid) < target:
+            lo = mid
         else:
-            hi = p
+            hi = mid
     return 0.5 * (lo + hi)
 
 
-def richardson_for_case(rows, case) -> dict:
-    """Extrapolation from the case's three finest resolutions (h = 1/n)."""
-    cr = sorted([r for r in rows if r.get("case") == case], key=lambda r: int(r["resolution"]))
-    if len(cr) < 3:
-        out = {k: "" for k in EXTRAP_FIELDS}
-        out["case"] = case
-        return out
-    n1, n2, n3 = (int(r["resolution"]) for r in cr[-3:])
-    k1, k2, k3 = (float(r["permeability_lu"]) for r in cr[-3:])
-    h1, h2, h3 = 1.0 / n1, 1.0 / n2, 1.0 / n3
-    p = observed_order(h1, h2, h3, k1, k2, k3)
-    den = h2 ** p - h3 ** p
-    k_inf = (k3 * h2 ** p - k2 * h3 ** p) / den if abs(den) > 1.0e-300 else k3
-    return {
-        "case": case, "coarse_resolution": n1, "mid_resolution": n2, "fine_resolution": n3,
-        "richardson_order": p, "permeability_extrapolated_lu": k_inf, "fine_grid_permeability_lu": k3,
-        "relative_fine_to_extrapolated": abs(k3 - k_inf) / max(abs(k_inf), 1.0e-300),
-        "K_over_skeleton_extrapolated": "",
-    }
+def observed_order(h1, h2, h3, k1, k2, k3) -> float:
+    """Convergence order p of k(h) from three spacings h1 > h2 > h3."""
+    d_coarse, d_fine = k1 - k2, k2 - k3
+    if d_coarse == 0.0 or d_fine == 0.0 or d_coarse * d_fine <= 0.0:
+        return ORDER_FALLBACK  # no monotone convergence to fit
+
+    def spacing_ratio(p):
+        return (h1 ** p - h2 ** p) / max(h2 ** p - h3 ** p, TINY)
+
+    return bisect_rising(spacing_ratio, abs(d_coarse / d_fine), *ORDER_BRACKET, ORDER_HALVINGS)
 
 
-def add_extrapolated_ratios(rows) -> None:
-    skel = next((r for r in rows if r["case"] == "skeleton"), None)
-    if not skel:
-        return
+@dataclass
+class Extrapolation:
+    """Richardson extrapolation of one case from resolutions n1 < n2 < n3."""
+
+    case: str
+    resolutions: tuple = ()
+    order: float | None = None
+    k_inf: float | None = None
+    k_fine: float | None = None
+    over_skeleton: float | str = ""
+
+    @classmethod
+    def from_points(cls, case: str, points: Sequence[tuple[int, float]]) -> "Extrapolation":
+        """``points``: (resolution, permeability) of the case, ascending; the
+        three finest are used, fewer give an empty row."""
+        if len(points) < 3:
+            return cls(case)
+        (n1, k1), (n2, k2), (n3, k3) = points[-3:]
+        h1, h2, h3 = 1.0 / n1, 1.0 / n2, 1.0 / n3
+        p = observed_order(h1, h2, h3, k1, k2, k3)
+        den = h2 ** p - h3 ** p
+        k_inf = (k3 * h2 ** p - k2 * h3 ** p) / den if abs(den) > TINY else k3
+        return cls(case, (n1, n2, n3), p, k_inf, k3)
+
+    def row(self) -> dict:
+        if self.order is None:
+            out = dict.fromkeys(EXTRAP_FIELDS, "")
+            out["case"] = self.case
+            return out
+        n1, n2, n3 = self.resolutions
+        return {
+            "case": self.case, "coarse_resolution": n1, "mid_resolution": n2, "fine_resolution": n3,
+            "richardson_order": self.order, "permeability_extrapolated_lu": self.k_inf,
+            "fine_grid_permeability_lu": self.k_fine,
+            "relative_fine_to_extrapolated": abs(self.k_fine - self.k_inf) / max(abs(self.k_inf), TINY),
+            "K_over_skeleton_extrapolated": self.over_skeleton,
+        }
+
+
+def _ratio(num, den):
+    """num / den as floats, or None when either cannot be read or den is 0."""
     try:
-        ks = float(skel["permeability_extrapolated_lu"])
-    except (TypeError, ValueError):
-        return
-    for r in rows:
-        try:
-            r["K_over_skeleton_extrapolated"] = float(r["permeability_extrapolated_lu"]) / ks
-        except (TypeError, ValueError, ZeroDivisionError):
-            r["K_over_skeleton_extrapolated"] = ""
+        return float(num) / float(den)
+    except (TypeError, ValueError, ZeroDivisionError):
+        return None
+
+
+@dataclass
+class ConvergenceStudy:
+    """The probe rows of one convergence campaign (dicts of CSV strings)."""
+
+    rows: list = field(default_factory=list)
+
+    @classmethod
+    def from_directory(cls, directory) -> "ConvergenceStudy":
+        rows = []
+        for path in sorted(Path(directory).glob("*_n*.csv")):
+            if PROBE_FILE.match(path.name) is None:
+                continue  # the merged tables and other files
+            with path.open(newline="") as fh:
+                data = list(csv.DictReader(fh))
+            if len(data) != 1:
+                raise ValueError(f"probe file {path} must hold one data row, it holds {len(data)}")
+            rows.append(data[0])
+        rows.sort(key=lambda r: (r["case"], int(r["resolution"])))
+        return cls(rows)
+
+    def cases(self) -> list:
+        return sorted({r["case"] for r in self.rows})
+
+    def points(self, case: str) -> list:
+        """(resolution, permeability) of a case, ascending in resolution."""
+        pts = [(int(r["resolution"]), float(r["permeability_lu"])) for r in self.rows if r.get("case") == case]
+        return sorted(pts, key=lambda t: t[0])
+
+    def raw_table(self) -> list:
+        """Probe rows with K_over_skeleton_same_resolution filled in."""
+        skeleton = {r["resolution"]: float(r["permeability_lu"]) for r in self.rows
+                    if r.get("case") == REFERENCE_CASE}
+        table = []
+        for r in self.rows:
+            q = _ratio(r["permeability_lu"], skeleton[r["resolution"]]) if r["resolution"] in skeleton else None
+            table.append({**r, "K_over_skeleton_same_resolution": "" if q is None else repr(q)})
+        return table
+
+    def richardson_table(self) -> list:
+        """One Extrapolation per case, with K_over_skeleton_extrapolated."""
+        ext = [Extrapolation.from_points(c, self.points(c)) for c in self.cases()]
+        skel = next((e for e in ext if e.case == REFERENCE_CASE), None)
+        if skel is not None and skel.k_inf is not None:
+            for e in ext:
+                q = _ratio(e.k_inf, skel.k_inf)
+                e.over_skeleton = "" if q is None else q
+        return ext
 
 
 def merge(input_dir, output_dir=None):
-    """Write ``permeability_convergence_raw.csv`` and
-    ``permeability_richardson.csv`` into ``output_dir`` (default: the input
-    directory); return (raw rows, extrapolation rows)."""
+    """Write the raw and Richardson tables of the probe files in
+    ``input_dir`` into ``output_dir`` (default: ``input_dir``); return
+    (raw rows, Richardson rows) as written."""
     out = Path(output_dir) if output_dir is not None else Path(input_dir)
-    rows = collect_rows(input_dir)
-    add_raw_ratios(rows)
-    write_rows(out / "permeability_convergence_raw.csv", SUMMARY_FIELDS, rows)
-    ext = [richardson_for_case(rows, c) for c in sorted({r["case"] for r in rows})]
-    add_extrapolated_ratios(ext)
-    write_rows(out / "permeability_richardson.csv", EXTRAP_FIELDS, ext)
-    return rows, ext
+    study = ConvergenceStudy.from_directory(input_dir)
+    raw = study.raw_table()
+    rich = [e.row() for e in study.richardson_table()]
+    write_csv(out / RAW_NAME, SUMMARY_FIELDS, raw)
+    write_csv(out / RICHARDSON_NAME, EXTRAP_FIELDS, rich)
+    return raw, rich
+
+
+def richardson_for_case(rows, case: str) -> dict:
+    """The Richardson row of ``case`` from probe rows (dicts with case,
+    resolution and permeability_lu)."""
+    return Extrapolation.from_points(case, ConvergenceStudy(list(rows)).points(case)).row()

@@ -1,10 +1,11 @@
 """Geometry input: LIGGGHTS dump readers and the permeability probe of a
 window (host side; the probe itself runs on the GPU solver).
 
-* ``parse_dump`` -- first ``ITEM: ATOMS`` section of a LIGGGHTS custom dump,
-  as rows (id, type, x, y, z, radius) (run_psc_permeability_resolution.py:44-68);
-* ``read_last_dump`` -- every ATOMS section of the file, columns from the
-  last header (analyze_local_fines_geometry.py:27-45);
+* ``atoms_frames`` -- the ATOMS sections of a LIGGGHTS custom dump;
+* ``parse_dump`` -- the first section as rows (id, type, x, y, z, radius)
+  (the probe driver's reader, run_psc_permeability_resolution.py:44-68);
+* ``read_last_dump`` -- the rows of every section of the file
+  (analyze_local_fines_geometry.py:27-45);
 * ``filter_particles`` ("all" or "type==1", the skeleton) and ``to_particles``
   / ``to_table`` (SI metres -> lattice units at a resolution, box 9 mm)
   (run_psc_permeability_resolution.py:71-82);
@@ -24,45 +25,52 @@ BOX_SIZE_M = 9.0e-3
 REQUIRED = ("id", "type", "x", "y", "z", "radius")
 
 
-def _rows_from(columns, rows, path):
-    idx = {name: i for i, name in enumerate(columns)}
-    missing = [c for c in REQUIRED if c not in idx]
+def atoms_frames(path) -> list:
+    """Every ``ITEM: ATOMS`` section of a LIGGGHTS/LAMMPS text dump, in file
+    order, as (column names, (rows, columns) float array).  A section holds
+    the non-blank lines up to the next ``ITEM:`` line."""
+    frames = []
+    names, body = None, []
+
+    def close():
+        if names is not None:
+            vals = np.array(" ".join(body).split(), dtype=float) if body else np.zeros(0)
+            frames.append((names, vals.reshape(-1, len(names)) if len(names) else vals.reshape(0, 0)))
+
+    for line in Path(path).read_text().splitlines():
+        if line.startswith("ITEM:"):
+            close()
+            names = tuple(line.split()[2:]) if line.startswith("ITEM: ATOMS") else None
+            body = []
+        elif names is not None and line.strip():
+            body.append(line)
+    close()
+    if not frames:
+        raise ValueError(f"No ITEM: ATOMS section in {path}")
+    return frames
+
+
+def _select(names, values, path) -> np.ndarray:
+    """The (id, type, x, y, z, radius) columns of one frame."""
+    missing = [c for c in REQUIRED if c not in names]
     if missing:
         raise ValueError(f"{path} missing dump columns: {missing}")
-    t = np.asarray(rows, dtype=float).reshape(-1, len(columns))
-    return np.column_stack([t[:, idx[c]] for c in REQUIRED]) if len(t) else np.zeros((0, 6))
+    return values[:, [names.index(c) for c in REQUIRED]] if len(values) else np.zeros((0, 6))
 
 
 def parse_dump(path) -> np.ndarray:
-    """(P, 6) rows of the first ATOMS section; id and type as integers."""
-    lines = Path(path).read_text().splitlines()
-    try:
-        head = next(i for i, ln in enumerate(lines) if ln.startswith("ITEM: ATOMS"))
-    except StopIteration:
-        raise ValueError(f"No ITEM: ATOMS section in {path}") from None
-    cols = lines[head].split()[2:]
-    rows = [[float(v) for v in ln.split()] for ln in lines[head + 1:] if ln.strip()]
-    out = _rows_from(cols, rows, path)
-    out[:, 0:2] = np.trunc(out[:, 0:2])  # int(float(...)) of the id and type fields
+    """(P, 6) rows (id, type, x, y, z, radius) of the first ATOMS section,
+    id and type truncated to integers as the probe driver reads them."""
+    names, values = atoms_frames(path)[0]
+    out = _select(names, values, path)
+    out[:, 0:2] = np.trunc(out[:, 0:2])
     return out
 
 
 def read_last_dump(path) -> np.ndarray:
-    """(P, 6) rows of every ATOMS section of the file (one frame per file
-    in the campaign's dumps)."""
-    rows, cols, inside = [], None, False
-    for ln in Path(path).read_text().splitlines():
-        if ln.startswith("ITEM: ATOMS"):
-            cols, inside = ln.split()[2:], True
-            continue
-        if ln.startswith("ITEM:"):
-            inside = False
-            continue
-        if inside:
-            rows.append([float(v) for v in ln.split()])
-    if cols is None:
-        raise ValueError(f"No ITEM: ATOMS section in {path}")
-    return _rows_from(cols, rows, path)
+    """(P, 6) rows of every ATOMS section of the file, one after the other
+    (the campaign's dumps hold one frame per file)."""
+    return np.concatenate([_select(names, values, path) for names, values in atoms_frames(path)])
 
 
 def filter_particles(rows, particle_filter: str = "all") -> np.ndarray:

@@ -99,5 +99,38 @@ def test_dump_readers(tmp_path):
     two = tmp_path / "two.dump"
     two.write_text(DUMP + DUMP)
     assert gio.read_last_dump(two).shape == (8, 6)  # every frame of the file
+    assert np.array_equal(gio.parse_dump(two), rows)  # the first frame only
+    gap = tmp_path / "gap.dump"
+    gap.write_text(DUMP.replace("3 2 0.0050", "\n3 2 0.0050"))
+    assert np.array_equal(gio.parse_dump(gap), rows)  # blank lines inside a section are skipped
     w = gio.windows_from_dumps([p], 32, "type==1")
     assert w[0][0] == 32 and w[0][1].shape == (2, 10)
+
+
+def test_study_edge_rules(tmp_path):
+    """A case without a skeleton row at its resolution gets a blank ratio; a
+    case with fewer than three resolutions gets a blank Richardson row; the
+    skeleton's own ratios are 1; a probe file must hold exactly one row."""
+    def probe(case, n, k):
+        cv.write_csv(tmp_path / f"{case}_n{n:03d}.csv", ["case", "resolution", "permeability_lu"],
+                     [{"case": case, "resolution": str(n), "permeability_lu": repr(k)}])
+    for n, k in ((32, 1.0), (48, 1.2), (64, 1.3)):
+        probe("skeleton", n, k)
+    probe("baseline", 32, 0.9)
+    probe("baseline", 96, 1.1)  # no skeleton at 96
+    raw, rich = cv.merge(tmp_path)
+    by = {(r["case"], r["resolution"]): r["K_over_skeleton_same_resolution"] for r in raw}
+    assert by[("baseline", "32")] == repr(0.9) and by[("baseline", "96")] == ""
+    assert by[("skeleton", "48")] == repr(1.0)
+    row = {r["case"]: r for r in rich}
+    assert all(v == "" for k, v in row["baseline"].items() if k != "case")
+    assert row["skeleton"]["K_over_skeleton_extrapolated"] == 1.0
+    assert row["skeleton"]["fine_resolution"] == 64
+    (tmp_path / "broken_n128.csv").write_text("case,resolution,permeability_lu\nx,128,1.0\nx,128,2.0\n")
+    with pytest.raises(ValueError, match="one data row"):
+        cv.merge(tmp_path)
+
+
+def test_bisection_helper_brackets_the_root():
+    got = cv.bisect_rising(lambda x: x ** 3, 2.0, 0.0, 4.0, 60)
+    assert got == pytest.approx(2.0 ** (1 / 3), abs=1e-15)

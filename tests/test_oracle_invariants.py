@@ -239,3 +239,129 @@ def test_published_permeability_reproduced_from_its_csv_columns():
 
 def test_hasimoto_small_lambda():
     assert O.hasimoto_factor(1e-6, 1.0) == pytest.approx(1.0 + 2.837297e-6, rel=1e-9)
+
+
+# ------------------------------------------------------- periodic geometry mode
+def _periodic_run(centre, shift, layout, steps=3, n=16):
+    """A rotating, translating sphere in periodic mode with a body force,
+    started from a seeded perturbed state; the whole system is translated by
+    the integer vector ``shift`` (centre and initial state)."""
+    rng = np.random.default_rng(8)
+    f0 = O.equilibrium(np.ones((n,) * 3), *np.zeros((3,) + (n,) * 3)) + 1e-5 * rng.standard_normal((19,) + (n,) * 3)
+    c = np.mod(np.asarray(centre) + np.asarray(shift), n)
+    p = O.Sphere(*c, 3.3, vx=2e-4, vy=-1e-4, vz=1.5e-4, wx=3e-5, wy=-2e-5, wz=4e-5)
+    s = O.OracleSolver(n, [p], tau=0.8, subsamples=3, body_force=(0.0, 2e-6, 1e-6), layout=layout,
+                       periodic_geometry=True)
+    s.f = np.roll(f0, tuple(int(v) for v in shift), axis=(1, 2, 3))
+    for _ in range(steps):
+        s.collide_stream_psc()
+    return s
+
+
+@pytest.mark.parametrize("layout", ["sparse", "dense"])
+@pytest.mark.parametrize("shift", [(9, 0, 0), (0, -8, 0), (3, 5, 9)])
+def test_periodic_mode_is_translation_invariant(layout, shift):
+    """Periodic geometry: translating the system by whole cells so that the
+    sphere straddles the boundary rolls alpha and f with it and leaves the
+    force and torque unchanged.  A box wrapped wrongly changes alpha; using
+    the wrapped instead of the unwrapped cell position as lever arm changes
+    the solid velocity (hence f) and the torque by O(n)."""
+    centre = (8.31, 7.62, 8.93)
+    a = _periodic_run(centre, (0, 0, 0), layout)
+    b = _periodic_run(centre, shift, layout)
+    p = b.particles[0]
+    lo = O.sphere_box(p, b.n, periodic=True)
+    assert min(lo[0], lo[2], lo[4]) < 0 or max(lo[1], lo[3], lo[5]) > b.n  # the shifted box wraps
+    sh = tuple(int(v) for v in shift)
+    assert np.array_equal(b.solid, np.roll(a.solid, sh, axis=(0, 1, 2)))
+    np.testing.assert_allclose(b.f, np.roll(a.f, sh, axis=(1, 2, 3)), rtol=0, atol=1e-17)
+    scale = np.max(np.abs(a.hydro_forces))
+    assert scale > 1e-9
+    np.testing.assert_allclose(b.hydro_forces, a.hydro_forces, rtol=0, atol=1e-11 * scale)
+    np.testing.assert_allclose(b.hydro_torques, a.hydro_torques, rtol=0, atol=1e-11 * scale * 4.3)
+    assert b.momentum_residual <= 1e-12 * scale
+
+
+def test_periodic_solid_velocity_uses_the_unwrapped_lever_arm():
+    """u_s = U + omega x r on every covered cell of a sphere that wraps around
+    the x and z faces, with r the offset of the cell's unwrapped position
+    (the wrapped one differs by n)."""
+    n = 12
+    p = O.Sphere(0.7, 6.2, 11.4, 2.9, vx=1e-4, vy=-2e-4, vz=3e-4, wx=1e-5, wy=2e-5, wz=-3e-5)
+    for layout in ("sparse", "dense"):
+        s = O.OracleSolver(n, [p], subsamples=2, layout=layout, periodic_geometry=True)
+        s.rebuild_geometry()
+        us = s.solid_velocity()
+        X, Y, Z = np.indices((n,) * 3) + 0.5
+        r = [X - p.x, Y - p.y, Z - p.z]
+        r = np.stack([v - n * np.round(v / n) for v in r])  # nearest image of the cell centre
+        w = np.array([p.wx, p.wy, p.wz])[:, None, None, None]
+        v = np.array([p.vx, p.vy, p.vz])[:, None, None, None] + np.cross(w, r, axis=0)
+        mask = s.solid > 0
+        assert mask[0].any() and mask[-1].any() and mask[:, :, 0].any()  # covers cells on both sides
+        for a in range(3):
+            np.testing.assert_allclose(us[a][mask], v[a][mask], rtol=1e-12, atol=1e-18)
+
+
+def test_periodic_dense_and_sparse_layouts_agree_on_wrapping_overlaps():
+    """The two layouts implement periodic mode separately (nearest-image
+    offsets over the full lattice vs unwrapped box coordinates): on
+    overlapping, rotating spheres that wrap, loads and state agree to
+    round-off, with the force closure of both."""
+    n = 14
+    parts = [O.Sphere(0.9, 7.1, 6.6, 3.1, vx=1e-4, wz=2e-5), O.Sphere(12.2, 7.4, 6.9, 2.6, vy=-1e-4, wx=-3e-5),
+             O.Sphere(6.3, 13.5, 0.4, 2.2, vz=2e-4)]
+    res = {}
+    for layout in ("sparse", "dense"):
+        s = O.OracleSolver(n, [O.Sphere(**vars(p)) for p in parts], tau=0.9, subsamples=3,
+                           body_force=(1e-6, 0, 0), layout=layout, periodic_geometry=True)
+        for _ in range(3):
+            s.collide_stream_psc()
+        res[layout] = s
+        scale = np.max(np.abs(s.hydro_forces))
+        assert s.momentum_residual <= 1e-12 * scale
+    a, b = res["sparse"], res["dense"]
+    assert np.max(a.alpha_sum) > 1.0  # the pair overlaps
+    np.testing.assert_allclose(a.f, b.f, rtol=0, atol=1e-16)
+    scale = np.max(np.abs(a.hydro_forces))
+    np.testing.assert_allclose(a.hydro_forces, b.hydro_forces, rtol=0, atol=1e-10 * scale)
+    np.testing.assert_allclose(a.hydro_torques, b.hydro_torques, rtol=0, atol=1e-10 * scale * 4)
+
+
+# ------------------------------------------------------------ readout closed forms
+def test_fluid_statistics_closed_forms():
+    """(intrinsic, superficial, porosity) of run_precision_validation.py:142-147:
+    a uniform velocity U gives intrinsic = U and superficial = porosity * U;
+    for any field superficial = porosity * intrinsic; solid is clipped to
+    [0, 1] first (overlaps above 1 count as fully solid)."""
+    rng = np.random.default_rng(9)
+    n = 6
+    solid = rng.uniform(-0.2, 1.4, (n,) * 3)
+    w = 1.0 - np.clip(solid, 0.0, 1.0)
+    por = float(w.sum()) / n ** 3
+    assert 0.2 < por < 0.8
+    U = 3.7e-5
+    i, s, p = O.fluid_statistics(solid, np.full((n,) * 3, U), n)
+    assert p == pytest.approx(por, rel=1e-15)
+    assert i == pytest.approx(U, rel=1e-14)
+    assert s == pytest.approx(por * U, rel=1e-14)
+    u = rng.standard_normal((n,) * 3)
+    i, s, p = O.fluid_statistics(solid, u, n)
+    assert s == pytest.approx(p * i, rel=1e-12)
+    assert i == pytest.approx(float((w * u).sum() / w.sum()), rel=1e-13)
+    assert O.fluid_weighted_mean(solid, u) == pytest.approx(i, rel=1e-14)
+    # an all-solid lattice: no fluid, the intrinsic mean is 0 (the reference's 1e-30 guard)
+    assert O.fluid_statistics(np.ones((n,) * 3), u, n) == (0.0, 0.0, 0.0)
+    assert O.fluid_weighted_mean(np.full((n,) * 3, 2.0), u) == 0.0
+
+
+def test_permeability_readout_closed_form():
+    """k = nu sum((1 - solid) u) / n^3 / F with nu = (tau - 1/2)/3
+    (run_psc_permeability_resolution.py:144-150): a uniform flow U through a
+    lattice without solids gives k = nu U / F; half-solid cells count half."""
+    n, tau, F, U = 5, 0.8, 2e-7, 4e-4
+    nu = (tau - 0.5) / 3.0
+    assert O.permeability(np.zeros((n,) * 3), np.full((n,) * 3, U), n, tau, F) == pytest.approx(nu * U / F,
+                                                                                                   rel=1e-14)
+    assert O.permeability(np.full((n,) * 3, 0.5), np.full((n,) * 3, U), n, tau, F) == pytest.approx(
+        0.5 * nu * U / F, rel=1e-14)

@@ -11,7 +11,7 @@ def test_candidates_and_flags():
     assert all(bz * by <= 256 and (bz * by) % 64 == 0 for bz, by, *_ in aligned)
     assert not any(bz > 64 and 128 < bz // 2 for bz, by, *_ in aligned)  # no mostly idle row tiles
     assert any(bz == 256 for bz, *_ in aligned)  # half-idle 256-wide tiles are timed (fastest at 128^3)
-    assert not any(bz == 256 for bz, *_ in at.candidates(96))
+    assert not any(c[0] == 256 and not c[4] for c in at.candidates(96))  # (z, y) tiles; flat ones are dense
     odd = at.candidates(136)
     flat = [c for c in odd if c[4]]
     assert {c[0] for c in flat} == set(at.FLAT) and all(c[1] == 1 for c in flat)

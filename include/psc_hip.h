@@ -277,6 +277,22 @@ int psc_moments(const void* d_state, int32_t storage, int32_t layout, const uint
 /* Number of partial-sum blocks psc_moments uses for N cells. */
 int64_t psc_moments_blocks(int64_t ncell);
 
+/* psc_moments without the d_partials workspace: it allocates nothing (no
+ * hipMalloc, no library allocation) and keeps the block partials in a
+ * library-owned device array of 2048 x PSC_NSUM doubles (208 KiB per GPU,
+ * held until the process ends), which every lattice fits (the block count
+ * is capped at 2048); a larger need is rejected with PSC_EINVAL.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under graph capture, serving loops, harnesses that forbid it).
+ * Every call on a GPU uses that GPU's one array and writes every partial
+ * it reads, so nothing has to start at zero; calls that overlap on one
+ * GPU (different streams or threads) race.  Same arguments, results (bit
+ * for bit) and errors as psc_moments otherwise. */
+int psc_moments_noalloc(const void* d_state, int32_t storage, int32_t layout, const uint32_t* d_count_sum,
+                        const void* d_imb, int32_t nx, int32_t ny, int32_t nz, int32_t pitch_z,
+                        int32_t subsamples, double fx, double fy, double fz, double* d_rho_u, double* d_sums,
+                        void* stream);
+
 /* Post-stream distributions f as FP64 (19 planes), the reference's `f`
  * attribute (psc3d.py:239-242): w + g for PSC_F32_DEV, gathered along c_i
  * for PSC_PULL. */

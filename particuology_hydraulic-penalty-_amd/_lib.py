@@ -50,6 +50,8 @@ SIGNATURES = {
                             _P, _P], ct.c_int),
     "psc_moments": ([_P, _I32, _I32, _P, _P, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _P, _P, _P, _P], ct.c_int),
     "psc_moments_blocks": ([_I64], ct.c_int64),
+    "psc_moments_noalloc": ([_P, _I32, _I32, _P, _P, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _P, _P, _P],
+                            ct.c_int),
     "psc_export_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_import_f64": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_run_steps": ([_P, _I32, _I32, _I32, _P, _P], ct.c_int),

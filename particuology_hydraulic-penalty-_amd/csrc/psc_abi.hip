@@ -52,6 +52,7 @@ int launch_loads(const double*, const int32_t*, const int32_t*, const int32_t*, 
 int launch_moments(const void*, int, int, const uint32_t*, const void*, int, int, int, int, unsigned long long, int,
                    double, double, double, double*, double*, double*, hipStream_t);
 int64_t moments_blocks(int64_t);
+int moments_scratch(double**, int64_t*);
 int launch_clear_entries(const uint32_t*, const int32_t*, int, int, int, int, int, uint32_t*, void*, int, hipStream_t);
 int launch_loads_chunks(const LoadsArgs&, const void*, int, int, int, int, double*, hipStream_t);
 int launch_loads_final(const int32_t*, int, const double*, double*, hipStream_t);
@@ -355,25 +356,66 @@ int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const 
 
 int64_t psc_moments_blocks(int64_t ncell) { return moments_blocks(ncell); }
 
+}  // extern "C"
+
+namespace psc {
+namespace {
+// Argument checks shared by psc_moments and psc_moments_noalloc; resolves the
+// row pitch and plane size.
+int moments_args(const char* who, const void* d_state, int32_t storage, int32_t layout, const uint32_t* d_count_sum,
+                 int32_t nx, int32_t ny, int32_t nz, int32_t pitch_z, int32_t subsamples, const double* d_rho_u,
+                 const double* d_sums, int* pz, unsigned long long* ps) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "%s: bad lattice", who);
+  if (storage != PSC_F32_DEV && storage != PSC_F64_PLAIN) return set_error(PSC_EINVAL, "%s: storage", who);
+  if (layout != PSC_PUSH && layout != PSC_PULL) return set_error(PSC_EINVAL, "%s: layout", who);
+  if (d_count_sum == nullptr || d_sums == nullptr) return set_error(PSC_EINVAL, "%s: null", who);
+  if (d_state == nullptr && d_rho_u != nullptr) return set_error(PSC_EINVAL, "%s: fields need a state", who);
+  if (subsamples < 0 || subsamples * subsamples * subsamples > 65535)
+    return set_error(PSC_EINVAL, "%s: subsamples %d", who, subsamples);
+  *pz = pitch_z > 0 ? pitch_z : nz;
+  if (*pz < nz || !dims_ok(nx, ny, *pz)) return set_error(PSC_EINVAL, "%s: pitch_z %d < nz %d", who, pitch_z, nz);
+  if (pitch_z > 0 && d_rho_u != nullptr) return set_error(PSC_EINVAL, "%s: fields need an unpadded lattice", who);
+  // explicit pitch: planes of nx*ny*pitch_z cells rounded up to 64 (batched-window layout)
+  const unsigned long long n = (unsigned long long)nx * ny * *pz;
+  *ps = pitch_z > 0 ? (n + 63ull) / 64ull * 64ull : n;
+  return PSC_OK;
+}
+}  // namespace
+}  // namespace psc
+
+extern "C" {
+
 int psc_moments(const void* d_state, int32_t storage, int32_t layout, const uint32_t* d_count_sum, const void* d_imb,
                 int32_t nx, int32_t ny, int32_t nz, int32_t pitch_z, int32_t subsamples, double fx, double fy, double fz,
                 double* d_rho_u, double* d_partials, double* d_sums, void* stream) {
-  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_moments: bad lattice");
-  if (storage != PSC_F32_DEV && storage != PSC_F64_PLAIN) return set_error(PSC_EINVAL, "psc_moments: storage");
-  if (layout != PSC_PUSH && layout != PSC_PULL) return set_error(PSC_EINVAL, "psc_moments: layout");
-  if (d_count_sum == nullptr || d_partials == nullptr || d_sums == nullptr)
-    return set_error(PSC_EINVAL, "psc_moments: null");
-  if (d_state == nullptr && d_rho_u != nullptr) return set_error(PSC_EINVAL, "psc_moments: fields need a state");
-  if (subsamples < 0 || subsamples * subsamples * subsamples > 65535)
-    return set_error(PSC_EINVAL, "psc_moments: subsamples %d", subsamples);
-  const int pz = pitch_z > 0 ? pitch_z : nz;
-  if (pz < nz || !dims_ok(nx, ny, pz)) return set_error(PSC_EINVAL, "psc_moments: pitch_z %d < nz %d", pitch_z, nz);
-  if (pitch_z > 0 && d_rho_u != nullptr) return set_error(PSC_EINVAL, "psc_moments: fields need an unpadded lattice");
-  // explicit pitch: planes of nx*ny*pitch_z cells rounded up to 64 (batched-window layout)
-  const unsigned long long n = (unsigned long long)nx * ny * pz;
-  const unsigned long long ps = pitch_z > 0 ? (n + 63ull) / 64ull * 64ull : n;
+  int pz = 0;
+  unsigned long long ps = 0;
+  int rc = moments_args("psc_moments", d_state, storage, layout, d_count_sum, nx, ny, nz, pitch_z, subsamples,
+                        d_rho_u, d_sums, &pz, &ps);
+  if (rc) return rc;
+  if (d_partials == nullptr) return set_error(PSC_EINVAL, "psc_moments: null");
   return launch_moments(d_state, storage, layout, d_count_sum, d_imb, nx, ny, nz, pz, ps, subsamples, fx, fy, fz,
                         d_rho_u, d_partials, d_sums, (hipStream_t)stream);
+}
+
+int psc_moments_noalloc(const void* d_state, int32_t storage, int32_t layout, const uint32_t* d_count_sum,
+                        const void* d_imb, int32_t nx, int32_t ny, int32_t nz, int32_t pitch_z, int32_t subsamples,
+                        double fx, double fy, double fz, double* d_rho_u, double* d_sums, void* stream) {
+  int pz = 0;
+  unsigned long long ps = 0;
+  int rc = moments_args("psc_moments_noalloc", d_state, storage, layout, d_count_sum, nx, ny, nz, pitch_z, subsamples,
+                        d_rho_u, d_sums, &pz, &ps);
+  if (rc) return rc;
+  double* partials = nullptr;
+  int64_t capacity = 0;
+  rc = moments_scratch(&partials, &capacity);
+  if (rc) return rc;
+  const int64_t need = moments_blocks((int64_t)((unsigned long long)nx * ny * pz)) * PSC_NSUM;
+  if (need > capacity)
+    return set_error(PSC_EINVAL, "psc_moments_noalloc: %lld partial sums exceed the %lld of the scratch array",
+                     (long long)need, (long long)capacity);
+  return launch_moments(d_state, storage, layout, d_count_sum, d_imb, nx, ny, nz, pz, ps, subsamples, fx, fy, fz,
+                        d_rho_u, partials, d_sums, (hipStream_t)stream);
 }
 
 int psc_export_f64(const void* d_state, int32_t storage, int32_t layout, int32_t nx, int32_t ny, int32_t nz,

@@ -4,6 +4,9 @@
 // that every multiply and add rounds separately, as NumPy does; the FP64 step
 // then reproduces src/lbm_dem/psc3d.py bit for bit on cells owned by one
 // particle.
+#include <mutex>
+#include <vector>
+
 #include "psc_common.h"
 #include "psc_loads.h"
 #include "../../include/psc_hip.h"
@@ -985,6 +988,42 @@ int launch_guo_f64(double fx, double fy, double fz, const double* ux, const doub
 int launch_init_plain(double* f, size_t N, hipStream_t stream) {
   k_init_plain<<<grid_for(N, 256), 256, 0, stream>>>(f, N);
   return check_launch("psc_init_state");
+}
+
+}  // namespace psc
+
+// Scratch memory of psc_moments_noalloc: the FP64 block partials of
+// k_moments, kMomMaxBlocks blocks x PSC_NSUM sums (208 KiB), which every
+// lattice fits (the block count is capped).  Each GPU holds its own copy
+// until the process ends; host code reaches it only through the device
+// address that moments_scratch looks up.
+static __device__ __align__(256) double scratch[psc::kMomMaxBlocks * PSC_NSUM];
+
+namespace psc {
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_dev;  // device address of `scratch` per device (nullptr: not looked up yet)
+}  // namespace
+
+// Device address and capacity (doubles) of the scratch array on the current
+// device, looked up once per device with that device current.
+int moments_scratch(double** out, int64_t* capacity) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error((int)e, "psc_moments_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize(dev + 1, nullptr);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+      return set_error(e != hipSuccess ? (int)e : PSC_EINVAL, "psc_moments_noalloc: hipGetSymbolAddress: %s",
+                       hipGetErrorString(e));
+    g_scratch_dev[dev] = (double*)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  *capacity = kMomMaxBlocks * PSC_NSUM;
+  return PSC_OK;
 }
 
 }  // namespace psc

@@ -110,7 +110,13 @@ __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, 
   mx = 0.f;
   my = 0.f;
   mz = 0.f;
-  if (cnt == 0) {
+  // The branch is taken per wavefront: only a wavefront of fluid cells takes
+  // the fluid form; any solid cell sends the whole wavefront through the
+  // solid form, whose B = 0 case reproduces the fluid form bit for bit (the
+  // relaxation factors reduce exactly to 1 - 1/tau and 1/tau and the solid
+  // term to an exact 0), so the bits of a cell do not depend on its
+  // neighbours in the wavefront and a mixed wavefront runs one form, not both.
+  if (!__any(cnt != 0)) {
     // fluid cell: g' = (1 - 1/tau) g + geq/tau + Guo, pairwise even/odd parts
     {
       const float geq0 = wdr(0) - wrh(0) * usq15;
@@ -133,11 +139,14 @@ __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, 
       gn[o] = fmaf(g[o], k.omt, P - M);
     }
   } else {
+    // B = 0 in fluid cells (a = 0): then om1 = 1/tau, om0 = 1 - 1/tau and
+    // every B * (...) is 0, and the expressions below are the fluid form's
     const float a = k.field ? __uint_as_float(cnt) : fminf((float)cnt * k.inv_s3, 1.0f);
     const float B = a / ((a + k.tau * (1.0f - a)) + 1.0e-14f);
-    const float omb = (1.0f - B) * k.itau;
+    const float om1 = (1.0f - B) * k.itau;  // (1 - B)/tau: weight of geq
+    const float om0 = fmaf(B, k.itau, k.omt);  // 1 - (1 - B)/tau: weight of g
     float sx = 0.f, sy = 0.f, sz = 0.f, ssq15 = 0.f;
-    if (MOVING) {
+    if (MOVING && cnt != 0) {
       const uint32_t c = cell;
       const float ia = 1.0f / ((float)cnt * k.inv_s3);
       sx = usn[c] * ia;
@@ -150,7 +159,7 @@ __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, 
       const float geqs0 = MOVING ? wdr(0) - wrh(0) * ssq15 : wdr(0);
       const float gu0 = -pw(0) * uF3;
       const float os0 = B * (geqs0 - geq0);
-      gn[0] = g[0] + (geq0 - g[0]) * omb + os0 + gu0;
+      gn[0] = fmaf(g[0], om0, fmaf(geq0, om1, gu0)) + os0;
     }
 #pragma unroll
     for (int i = 1; i < Q; i += 2) {
@@ -171,8 +180,10 @@ __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, 
       const float geq_i = S + A, geq_o = S - A;
       const float os_i = B * ((g[o] - g[i]) + ((Ss + As) - geq_o));
       const float os_o = B * ((g[i] - g[o]) + ((Ss - As) - geq_i));
-      gn[i] = g[i] + (geq_i - g[i]) * omb + os_i + (Gs + Ga);
-      gn[o] = g[o] + (geq_o - g[o]) * omb + os_o + (Gs - Ga);
+      const float P = fmaf(S, om1, Gs);
+      const float M = fmaf(A, om1, Ga);
+      gn[i] = fmaf(g[i], om0, P + M) + os_i;
+      gn[o] = fmaf(g[o], om0, P - M) + os_o;
       const float dm = os_i - os_o;  // c_i os_i + c_o os_o
       if (cx_of(i)) mx += (float)cx_of(i) * dm;
       if (cy_of(i)) my += (float)cy_of(i) * dm;

@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--launch", default=None,
                     help="fixed psc_step launch 'block_z,block_y,flags' instead of autotuning (for profiling); "
                          "flags may add PSC_RUN_INPLACE (0x200000) for in-place step pairs")
+    ap.add_argument("--batch-launch", default=None,
+                    help="windows workload: fixed launch 'flags,grid,mode' (mode spans|rows) instead of autotuning")
     ap.add_argument("--fuse-period", type=int, default=None,
                     help="placement period of the fused load workgroups (default: tuned; with --launch: 5)")
     ap.add_argument("--no-copy", action="store_true")
@@ -68,6 +70,9 @@ def parse():
                     help="fixed_array: 128^3 permeability (default); moving: 512^3 window, 2000 moving spheres; "
                          "windows: 64 batched pore windows; adjoint: 96^3 permeability gradient; "
                          "bounce_back: 256^3 plain bounce-back window; coupling: one exchange's flow solve")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes that measure roofline.traffic")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -282,6 +287,82 @@ def read_traffic(name, launch):
     return traffic if measured == tuple(int(v) for v in launch) else None
 
 
+COPY_PROBE_BYTES = 256 << 20  # each buffer of the calibration copy in the traffic probe
+
+
+def traffic_probe(args, psc3d, workloads, lib, torch):
+    """Child of measure_traffic, run under rocprofv3 --pmc: the timed
+    configuration with the launch fixed (--launch, --fuse-period) for
+    12 direct steps, then the streaming copy kernel on 256 MiB buffers
+    (read 256 MiB, write 256 MiB per dispatch) as the counter calibration."""
+    n = args.n
+    bz, by, flags = (int(v) for v in args.launch.split(","))
+    sol = psc3d.PscD3Q19(n, workloads.fixed_array(seed=0, n=n), tau=0.8, subsamples=4, body_force=(1.0e-6, 0.0, 0.0),
+                         static_geometry=True, precision="fp32", stream_layout=args.layout, device="cuda:0",
+                         block=(bz, by))
+    sol._flags = flags & ~lib.PSC_RUN_INPLACE
+    sol.inplace = bool(flags & lib.PSC_RUN_INPLACE)
+    sol.fuse_period = args.fuse_period or 1
+    sol.rebuild_geometry()
+    sol.run(12, graph=False)
+    torch.cuda.synchronize()
+    m = COPY_PROBE_BYTES // 4
+    src = torch.ones(m, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        lib.call("psc_copy_f32", src.data_ptr(), dst.data_ptr(), m, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+
+
+def measure_traffic(args, launch, period, kernel_prefix, quiet):
+    """HBM traffic per launch of the timed step kernel: two rocprofv3 --pmc
+    passes over ``bench.py --traffic-probe`` with the same launch (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
+    reads for the copy kernel of known size in that run.  Returns a dict, or
+    None when rocprofv3 is missing or a pass fails."""
+    import csv
+    import shutil
+    import statistics
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None
+    out = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="psc_traffic_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "p", "--", sys.executable, "-u",
+               os.path.join(ROOT, "bench.py"), "--traffic-probe", "--n", str(args.n), "--layout", args.layout,
+               "--launch", ",".join(str(int(v)) for v in launch), "--fuse-period", str(int(period))]
+        log(f"traffic: rocprofv3 --pmc {counter} over the probe", quiet)
+        try:
+            subprocess.run(cmd, cwd=ROOT, env=env, timeout=180, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        except (subprocess.TimeoutExpired, OSError):
+            shutil.rmtree(d, ignore_errors=True)
+            return None
+        files = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+        step, copy = [], []
+        for f in files:
+            for row in csv.DictReader(open(f)):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row["Kernel_Name"]
+                if kernel_prefix in name:
+                    step.append(float(row["Counter_Value"]))
+                elif "k_copy4" in name:
+                    copy.append(float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        if len(step) < 8 or not copy:
+            return None
+        scale = COPY_PROBE_BYTES / (statistics.median(copy) * 1024.0)
+        kib = statistics.median(step[4:])  # steady state: after the two head steps of each form
+        out[counter] = {"kib_per_dispatch": kib, "copy_scale": scale, "bytes": kib * 1024.0 * scale}
+    out["bytes_per_launch"] = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    return out
+
+
 def bench_moving(args, psc3d, workloads, lib, torch, np):
     """Large moving-particle window: periodic n^3 (512 by default), 2000
     spheres r ~ U[6,18], |U|, |omega| r <= 1e-3, centres advanced by U every
@@ -382,8 +463,14 @@ def bench_windows(args, workloads, torch, np):
     warm = min(args.warmup, 20)
     log("generating 64 windows", args.quiet)
     wins = workloads.pebble_windows(seed=0)
+    launch = "auto"
+    if args.batch_launch:
+        f_, g_, m_ = args.batch_launch.split(",")
+        launch = (int(f_, 0), int(g_), m_)
     B = batch.BatchedPscD3Q19([(L, t) for L, t, _ in wins], tau=0.8, subsamples=3, body_force=(0.0, 0.0, 1.0e-6),
-                              device="cuda:0")
+                              device="cuda:0", launch=launch)
+    if args.fuse_period is not None:
+        B.fuse_period = args.fuse_period
     B.run(warm)
     stream = torch.cuda.current_stream()
     g0 = torch.cuda.Event(enable_timing=True)
@@ -726,6 +813,8 @@ def main():
     workloads = importlib.import_module(PKG + ".workloads")
     lib = importlib.import_module(PKG + "._lib")
     lib.load()
+    if args.traffic_probe:
+        return traffic_probe(args, psc3d, workloads, lib, torch)
     if args.workload == "moving":
         return bench_moving(args, psc3d, workloads, lib, torch, np)
     if args.workload == "windows":
@@ -828,21 +917,29 @@ def main():
         achieved = launch_bytes / (fused_ms * 1e-3) / 1e9
         run_launch = (sol32._block[0], sol32._block[1],
                       sol32._flags | (lib_flag("PSC_RUN_INPLACE") if inplace else 0), sol32.effective_fuse_period())
+        prefix = "k_step_f32_aa" if inplace else "k_step_f32_loads"
         if inplace:
-            roof_kernel, roof_us, traffic = ("k_step_f32_aa (static; step + loads of the two previous steps; "
-                                             "in-place pairs of a gather-scatter and a local step on one state "
-                                             "buffer)", fused_ms * 1e3,
-                                             read_traffic(f"k_step_f32_aa_{n}_pmc.txt", run_launch))
+            roof_kernel, roof_us = ("k_step_f32_aa (static; step + loads of the two previous steps; in-place pairs "
+                                    "of a gather-scatter and a local step on one state buffer)", fused_ms * 1e3)
         else:
-            roof_kernel, roof_us, traffic = ("k_step_f32_loads (pull, static; step + loads of the two previous "
-                                             "steps)", fused_ms * 1e3,
-                                             read_traffic(f"k_step_f32_loads_{n}_pmc.txt", run_launch))
+            roof_kernel, roof_us = ("k_step_f32_loads (pull, static; step + loads of the two previous steps)",
+                                    fused_ms * 1e3)
+        traffic_detail = None
+        if not args.no_traffic:
+            traffic_detail = measure_traffic(args, run_launch[:3], run_launch[3], prefix, args.quiet)
+        if traffic_detail is not None:
+            traffic = traffic_detail["bytes_per_launch"]
+            traffic_detail["source"] = "measured in this run: rocprofv3 --pmc over bench.py --traffic-probe, same launch"
+        else:
+            traffic = read_traffic(f"{prefix}_{n}_pmc.txt", run_launch)
+            traffic_detail = {"source": "committed profile with the same launch" if traffic else "unmeasured"}
     else:
         launch_bytes = step_bytes
         achieved = launch_bytes / (kern_ms * 1e-3) / 1e9
         run_launch = (sol32._block[0], sol32._block[1], sol32._flags)
         roof_kernel, roof_us, traffic = ("k_step_f32 (pull, static)", kern_ms * 1e3,
                                          read_traffic(f"k_step_f32_{n}_pmc.txt", run_launch))
+        traffic_detail = {"source": "committed profile with the same launch" if traffic else "unmeasured"}
     out = {
         "metric": METRIC,
         "value": mlups,
@@ -888,6 +985,8 @@ def main():
             "step_kernel_alone_launch": list(kern_launch) if kern_launch else [sol32._block[0], sol32._block[1],
                                                                                sol32._flags],
             "step_kernel_alone_GBps": step_bytes / (kern_ms * 1e-3) / 1e9,
+            "traffic_detail": traffic_detail,
+            "traffic_over_algorithmic": (traffic / launch_bytes) if traffic else None,
         },
         "cpu_baseline": cpu,
         "hbm_only_check": hbm_check,

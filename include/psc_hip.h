@@ -358,12 +358,16 @@ typedef struct PscStepPlan {
  * have their usual meaning; the bits equal those of ordinary steps.  At
  * 128^3 one 160 MB state buffer stays resident in the 256 MB Infinity Cache
  * where two do not.
- * use_graph = 1 captures `graph_steps` steps (even, >= 2) once per plan into
- * a hipGraph and replays it as often as it fits, then one graph of the
- * largest even step count of the remainder (a run shorter than graph_steps
- * is that graph alone); at most one step runs as a direct launch besides the
- * two that start the fused form.  Graphs are cached per plan contents,
- * step count and starting parity until psc_graph_cache_clear(). */
+ * use_graph = 1 runs the steps as captured hipGraphs (graph_steps even,
+ * >= 2): a fused run of at most graph_steps steps is one graph, from its
+ * first step through the pending loads; a longer one is a graph of its first
+ * graph_steps steps, replays of a graph of graph_steps steps, and a graph of
+ * the remaining 1 .. graph_steps steps that ends with the pending loads.  An
+ * unfused run is replays of a graph_steps graph and one graph of the rest.
+ * Consecutive graph launches leave the GPU idle for a few microseconds, so
+ * longer graphs cost less.  Graphs are cached per plan contents, kind, step
+ * count and starting parity until psc_graph_cache_clear(); use_graph = 0
+ * launches the same kernels directly. */
 int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t use_graph, int32_t graph_steps,
                   int32_t* parity, void* stream);
 

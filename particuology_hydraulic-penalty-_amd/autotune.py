@@ -163,9 +163,8 @@ def time_fused_config(solver, scratch, bz, by, flags, repeats, period=None, grap
     _lib.check(lib.psc_run_steps(ct.byref(p), 4, 0, 16, ct.byref(parity), h), "psc_run_steps")
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    if graph:  # capture the graph outside the timed region
-        _lib.check(lib.psc_run_steps(ct.byref(p), 2 + GRAPH_STEPS, 1, GRAPH_STEPS, ct.byref(parity), h),
-                   "psc_run_steps")
+    if graph:  # capture the graphs of the timed run outside the timed region
+        _lib.check(lib.psc_prepare_steps(ct.byref(p), repeats, GRAPH_STEPS, ct.byref(parity)), "psc_prepare_steps")
     e0.record(stream)
     rc = lib.psc_run_steps(ct.byref(p), repeats, 1 if graph else 0, GRAPH_STEPS, ct.byref(parity), h)
     e1.record(stream)
@@ -176,7 +175,7 @@ def time_fused_config(solver, scratch, bz, by, flags, repeats, period=None, grap
 
 PERIODS = (1, 3, 5, 7, 9)
 GRAPH_STEPS = 32  # steps per captured graph, as run() replays them
-LONG_STEPS = 8 * GRAPH_STEPS + 2  # steps of a leader's timed run (two head steps, then eight replays)
+LONG_STEPS = 8 * GRAPH_STEPS + 2  # steps of a leader's timed run (its graphs: 32 steps, 7 x 32, 2)
 LEADERS = 6  # launches of the quick pass timed again with every placement period
 
 

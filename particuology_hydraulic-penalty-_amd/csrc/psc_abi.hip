@@ -557,7 +557,7 @@ struct GraphEntry {
   PscStepPlan plan;
   int steps;
   int start_parity;
-  int kind;  // 0: run_sequence; 1: fused steps continuing a sequence (both loads stages pending, no flush)
+  int kind;  // one of SegmentKind
   hipGraphExec_t exec;
 };
 std::mutex g_graph_mu;
@@ -796,6 +796,27 @@ int run_sequence(const PscStepPlan& p, int n, int* par, hipStream_t s) {
   return PSC_OK;
 }
 
+// The pieces a run is made of.  The fused kinds differ in whether the loads
+// of earlier steps are pending at the start (a fresh start has none) and in
+// whether the pending loads are reduced at the end (flush).
+enum SegmentKind {
+  kSeq = 0,          // run_sequence: unfused steps (loads in line or on the side stream)
+  kFusedFresh = 1,   // fused steps from a fresh start, no flush
+  kFusedCont = 2,    // fused steps continuing (both loads stages pending), no flush
+  kFusedTail = 3,    // fused steps continuing, then the pending loads
+  kFusedWhole = 4,   // fused steps from a fresh start, then the pending loads
+};
+
+// Run one segment on stream s (also what a captured graph of it holds).
+int exec_segment(const PscStepPlan& p, int kind, int steps, int* par, hipStream_t s) {
+  if (kind == kSeq) return run_sequence(p, steps, par, s);
+  const bool cont = kind == kFusedCont || kind == kFusedTail;
+  bool pc = cont, pf = cont;
+  const int rc = run_fused(p, steps, par, &pc, &pf, s);
+  if (rc || kind == kFusedFresh || kind == kFusedCont) return rc;
+  return flush_fused(p, *par, pc, pf, s);
+}
+
 int graph_for(const PscStepPlan& p, int steps, int parity, int kind, hipGraphExec_t* out) {
   std::lock_guard<std::mutex> lock(g_graph_mu);
   for (const GraphEntry& e : g_graphs) {
@@ -811,8 +832,7 @@ int graph_for(const PscStepPlan& p, int steps, int parity, int kind, hipGraphExe
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return set_error((int)e, "psc_run_steps: begin capture: %s", hipGetErrorString(e));
   int par = parity;
-  bool pc = true, pf = true;
-  const int rc = kind == 1 ? run_fused(p, steps, &par, &pc, &pf, s) : run_sequence(p, steps, &par, s);
+  const int rc = exec_segment(p, kind, steps, &par, s);
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(s, &graph);
   if (rc) {
@@ -839,33 +859,46 @@ int graph_for(const PscStepPlan& p, int steps, int parity, int kind, hipGraphExe
 namespace psc {
 namespace {
 
-// Steps per replayed graph for `avail` steps: graph_steps, or for a shorter
-// run the largest even count that fits (0: too short for a graph).
-int graph_size(int avail, int graph_steps) {
-  if (avail >= graph_steps) return graph_steps;
-  return avail >= 2 ? (avail & ~1) : 0;
-}
-
-// The graphs a run of nsteps replays, in order: `reps` replays of `g1` steps
-// starting at parity `par1`, then one replay of `g2` steps (0: none) at
-// parity `par2`.  A fused run first takes two direct steps (after them both
-// loads stages are pending); even graphs leave the parity unchanged.
-struct RunShape {
-  int head = 0, g1 = 0, reps = 0, g2 = 0, kind = 0;
+// A run as segments: `times` executions of `steps` steps of one kind, each
+// a replay of a cached graph (or direct launches without graphs).  With
+// graphs, a fused run of at most graph_steps steps is one graph from its
+// fresh start through the pending loads; a longer one is a fresh graph of
+// graph_steps steps, replays of a continuing graph of graph_steps steps and a
+// tail graph of the rest (1 .. graph_steps steps) that ends with the pending
+// loads.  Consecutive graph launches leave the GPU idle for a few
+// microseconds between them, so a run has as few as its length allows.  An
+// unfused run is replays of graph_steps steps, then one graph of the rest.
+struct Segment {
+  int kind, steps, times;
 };
 
-RunShape run_shape(const PscStepPlan& key, int nsteps, bool use_graph, int graph_steps) {
-  RunShape r;
+std::vector<Segment> run_segments(const PscStepPlan& key, int nsteps, bool use_graph, int graph_steps) {
+  std::vector<Segment> out;
+  if (nsteps <= 0) return out;
   const bool fused = fused_plan(key);
-  r.kind = fused ? 1 : 0;
-  r.head = fused ? (nsteps < 2 ? nsteps : 2) : 0;
-  if (!use_graph) return r;
-  const int avail = nsteps - r.head;
-  r.g1 = graph_size(avail, graph_steps);
-  if (r.g1 == 0) return r;
-  r.reps = avail / r.g1;
-  r.g2 = graph_size(avail - r.reps * r.g1, graph_steps);
-  return r;
+  if (!use_graph) {
+    out.push_back({fused ? kFusedWhole : kSeq, nsteps, 1});
+    return out;
+  }
+  const int G = graph_steps;
+  if (fused) {
+    if (nsteps <= G) {
+      out.push_back({kFusedWhole, nsteps, 1});
+      return out;
+    }
+    out.push_back({kFusedFresh, G, 1});
+    int reps = (nsteps - G) / G, tail = (nsteps - G) % G;
+    if (tail == 0) {
+      --reps;
+      tail = G;
+    }
+    if (reps > 0) out.push_back({kFusedCont, G, reps});
+    out.push_back({kFusedTail, tail, 1});
+    return out;
+  }
+  if (nsteps / G > 0) out.push_back({kSeq, G, nsteps / G});
+  if (nsteps % G) out.push_back({kSeq, nsteps % G, 1});
+  return out;
 }
 
 int check_run_plan(const PscStepPlan& key) {
@@ -916,33 +949,16 @@ extern "C" int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t us
   const PscStepPlan key = normalized(*plan);
   rc = check_run_plan(key);
   if (rc) return rc;
-  const RunShape r = run_shape(key, nsteps, use_graph != 0, graph_steps);
   int par = *parity;
-  bool pc = false, pf = false;
-  if (r.kind == 1) {
-    // two direct steps (after them both loads stages are pending), the graph
-    // replays, the remainder, then the pending loads
-    rc = run_fused(key, r.head, &par, &pc, &pf, s);
+  for (const Segment& g : run_segments(key, nsteps, use_graph != 0, graph_steps)) {
+    if (!use_graph) {
+      rc = exec_segment(key, g.kind, g.steps, &par, s);
+    } else {
+      rc = replay(key, g.steps, par, g.kind, g.times, s);
+      if (g.steps & g.times & 1) par ^= 1;
+    }
     if (rc) return rc;
   }
-  int done = r.head;
-  if (r.reps > 0) {
-    rc = replay(key, r.g1, par, r.kind, r.reps, s);
-    if (rc) return rc;
-    done += r.reps * r.g1;
-  }
-  if (r.g2 > 0) {
-    rc = replay(key, r.g2, par, r.kind, 1, s);
-    if (rc) return rc;
-    done += r.g2;
-  }
-  if (r.kind == 1) {
-    rc = run_fused(key, nsteps - done, &par, &pc, &pf, s);
-    if (rc == PSC_OK) rc = flush_fused(key, par, pc, pf, s);
-  } else if (done < nsteps) {
-    rc = run_sequence(key, nsteps - done, &par, s);
-  }
-  if (rc) return rc;
   *parity = par;
   return PSC_OK;
 }
@@ -954,15 +970,14 @@ extern "C" int psc_prepare_steps(const PscStepPlan* plan, int32_t nsteps, int32_
   const PscStepPlan key = normalized(*plan);
   rc = check_run_plan(key);
   if (rc) return rc;
-  const RunShape r = run_shape(key, nsteps, true, graph_steps);
-  // the head is even (or no graph follows it), so every graph starts at *parity
-  hipGraphExec_t exec = nullptr;
-  if (r.reps > 0) {
-    rc = graph_for(key, r.g1, *parity, r.kind, &exec);
+  int par = *parity;
+  for (const Segment& g : run_segments(key, nsteps, true, graph_steps)) {
+    hipGraphExec_t exec = nullptr;
+    rc = graph_for(key, g.steps, par, g.kind, &exec);
     if (rc) return rc;
+    if (g.steps & g.times & 1) par ^= 1;
   }
-  if (r.g2 > 0) rc = graph_for(key, r.g2, *parity, r.kind, &exec);
-  return rc;
+  return PSC_OK;
 }
 
 extern "C" int psc_graph_cache_clear(void) {

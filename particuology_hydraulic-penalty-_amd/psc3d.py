@@ -36,6 +36,7 @@ PIPELINE_MIN_CELLS = 1 << 19  # unfused FP32 lattices from this size overlap the
 FUSE_MAX_CELLS = 1 << 23      # FP32 lattices below this size fuse the loads into the step kernels
 FUSE_PERIOD = 1               # fused loads (untuned): every load workgroup ahead of the cell workgroups
 MAX_F32_CELLS = 1 << 30        # FP32 kernels address a direction plane with 32-bit byte offsets
+GRAPH_STEPS = 256              # steps per captured graph of run() (graph launches leave ~8 us gaps between them)
 MAX_CELLS = (1 << 31) - 1      # cell ids are 32-bit
 
 # Lattice constants (psc3d.py:17-43): identical values and order.
@@ -441,13 +442,13 @@ class PscD3Q19:
             d._fill()
         return d
 
-    def run(self, steps: int, graph: bool = True, graph_steps: int = 32) -> None:
+    def run(self, steps: int, graph: bool = True, graph_steps: int = GRAPH_STEPS) -> None:
         """``steps`` calls of collide_stream_psc without building diag dicts.
 
         With static geometry the loop runs natively (psc_run_steps): direct
-        launches, or captured hipGraphs of ``graph_steps`` steps (and of the
-        even part of the remainder, or of a shorter run) replayed.  With
-        dynamic geometry every step re-voxelizes from Python objects."""
+        launches, or captured hipGraphs of at most ``graph_steps`` steps
+        (a run that short is one graph).  With dynamic geometry every step
+        re-voxelizes from Python objects."""
         steps = int(steps)
         if steps <= 0:
             return
@@ -484,7 +485,7 @@ class PscD3Q19:
             self._plan_cache = (key, self._plan(pipelined, fused))
         return self._plan_cache[1], fused, pipelined
 
-    def prepare_run(self, steps: int, graph_steps: int = 32) -> None:
+    def prepare_run(self, steps: int, graph_steps: int = GRAPH_STEPS) -> None:
         """Build the geometry, tune the launch and capture the hipGraphs that
         ``run(steps, graph_steps=graph_steps)`` replays, without stepping
         (psc_prepare_steps), so that run() itself starts at once."""

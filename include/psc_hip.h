@@ -351,19 +351,20 @@ typedef struct PscStepPlan {
  * pending after the last step run after the loop.  Every step's loads are
  * computed, with the bits of the separate reduction, and d_loads ends with
  * the last step's.  PSC_RUN_INPLACE in flags (fused form only) runs the steps
- * in pairs on the state buffer of parity *parity, in place: a gather-scatter
- * step that leaves the post-stream populations of direction i in plane
- * opp(i), then a local step back to the pull form.  An odd last step is an
- * ordinary step to the other buffer, so on return *parity and the state
- * have their usual meaning; the bits equal those of ordinary steps.  At
+ * in pairs on one state buffer, in place: a gather-scatter step that leaves
+ * the post-stream populations of direction i in plane opp(i), then a local
+ * step back to the pull form.  An odd count starts with an ordinary step to
+ * the other buffer and pairs on that one, so on return *parity and the state
+ * have their usual meaning, and the next run continues on the buffer the
+ * cache holds; the bits equal those of ordinary steps.  At
  * 128^3 one 160 MB state buffer stays resident in the 256 MB Infinity Cache
  * where two do not.
  * use_graph = 1 runs the steps as captured hipGraphs (graph_steps even,
- * >= 2): a fused run of at most graph_steps steps is one graph, from its
- * first step through the pending loads; a longer one is a graph of its first
- * graph_steps steps, replays of a graph of graph_steps steps, and a graph of
- * the remaining 1 .. graph_steps steps that ends with the pending loads.  An
- * unfused run is replays of a graph_steps graph and one graph of the rest.
+ * >= 2): a fused run takes two direct steps (the GPU starts at once while
+ * the host submits the first graph), replays a graph of graph_steps steps as
+ * often as it fits and ends with a graph of the remaining 1 .. graph_steps
+ * steps that also reduces the pending loads.  An unfused run is replays of a
+ * graph_steps graph and one graph of the rest.
  * Consecutive graph launches leave the GPU idle for a few microseconds, so
  * longer graphs cost less.  Graphs are cached per plan contents, kind, step
  * count and starting parity until psc_graph_cache_clear(); use_graph = 0

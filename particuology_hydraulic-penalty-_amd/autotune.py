@@ -175,7 +175,7 @@ def time_fused_config(solver, scratch, bz, by, flags, repeats, period=None, grap
 
 PERIODS = (1, 3, 5, 7, 9)
 GRAPH_STEPS = 32  # steps per captured graph, as run() replays them
-LONG_STEPS = 8 * GRAPH_STEPS + 2  # steps of a leader's timed run (its graphs: 32 steps, 7 x 32, 2)
+LONG_STEPS = 8 * GRAPH_STEPS + 2  # steps of a leader's timed run (two direct steps, then graphs of 7 x 32 and 32)
 LEADERS = 6  # launches of the quick pass timed again with every placement period
 
 

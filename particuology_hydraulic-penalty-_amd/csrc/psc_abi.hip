@@ -689,8 +689,10 @@ int side_resources(SideRes** out) {
 // With PSC_RUN_INPLACE in the plan's flags, steps go in pairs that update the
 // state buffer of parity *par in place (a gather-scatter step, then a local
 // step; see step_cell): the imb and partials buffers still alternate every
-// step, and after a pair both parities agree again.  An odd last step is an
-// ordinary pull step from one buffer to the other.
+// step, and after a pair both parities agree again.  An odd count starts with
+// an ordinary pull step from one buffer to the other, so the pairs (and the
+// runs that follow) stay on the buffer the run ends in, which the Infinity
+// Cache then already holds.
 int run_fused(const PscStepPlan& p, int n, int* par, bool* pc, bool* pf, hipStream_t s) {
   const LoadsArgs la{p.particles, p.boxes, p.chunks, p.chunk_first, p.nparticles, p.nchunks, p.chunk_cells,
                      p.ent_cell, p.ent_loc, p.ent_cnt, p.chunk_nnz, p.count_sum};
@@ -703,8 +705,10 @@ int run_fused(const PscStepPlan& p, int n, int* par, bool* pc, bool* pf, hipStre
   const int flags = p.flags & ~PSC_RUN_INPLACE;
   for (int k = 0; k < n; ++k) {
     const int q = *par;
-    // in-place pair: steps k and k+1 (k even within this call, k+1 < n)
-    const int aa = !inplace ? 0 : ((k & 1) ? 2 : (k + 1 < n ? 1 : 0));
+    // in-place pairs: steps j and j+1 for even j = k - (n odd); the odd
+    // count's first step (j = -1) is an ordinary one
+    const int j = k - (n & 1);
+    const int aa = (!inplace || j < 0) ? 0 : ((j & 1) ? 2 : 1);
     const int sb = aa == 2 ? q ^ 1 : q;  // the pair's state buffer (the parity at its start)
     float* st_in = (float*)(sb ? p.state_b : p.state_a);
     float* st_out = aa ? st_in : (float*)(sb ? p.state_a : p.state_b);
@@ -859,45 +863,46 @@ int graph_for(const PscStepPlan& p, int steps, int parity, int kind, hipGraphExe
 namespace psc {
 namespace {
 
-// A run as segments: `times` executions of `steps` steps of one kind, each
-// a replay of a cached graph (or direct launches without graphs).  With
-// graphs, a fused run of at most graph_steps steps is one graph from its
-// fresh start through the pending loads; a longer one is a fresh graph of
-// graph_steps steps, replays of a continuing graph of graph_steps steps and a
-// tail graph of the rest (1 .. graph_steps steps) that ends with the pending
-// loads.  Consecutive graph launches leave the GPU idle for a few
-// microseconds between them, so a run has as few as its length allows.  An
-// unfused run is replays of graph_steps steps, then one graph of the rest.
+// A run as segments: `times` executions of `steps` steps of one kind, as
+// replays of a cached graph or as direct launches.  With graphs, a fused run
+// starts with two direct steps (the GPU starts on them at once, while the
+// host submits the first graph), then replays a continuing graph of
+// graph_steps steps as often as it fits and ends with one graph of the
+// remaining 1 .. graph_steps steps that also reduces the pending loads, so
+// no graph is followed by a direct launch.  Consecutive graph launches leave
+// the GPU idle for a few microseconds between them, so a run has as few as
+// its length allows.  An unfused run is replays of graph_steps steps, then
+// one graph of the rest.
 struct Segment {
   int kind, steps, times;
+  bool direct;
 };
 
 std::vector<Segment> run_segments(const PscStepPlan& key, int nsteps, bool use_graph, int graph_steps) {
   std::vector<Segment> out;
   if (nsteps <= 0) return out;
   const bool fused = fused_plan(key);
-  if (!use_graph) {
-    out.push_back({fused ? kFusedWhole : kSeq, nsteps, 1});
+  // fused: two direct steps, three for an odd count (its ordinary step
+  // first), so that every graph holds an even number of steps
+  const int head = 2 + (nsteps & 1);
+  if (!use_graph || (fused && nsteps <= head)) {
+    out.push_back({fused ? kFusedWhole : kSeq, nsteps, 1, true});
     return out;
   }
   const int G = graph_steps;
   if (fused) {
-    if (nsteps <= G) {
-      out.push_back({kFusedWhole, nsteps, 1});
-      return out;
-    }
-    out.push_back({kFusedFresh, G, 1});
-    int reps = (nsteps - G) / G, tail = (nsteps - G) % G;
+    out.push_back({kFusedFresh, head, 1, true});
+    int reps = (nsteps - head) / G, tail = (nsteps - head) % G;
     if (tail == 0) {
       --reps;
       tail = G;
     }
-    if (reps > 0) out.push_back({kFusedCont, G, reps});
-    out.push_back({kFusedTail, tail, 1});
+    if (reps > 0) out.push_back({kFusedCont, G, reps, false});
+    out.push_back({kFusedTail, tail, 1, false});
     return out;
   }
-  if (nsteps / G > 0) out.push_back({kSeq, G, nsteps / G});
-  if (nsteps % G) out.push_back({kSeq, nsteps % G, 1});
+  if (nsteps / G > 0) out.push_back({kSeq, G, nsteps / G, false});
+  if (nsteps % G) out.push_back({kSeq, nsteps % G, 1, false});
   return out;
 }
 
@@ -951,7 +956,7 @@ extern "C" int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t us
   if (rc) return rc;
   int par = *parity;
   for (const Segment& g : run_segments(key, nsteps, use_graph != 0, graph_steps)) {
-    if (!use_graph) {
+    if (g.direct) {
       rc = exec_segment(key, g.kind, g.steps, &par, s);
     } else {
       rc = replay(key, g.steps, par, g.kind, g.times, s);
@@ -972,9 +977,11 @@ extern "C" int psc_prepare_steps(const PscStepPlan* plan, int32_t nsteps, int32_
   if (rc) return rc;
   int par = *parity;
   for (const Segment& g : run_segments(key, nsteps, true, graph_steps)) {
-    hipGraphExec_t exec = nullptr;
-    rc = graph_for(key, g.steps, par, g.kind, &exec);
-    if (rc) return rc;
+    if (!g.direct) {
+      hipGraphExec_t exec = nullptr;
+      rc = graph_for(key, g.steps, par, g.kind, &exec);
+      if (rc) return rc;
+    }
     if (g.steps & g.times & 1) par ^= 1;
   }
   return PSC_OK;

@@ -154,7 +154,7 @@ def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, bl
     """psc_run_steps (direct launches and hipGraph replays, with eager reads in
     between the replays; load reduction fused into the next step's kernel, in
     line, or pipelined on the side stream; fused steps also in place in pairs
-    on one state buffer, with odd counts ending in an ordinary step; with
+    on one state buffer, with odd counts starting with an ordinary step; with
     64/128/192/256-thread workgroups) gives the same state and loads as
     stepping one call at a time."""
     g = G.load("perm24")

@@ -35,6 +35,17 @@ __host__ __device__ constexpr int opp_of(int i) { return i == 0 ? 0 : ((i & 1) ?
 __host__ __device__ constexpr double w_of(int i) {
   return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
 }
+// q = n / d, r = n % d for n < 2^24 from an FP32 reciprocal inv ~ 1/d,
+// corrected to the exact quotient (a few instructions instead of the
+// integer division sequence).
+__device__ __forceinline__ void divmod_f(uint32_t n, uint32_t d, float inv, uint32_t& q, uint32_t& r) {
+  q = (uint32_t)((float)n * inv);
+  int rr = (int)n - (int)(q * d);
+  while (rr < 0) { --q; rr += (int)d; }
+  while (rr >= (int)d) { ++q; rr -= (int)d; }
+  r = (uint32_t)rr;
+}
+
 constexpr double kCs2 = 1.0 / 3.0;
 constexpr double kCs2Sq = kCs2 * kCs2;
 

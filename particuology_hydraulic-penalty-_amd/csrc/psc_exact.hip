@@ -431,15 +431,28 @@ __global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ par
   const size_t N = (size_t)d.ps;
   const size_t out0 = (size_t)blockIdx.x * chunk_cells;
   int written = 0;
+  // box coordinates of entry e: FP32-reciprocal division (exact after
+  // correction) while the box is below 2^24 cells, integer division beyond
+  const bool fast = vol < (1 << 24);
+  const float inv_ez = 1.0f / (float)bx.ez, inv_ey = 1.0f / (float)bx.ey;
   for (int r = e0; r < e1; r += blockDim.x) {
     const int e = r + (int)threadIdx.x;
     int cnt = 0, lx = 0, ly = 0, lz = 0;
     uint32_t cell = 0;
     if (e < e1) {
-      lz = e % bx.ez;
-      const int t = e / bx.ez;
-      ly = t % bx.ey;
-      lx = t / bx.ey;
+      if (fast) {
+        uint32_t t, qz, qy, qx;
+        divmod_f((uint32_t)e, (uint32_t)bx.ez, inv_ez, t, qz);
+        divmod_f(t, (uint32_t)bx.ey, inv_ey, qx, qy);
+        lz = (int)qz;
+        ly = (int)qy;
+        lx = (int)qx;
+      } else {
+        lz = e % bx.ez;
+        const int t = e / bx.ez;
+        ly = t % bx.ey;
+        lx = t / bx.ey;
+      }
       const int gx = bx.x0 + lx, gy = bx.y0 + ly, gz = bx.z0 + lz;
       cnt = sample_count(gx, gy, gz, p, r2, s);
       if (cnt) {

@@ -264,14 +264,6 @@ __device__ __forceinline__ void step_cell(const float* gin, float* gout,
   }
 }
 
-// q = n / d, r = n % d for n < 2^24 from an FP32 reciprocal, corrected.
-__device__ __forceinline__ void divmod_f(uint32_t n, uint32_t d, float inv, uint32_t& q, uint32_t& r) {
-  q = (uint32_t)((float)n * inv);
-  int rr = (int)n - (int)(q * d);
-  while (rr < 0) { --q; rr += (int)d; }
-  while (rr >= (int)d) { ++q; rr -= (int)d; }
-  r = (uint32_t)rr;
-}
 
 // Grid: (z tiles, y tiles, x).  A thread updates CPT cells at consecutive y.
 // The dynamic LDS allocation is unused storage that caps the number of

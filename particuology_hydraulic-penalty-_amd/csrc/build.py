@@ -48,22 +48,32 @@ def hipcc() -> str:
     return cand
 
 
-def build(verbose: bool = False, out_dir: str | None = None) -> str:
-    """Compile every unit and link the shared library; return its path."""
+# Host-only AddressSanitizer and UndefinedBehaviorSanitizer instrumentation (each flag
+# applies to the host pass only; device code is built as usual), for the CPU checks of
+# the C ABI's argument handling (tests/test_abi_sanitize.py).
+HOST_SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-omit-frame-pointer", "-g"]
+
+
+def build(verbose: bool = False, out_dir: str | None = None, sanitize: bool = False,
+          lib_name: str = LIB_NAME) -> str:
+    """Compile every unit and link the shared library; return its path.
+    sanitize=True instruments the host code (a separate library for tests)."""
     cc = hipcc()
     out_dir = out_dir or PKG
-    target = os.path.join(out_dir, LIB_NAME)
+    target = os.path.join(out_dir, lib_name)
+    extra_all = HOST_SANITIZE if sanitize else []
     with tempfile.TemporaryDirectory(prefix="psc_build_") as tmp:
         objs = []
         for src, extra in UNITS:
             obj = os.path.join(tmp, src.replace(".hip", ".o"))
-            cmd = [cc, "-c", os.path.join(HERE, src), "-o", obj] + COMMON + extra
+            cmd = [cc, "-c", os.path.join(HERE, src), "-o", obj] + COMMON + extra + extra_all
             if verbose:
                 print(" ".join(cmd), flush=True)
             subprocess.run(cmd, check=True)
             objs.append(obj)
-        tmp_lib = os.path.join(tmp, LIB_NAME)
-        cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp_lib] + objs
+        tmp_lib = os.path.join(tmp, lib_name)
+        cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp_lib] + objs + extra_all
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

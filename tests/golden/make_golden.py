@@ -1,6 +1,6 @@
 """Write the golden fixtures of the PSC step from the reference implementation.
 
-Run in the build container only (the reference is not on the GPU box):
+Run where the reference sources are available (the GPU tests only read the fixtures):
 
     python tests/golden/make_golden.py [/path/to/reference]
 

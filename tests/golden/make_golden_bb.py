@@ -1,7 +1,7 @@
 """Write the golden fixtures of the plain bounce-back D3Q19 solver and its
 binary voxelizers from the reference implementation.
 
-Run in the build container only (the reference is not on the GPU box):
+Run where the reference sources are available (the GPU tests only read the fixtures):
 
     python tests/golden/make_golden_bb.py [/path/to/reference]
 

@@ -40,7 +40,7 @@ def driver(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("sanitize"))
     build.build(out_dir=out, sanitize=True, lib_name="libpsc_hip_sanitize.so")
     exe = os.path.join(out, "abi_checks")
-    # the driver is host C++ only; its sanitizer runtime carries the library's instrumentation
+    # the checking program is host C++ only; its sanitizer runtime serves the library's instrumentation
     cmd = [cxx, "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
            "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "sanitize", "abi_checks.cpp"),
            "-L", out, "-lpsc_hip_sanitize", f"-Wl,-rpath,{out}", "-o", exe]

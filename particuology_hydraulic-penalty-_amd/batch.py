@@ -33,7 +33,14 @@ from .geometry import build_tables, pack_particles
 
 MAX_WINDOW_CELLS = 1 << 22
 FUSE_PERIOD = 5  # fused loads: one load item in every 5 at the head of a launch's item sequence
-ROW_ALIGN = 4  # default z-row padding: a multiple of 4 cells (16-byte aligned rows)
+# Default z-row padding: none.  A padding cell is never written, so a row pitch
+# above the row length leaves every 64-byte memory granule that holds padding
+# partly written by the step's stores, and a partial granule costs HBM a
+# read-modify-write: empty windows of 137, 141, 101 and 75 cells per row
+# stream at 4.1-4.5 TB/s with rows padded to a multiple of 4 cells and at
+# 5.3-5.6 TB/s unpadded (spans of 256 cells, whose wavefront stores then
+# cover whole aligned 256-byte runs).
+ROW_ALIGN = 1
 PLANE_ALIGN = 64  # direction planes padded to a multiple of 64 cells (256-byte aligned planes)
 
 

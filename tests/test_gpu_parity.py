@@ -452,13 +452,14 @@ def test_batched_fused_loads_match_separate_reduction(launch, period):
 
 @pytest.mark.parametrize("launch", ["auto", (1, 0, "spans"), (1 | 0x40000, 0, "rows"), (0x200, 0, "rows"),
                                     (0x40000, 64, "spans"), (1 | 0x80000, 0, "spans")])
-def test_batched_windows_match_single_solvers(psc3d, launch):
+@pytest.mark.parametrize("row_align", [1, 32])
+def test_batched_windows_match_single_solvers(psc3d, launch, row_align):
     """Every window of a batch (uneven sizes, every workgroup size and item
-    mode, persistent grid, graph replays) equals a standalone FP32 solver of
-    that window bit for bit, state and loads, and the per-window
-    permeability matches."""
+    mode, persistent grid, graph replays, unpadded rows and rows padded to 32
+    cells) equals a standalone FP32 solver of that window bit for bit, state
+    and loads, and the per-window permeability matches."""
     batch = load_package("batch")
-    print(f"[batched windows] launch {launch}", flush=True)
+    print(f"[batched windows] launch {launch} row_align {row_align}", flush=True)
     rng = np.random.default_rng(31)
     wins = []
     for L in (20, 33, 48, 27):
@@ -466,7 +467,9 @@ def test_batched_windows_match_single_solvers(psc3d, launch):
         t[:, 0:3] = rng.uniform(0, L, (4, 3))
         t[:, 3] = rng.uniform(2.0, 4.5, 4)
         wins.append((L, t))
-    B = batch.BatchedPscD3Q19(wins, tau=0.8, subsamples=3, body_force=(0, 0, 2e-6), device="cuda:0", launch=launch)
+    B = batch.BatchedPscD3Q19(wins, tau=0.8, subsamples=3, body_force=(0, 0, 2e-6), device="cuda:0", launch=launch,
+                              row_align=row_align)
+    assert B.pitch == [-(-L // row_align) * row_align for L, _ in wins]
     B.run(13)
     singles = []
     for L, t in wins:

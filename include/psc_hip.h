@@ -125,7 +125,9 @@ int psc_guo_force(double fx, double fy, double fz, const double* d_ux, const dou
  * Box cells are wrapped into the lattice, so clamped boxes (the reference,
  * psc3d.py:73-78) and unclamped periodic boxes use the same call; lever
  * arms use the unwrapped cell position.  pitch_z (0 = nz) is the row pitch
- * of count_sum/us_num when rows are padded (batched windows).  Deterministic apart from the
+ * of count_sum/us_num when rows are padded (batched windows); N = nx*ny*pitch_z
+ * is the plane size of d_us_num, the planes psc_step and psc_clear_entries
+ * read.  Deterministic apart from the
  * order of the FP atomics into d_us_num on cells shared by two spheres. */
 int psc_voxelize(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
                  int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz,

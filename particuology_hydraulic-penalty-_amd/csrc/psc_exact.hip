@@ -830,7 +830,9 @@ int launch_voxelize(const double* parts, const int32_t* boxes, const int32_t* ch
                     int32_t* chunk_nnz, uint32_t* csum, void* usn, int usp, uint32_t cell_base,
                     hipStream_t stream) {
   if (nchunks == 0) return PSC_OK;
-  const Dims d{nx, ny, nz, pz, plane_cells(nx, ny, pz)};
+  // us_num: three planes of nx*ny*pz cells, the planes the step kernels and
+  // the incremental clear read (count_sum is a single plane)
+  const Dims d{nx, ny, nz, pz, (unsigned long long)nx * ny * pz};
   if (usn == nullptr || usp == 0)
     k_voxelize<0><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
                                                chunk_nnz, csum, nullptr, cell_base);

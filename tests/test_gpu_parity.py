@@ -376,20 +376,24 @@ def test_moving_periodic_fp64_matches_oracle(psc3d):
     assert np.max(np.abs(sol.hydro_torques - o.hydro_torques)) <= 1e-9 * scale * 8
 
 
-def test_incremental_clear_leaves_no_stale_counts(psc3d):
+@pytest.mark.parametrize("n", [40, 37])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_incremental_clear_leaves_no_stale_counts(psc3d, n, precision):
     """After many re-voxelizations of moving spheres, count_sum and the
-    solid-velocity numerator equal those of a fresh voxelization."""
+    solid-velocity numerator equal those of a fresh voxelization.  40^3 takes
+    the clear by whole 16-cell groups (16 | N), 37^3 the clear by cells."""
     rng = np.random.default_rng(22)
-    n = 40
+    print(f"[incremental clear] {n}^3 {precision}", flush=True)
     t = _moving_table(rng, n, 25, 2.0, 6.0, umax=0.3)   # large steps: boxes move by cells
-    a = psc3d.SparsePscD3Q19(n, t, subsamples=2, periodic_geometry=True, precision="fp32", device="cuda:0")
+    a = psc3d.SparsePscD3Q19(n, t, subsamples=2, periodic_geometry=True, precision=precision, device="cuda:0")
     for _ in range(12):
         t[:, 0:3] = np.mod(t[:, 0:3] + t[:, 4:7], n)
         a.particles = t
         a.collide_stream_psc()
-    b = psc3d.SparsePscD3Q19(n, t.copy(), subsamples=2, periodic_geometry=True, precision="fp32", device="cuda:0")
+    b = psc3d.SparsePscD3Q19(n, t.copy(), subsamples=2, periodic_geometry=True, precision=precision, device="cuda:0")
     b.rebuild_geometry()
     assert np.array_equal(a._csum.cpu().numpy(), b._csum.cpu().numpy())
+    assert np.count_nonzero(a._csum.cpu().numpy()) > 0.05 * n ** 3
     np.testing.assert_allclose(a._us.cpu().numpy(), b._us.cpu().numpy(), rtol=1e-6, atol=1e-12)
 
 

@@ -241,10 +241,13 @@ __device__ __forceinline__ void step_cell(const float* gin, float* gout,
   float gn[Q];
   float mx, my, mz;  // imb of a solid cell (0 in fluid)
   collide_cell<MOVING>(g, cnt, k, usn, cell_id(d, ix, iy, iz), (size_t)d.ps, gn, mx, my, mz);
-  // imb is defined at solid cells only; a wavefront with any solid cell
-  // writes all its lanes (zeros at fluid cells) so the stores fill whole
-  // cache lines instead of leaving partial lines to merge.
-  if (IMB && __any(cnt != 0)) {
+  // imb is defined at solid cells only.  Lanes write it in groups of 16: a
+  // group that holds a solid cell writes all 16 lanes (zeros at fluid cells),
+  // so that 16 consecutive cells starting at a multiple of 16 fill three
+  // whole 64-byte granules (a partly written granule costs HBM a
+  // read-modify-write), and groups of fluid cells write nothing.
+  const unsigned long long solid_lanes = __ballot(cnt != 0);
+  if (IMB && ((solid_lanes >> (__lane_id() & 48u)) & 0xFFFFull) != 0) {
     float* q = imb + 3 * (size_t)cell_id(d, ix, iy, iz);
     if (IMBNT) {  // streaming stores: imb much larger than the caches
       __builtin_nontemporal_store(mx, q);

@@ -143,6 +143,44 @@ int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, in
                       int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, uint32_t* d_count_sum,
                       void* d_us_num, int32_t us_precision, void* stream);
 
+/* Owner-table voxelization for FP32 moving steps: the same count_sum and
+ * entries as psc_voxelize (pitch_z = nz, cell_base 0; psc3d.py:151-179,
+ * :328-345), and the solid velocity of psc3d.py:181-206 / :347-373 in a
+ * second form.  d_owner[cell] (N uint32) receives the particle whose sphere
+ * alone covers the cell, or 0xffffffff where several spheres cover it; at
+ * those shared cells only, d_us_shared (three FP32 planes of N cells,
+ * zero elsewhere on entry) accumulates sum_p alpha_p (U + omega x r)_p as
+ * psc_voxelize's d_us_num does everywhere.  psc_step_owners forms the
+ * single-owner velocity itself, so the three solid-velocity atomics per
+ * covered cell are gone.  d_count_sum must be cleared before the call; the
+ * owner table needs no clearing.  Sphere boxes must span at most one period
+ * (as the periodic table builder enforces). */
+int psc_voxelize_owners(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks,
+                        int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz,
+                        int32_t subsamples, uint32_t* d_ent_cell, uint32_t* d_ent_loc, uint16_t* d_ent_cnt,
+                        int32_t* d_chunk_nnz, uint32_t* d_count_sum, uint32_t* d_owner, float* d_us_shared,
+                        void* stream);
+
+/* The incremental clear of the owner-table form (psc3d.py:209-210 with
+ * static_geometry=False): zero d_count_sum at every cell of a previous
+ * psc_voxelize_owners call's entries and d_us_shared at those that were
+ * shared, which leaves d_us_shared zero everywhere. */
+int psc_clear_entries_owners(const uint32_t* d_ent_cell, const uint16_t* d_ent_cnt, const int32_t* d_chunk_nnz,
+                             int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz,
+                             uint32_t* d_count_sum, float* d_us_shared, void* stream);
+
+/* psc_step for FP32 deviation storage, pull layout, moving spheres voxelized
+ * by psc_voxelize_owners (psc3d.py:208-246 with the solid velocity of
+ * psc3d.py:181-206): at a cell with one owner the solid velocity is that
+ * particle's U + omega x r (lever arm from the cell centre's box coordinate,
+ * FP64, rounded to FP32); at a shared cell it is d_us_shared/(count_sum/s^3)
+ * as in psc_step.  d_particles / d_boxes: the tables of that voxelization
+ * (10 doubles and 8 int32 per particle).  Other arguments as psc_step. */
+int psc_step_owners(const float* d_in, float* d_out, const uint32_t* d_count_sum, const uint32_t* d_owner,
+                    const float* d_us_shared, const double* d_particles, const int32_t* d_boxes, float* d_imb,
+                    int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double tau, double fx, double fy,
+                    double fz, int32_t block_z, int32_t block_y, int32_t flags, void* stream);
+
 /* K2 -- fused PSC collision + streaming, one lattice step.
  * Replaces PscD3Q19.collide_stream_psc (psc3d.py:208-246) with equilibrium
  * (psc3d.py:97-103) and guo_force (psc3d.py:106-114) inlined.  The solid

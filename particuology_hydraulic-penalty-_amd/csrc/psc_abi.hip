@@ -40,7 +40,12 @@ int check_launch(const char* what) {
 int launch_step_f64(const double*, double*, const uint32_t*, const double*, double*, int, int, int, int, int, double,
                     double, double, double, int, int, hipStream_t);
 int launch_step_f32(const float*, float*, const uint32_t*, const float*, float*, int, int, int, int, int, double,
-                    double, double, double, int, int, int, hipStream_t);
+                    double, double, double, int, int, int, hipStream_t, const uint32_t* = nullptr,
+                    const double* = nullptr, const int32_t* = nullptr);
+int launch_voxelize_owners(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, uint32_t*,
+                           uint32_t*, uint16_t*, int32_t*, uint32_t*, uint32_t*, float*, hipStream_t);
+int launch_clear_owners(const uint32_t*, const uint16_t*, const int32_t*, int, int, int, int, int, uint32_t*, float*,
+                        hipStream_t);
 int launch_voxelize(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, int, uint32_t*,
                     uint32_t*, uint16_t*, int32_t*, uint32_t*, void*, int, uint32_t, hipStream_t);
 int launch_step_f32_batched(const float*, float*, const uint32_t*, float*, size_t, const void*, const void*, int, int,
@@ -264,6 +269,49 @@ int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, in
     return set_error(PSC_EINVAL, "psc_clear_entries: us_precision %d", us_precision);
   return launch_clear_entries(d_ent_cell, d_chunk_nnz, nchunks, chunk_cells, nx, ny, nz, d_count_sum, d_us_num,
                               d_us_num ? us_precision : 0, (hipStream_t)stream);
+}
+
+int psc_voxelize_owners(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks, int32_t nchunks,
+                        int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples,
+                        uint32_t* d_ent_cell, uint32_t* d_ent_loc, uint16_t* d_ent_cnt, int32_t* d_chunk_nnz,
+                        uint32_t* d_count_sum, uint32_t* d_owner, float* d_us_shared, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_voxelize_owners: bad lattice %dx%dx%d", nx, ny, nz);
+  if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
+    return set_error(PSC_EINVAL, "psc_voxelize_owners: subsamples %d outside [1, 40]", subsamples);
+  if (nchunks < 0 || chunk_cells < 1) return set_error(PSC_EINVAL, "psc_voxelize_owners: bad chunking");
+  if (nchunks > 0 && (d_particles == nullptr || d_boxes == nullptr || d_chunks == nullptr || d_ent_cell == nullptr ||
+                      d_ent_loc == nullptr || d_ent_cnt == nullptr || d_chunk_nnz == nullptr ||
+                      d_count_sum == nullptr || d_owner == nullptr || d_us_shared == nullptr))
+    return set_error(PSC_EINVAL, "psc_voxelize_owners: null pointer");
+  return launch_voxelize_owners(d_particles, d_boxes, d_chunks, nchunks, chunk_cells, nx, ny, nz, subsamples,
+                                d_ent_cell, d_ent_loc, d_ent_cnt, d_chunk_nnz, d_count_sum, d_owner, d_us_shared,
+                                (hipStream_t)stream);
+}
+
+int psc_clear_entries_owners(const uint32_t* d_ent_cell, const uint16_t* d_ent_cnt, const int32_t* d_chunk_nnz,
+                             int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz,
+                             uint32_t* d_count_sum, float* d_us_shared, void* stream) {
+  if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_clear_entries_owners: bad lattice");
+  if (nchunks < 0 || chunk_cells < 1) return set_error(PSC_EINVAL, "psc_clear_entries_owners: sizes");
+  if (nchunks > 0 && (d_ent_cell == nullptr || d_ent_cnt == nullptr || d_chunk_nnz == nullptr ||
+                      d_count_sum == nullptr || d_us_shared == nullptr))
+    return set_error(PSC_EINVAL, "psc_clear_entries_owners: null pointer");
+  return launch_clear_owners(d_ent_cell, d_ent_cnt, d_chunk_nnz, nchunks, chunk_cells, nx, ny, nz, d_count_sum,
+                             d_us_shared, (hipStream_t)stream);
+}
+
+int psc_step_owners(const float* d_in, float* d_out, const uint32_t* d_count_sum, const uint32_t* d_owner,
+                    const float* d_us_shared, const double* d_particles, const int32_t* d_boxes, float* d_imb,
+                    int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double tau, double fx, double fy,
+                    double fz, int32_t block_z, int32_t block_y, int32_t flags, void* stream) {
+  if (d_owner == nullptr || d_us_shared == nullptr || d_particles == nullptr || d_boxes == nullptr)
+    return set_error(PSC_EINVAL, "psc_step_owners: null owner table, numerators or particle tables");
+  int bz = block_z, by = block_y;
+  const int rc = psc::step_args(d_in, d_out, d_count_sum, d_us_shared, PSC_F32_DEV, PSC_PULL, nx, ny, nz, subsamples,
+                                tau, flags, &bz, &by);
+  if (rc) return rc;
+  return launch_step_f32(d_in, d_out, d_count_sum, d_us_shared, d_imb, PSC_PULL, nx, ny, nz, subsamples, tau, fx, fy,
+                         fz, bz, by, flags, (hipStream_t)stream, d_owner, d_particles, d_boxes);
 }
 
 int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const void* d_us_num, void* d_imb,

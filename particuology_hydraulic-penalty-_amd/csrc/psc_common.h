@@ -63,6 +63,27 @@ __device__ __forceinline__ int wrap_any(int v, int n) {
   return r < 0 ? r + n : r;
 }
 
+// Wrap a coordinate v in [-n, 2n) into [0, n): the box of every sphere
+// narrower than the lattice stays within one period of it.
+__device__ __forceinline__ int wrap_near(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Owner table of the FP32 moving step: owner[c] is the particle whose sphere
+// alone covers cell c, or kSharedCell where several spheres cover it.
+constexpr uint32_t kSharedCell = 0xffffffffu;
+
+// Solid-velocity inputs of a moving FP32 step.  Without an owner table the
+// numerator planes hold sum_p alpha_p (U + omega x r)_p at every covered cell
+// (plane stride N); with one, they hold it at shared cells only, and a cell
+// with a single owner takes that particle's U + omega x r (particle table:
+// 10 doubles x, y, z, r, vx, vy, vz, wx, wy, wz; box table: 8 int32 per
+// particle, the box origin first).
+struct SolidMotion {
+  const float* us_num;
+  const uint32_t* owner;
+  const double* parts;
+  const int32_t* boxes;
+};
+
 // Sum of K doubles per thread over the workgroup: wave64 butterfly then LDS across waves.
 // Result valid in thread 0.  blockDim.x must be a multiple of 64, <= 1024.
 template <int K>

@@ -367,16 +367,12 @@ __device__ int sample_count_any(int gx, int gy, int gz, const Particle& p, doubl
   return count;
 }
 
+// Sample count with the sub-sample count fixed at compile time (S = 0: s at
+// run time), so that a kernel holds the registers of its own S only.
+template <int S>
 __device__ __forceinline__ int sample_count(int gx, int gy, int gz, const Particle& p, double r2, int s) {
-  switch (s) {
-    case 1: return sample_count_t<1>(gx, gy, gz, p, r2);
-    case 2: return sample_count_t<2>(gx, gy, gz, p, r2);
-    case 3: return sample_count_t<3>(gx, gy, gz, p, r2);
-    case 4: return sample_count_t<4>(gx, gy, gz, p, r2);
-    case 5: return sample_count_t<5>(gx, gy, gz, p, r2);
-    case 6: return sample_count_t<6>(gx, gy, gz, p, r2);
-    default: return sample_count_any(gx, gy, gz, p, r2, s);
-  }
+  if constexpr (S == 0) return sample_count_any(gx, gy, gz, p, r2, s);
+  else return sample_count_t<S>(gx, gy, gz, p, r2);
 }
 
 struct Box {
@@ -387,39 +383,63 @@ __device__ __forceinline__ Box load_box(const int32_t* __restrict__ b, int p) {
   return Box{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
 }
 
-// Ordered compaction of the non-empty cells of one workgroup round: returns
-// the position of this thread's entry among the round's entries and writes
-// the round total to *total (uniform).  Wave ballots + an LDS sum over the
-// (at most 4) waves keep entry order = box order, so the output is
-// deterministic.
-__device__ __forceinline__ int round_position(bool flag, int* lds_waves, int* total) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const unsigned long long m = __ballot(flag);
-  const int before = __popcll(m & ((1ull << lane) - 1ull));
-  if (lane == 0) lds_waves[wave] = __popcll(m);
-  __syncthreads();
-  int off = 0, tot = 0;
-  const int nw = blockDim.x >> 6;
-  for (int w = 0; w < nw; ++w) {
-    const int c = lds_waves[w];
-    off += (w < wave) ? c : 0;
-    tot += c;
+// alpha = count / s^3 for counts up to kAlphaTable - 1, made by the same FP64
+// division as the direct form (identical bits), looked up instead of divided.
+constexpr int kAlphaTable = 513;
+
+// Solid count of one box cell into count_sum and, by USP: 0 nothing else;
+// 1 / 2 its solid-velocity numerator contribution into the FP32 / FP64
+// planes us_num; 3 (owner table) particle pid into the owner slot usn[cell]
+// (plain store: a cell covered by one sphere has one writer; a shared cell's
+// slot is overwritten by k_shared_owners).  NEAR: the box lies within one
+// period of the lattice on every axis (wrap by compare and add).
+template <int USP, bool NEAR>
+__device__ __forceinline__ uint32_t voxel_deposit(int gx, int gy, int gz, int cnt, const Particle& p, int pid,
+                                                  const Dims d, double s3, const double* alpha, size_t N,
+                                                  uint32_t* __restrict__ csum, void* __restrict__ usn) {
+  const uint32_t cell = NEAR ? cell_id(d, wrap_near(gx, d.nx), wrap_near(gy, d.ny), wrap_near(gz, d.nz))
+                             : cell_id(d, wrap_any(gx, d.nx), wrap_any(gy, d.ny), wrap_any(gz, d.nz));
+  atomicAdd(csum + cell, (uint32_t)cnt);
+  if (USP == 3) {
+    ((uint32_t*)usn)[cell] = (uint32_t)pid;
+  } else if (USP != 0) {
+    const double a = alpha != nullptr ? alpha[cnt] : (double)cnt / s3;
+    const double rx = ((double)gx + 0.5) - p.x;
+    const double ry = ((double)gy + 0.5) - p.y;
+    const double rz = ((double)gz + 0.5) - p.z;
+    const double vx = (p.vx + p.wy * rz) - p.wz * ry;
+    const double vy = (p.vy + p.wz * rx) - p.wx * rz;
+    const double vz = (p.vz + p.wx * ry) - p.wy * rx;
+    if (USP == 2) {
+      double* u = (double*)usn;
+      unsafeAtomicAdd(u + cell, a * vx);
+      unsafeAtomicAdd(u + N + cell, a * vy);
+      unsafeAtomicAdd(u + 2 * N + cell, a * vz);
+    } else {
+      float* u = (float*)usn;
+      unsafeAtomicAdd(u + cell, (float)(a * vx));
+      unsafeAtomicAdd(u + N + cell, (float)(a * vy));
+      unsafeAtomicAdd(u + 2 * N + cell, (float)(a * vz));
+    }
   }
-  __syncthreads();
-  *total = tot;
-  return off + before;
+  return cell;
 }
 
+// One wavefront per chunk: the chunk's cells in rounds of 64 (box order), the
+// entries of each round compacted with a ballot, so entry order = box order
+// (deterministic) and no workgroup barrier is needed.
+constexpr int kVoxThreads = 64;
 
-template <int USP>
-__global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ parts, const int32_t* __restrict__ boxes,
-                                                  const int32_t* __restrict__ chunks, int chunk_cells, Dims d,
-                                                  int s, uint32_t* __restrict__ ent_cell,
-                                                  uint32_t* __restrict__ ent_loc, uint16_t* __restrict__ ent_cnt,
-                                                  int32_t* __restrict__ chunk_nnz, uint32_t* __restrict__ csum,
-                                                  void* __restrict__ usn, uint32_t cell_base) {
-  __shared__ int lds_waves[4];
+template <int USP, int S>
+__global__ __launch_bounds__(kVoxThreads) void k_voxelize(const double* __restrict__ parts,
+                                                          const int32_t* __restrict__ boxes,
+                                                          const int32_t* __restrict__ chunks, int chunk_cells, Dims d,
+                                                          int s, uint32_t* __restrict__ ent_cell,
+                                                          uint32_t* __restrict__ ent_loc, uint16_t* __restrict__ ent_cnt,
+                                                          int32_t* __restrict__ chunk_nnz, uint32_t* __restrict__ csum,
+                                                          void* __restrict__ usn, uint32_t cell_base) {
+  __shared__ double alpha_lds[(USP == 1 || USP == 2) ? kAlphaTable : 1];
+  const int lane = (int)threadIdx.x;
   const int pid = chunks[2 * blockIdx.x];
   const int e0 = chunks[2 * blockIdx.x + 1];
   const Box bx = load_box(boxes, pid);
@@ -427,16 +447,25 @@ __global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ par
   const int vol = bx.ex * bx.ey * bx.ez;
   const int e1 = min(e0 + chunk_cells, vol);
   const double r2 = p.r * p.r;
-  const double s3 = (double)(s * s * s);
+  const int s3i = s * s * s;
+  const double s3 = (double)s3i;
   const size_t N = (size_t)d.ps;
   const size_t out0 = (size_t)blockIdx.x * chunk_cells;
+  const double* alpha = nullptr;
+  if ((USP == 1 || USP == 2) && s3i < kAlphaTable) {
+    for (int i = lane; i <= s3i; i += kVoxThreads) alpha_lds[i] = (double)i / s3;
+    __syncthreads();
+    alpha = alpha_lds;
+  }
+  const bool near = bx.x0 >= -d.nx && bx.x0 + bx.ex <= 2 * d.nx && bx.y0 >= -d.ny && bx.y0 + bx.ey <= 2 * d.ny &&
+                    bx.z0 >= -d.nz && bx.z0 + bx.ez <= 2 * d.nz;
   int written = 0;
   // box coordinates of entry e: FP32-reciprocal division (exact after
   // correction) while the box is below 2^24 cells, integer division beyond
   const bool fast = vol < (1 << 24);
   const float inv_ez = 1.0f / (float)bx.ez, inv_ey = 1.0f / (float)bx.ey;
-  for (int r = e0; r < e1; r += blockDim.x) {
-    const int e = r + (int)threadIdx.x;
+  for (int r = e0; r < e1; r += kVoxThreads) {
+    const int e = r + lane;
     int cnt = 0, lx = 0, ly = 0, lz = 0;
     uint32_t cell = 0;
     if (e < e1) {
@@ -454,43 +483,91 @@ __global__ __launch_bounds__(256) void k_voxelize(const double* __restrict__ par
         lx = t / bx.ey;
       }
       const int gx = bx.x0 + lx, gy = bx.y0 + ly, gz = bx.z0 + lz;
-      cnt = sample_count(gx, gy, gz, p, r2, s);
+      cnt = sample_count<S>(gx, gy, gz, p, r2, s);
       if (cnt) {
-        cell = cell_id(d, wrap_any(gx, d.nx), wrap_any(gy, d.ny), wrap_any(gz, d.nz));
-        atomicAdd(csum + cell, (uint32_t)cnt);
-        if (USP != 0) {
-          const double a = (double)cnt / s3;
-          const double rx = ((double)gx + 0.5) - p.x;
-          const double ry = ((double)gy + 0.5) - p.y;
-          const double rz = ((double)gz + 0.5) - p.z;
-          const double vx = (p.vx + p.wy * rz) - p.wz * ry;
-          const double vy = (p.vy + p.wz * rx) - p.wx * rz;
-          const double vz = (p.vz + p.wx * ry) - p.wy * rx;
-          if (USP == 2) {
-            double* u = (double*)usn;
-            unsafeAtomicAdd(u + cell, a * vx);
-            unsafeAtomicAdd(u + N + cell, a * vy);
-            unsafeAtomicAdd(u + 2 * N + cell, a * vz);
-          } else {
-            float* u = (float*)usn;
-            unsafeAtomicAdd(u + cell, (float)(a * vx));
-            unsafeAtomicAdd(u + N + cell, (float)(a * vy));
-            unsafeAtomicAdd(u + 2 * N + cell, (float)(a * vz));
-          }
-        }
+        cell = near ? voxel_deposit<USP, true>(gx, gy, gz, cnt, p, pid, d, s3, alpha, N, csum, usn)
+                    : voxel_deposit<USP, false>(gx, gy, gz, cnt, p, pid, d, s3, alpha, N, csum, usn);
       }
     }
-    int tot = 0;
-    const int pos = round_position(cnt != 0, lds_waves, &tot);
+    const unsigned long long m = __ballot(cnt != 0);
     if (cnt) {
-      const size_t o = out0 + written + pos;
+      const size_t o = out0 + written + __popcll(m & ((1ull << lane) - 1ull));
       ent_cell[o] = cell + cell_base;
       ent_loc[o] = (uint32_t)lx | ((uint32_t)ly << kLocBits) | ((uint32_t)lz << (2 * kLocBits));
       ent_cnt[o] = (uint16_t)cnt;
     }
-    written += tot;
+    written += __popcll(m);
   }
-  if (threadIdx.x == 0) chunk_nnz[blockIdx.x] = written;
+  if (lane == 0) chunk_nnz[blockIdx.x] = written;
+}
+
+// Owner table, second pass over the entries: a cell whose count sum differs
+// from the count of one of its entries is covered by more than one sphere.
+// Its owner slot is marked shared and each of its spheres adds
+// alpha (U + omega x r) into the FP32 numerator planes there, exactly as the
+// plane form (USP 1) does at every cell; cells with one owner are left alone.
+__global__ __launch_bounds__(256) void k_shared_owners(const double* __restrict__ parts,
+                                                       const int32_t* __restrict__ boxes,
+                                                       const int32_t* __restrict__ chunks, int chunk_cells, Dims d,
+                                                       int s, const uint32_t* __restrict__ ent_cell,
+                                                       const uint32_t* __restrict__ ent_loc,
+                                                       const uint16_t* __restrict__ ent_cnt,
+                                                       const int32_t* __restrict__ chunk_nnz,
+                                                       const uint32_t* __restrict__ csum, uint32_t* __restrict__ owner,
+                                                       float* __restrict__ us_num) {
+  const int c = blockIdx.x;
+  const int nnz = chunk_nnz[c];
+  const size_t out0 = (size_t)c * chunk_cells;
+  const size_t N = (size_t)d.ps;
+  const double s3 = (double)(s * s * s);
+  const int pid = chunks[2 * c];
+  constexpr uint32_t kLocMask = (1u << kLocBits) - 1u;
+  for (int k = threadIdx.x; k < nnz; k += blockDim.x) {
+    const uint32_t cell = ent_cell[out0 + k];
+    const int cnt = ent_cnt[out0 + k];
+    if (csum[cell] == (uint32_t)cnt) continue;  // one owner
+    owner[cell] = kSharedCell;
+    const Box bx = load_box(boxes, pid);
+    const Particle p = load_particle(parts, pid);
+    const uint32_t loc = ent_loc[out0 + k];
+    const int gx = bx.x0 + (int)(loc & kLocMask);
+    const int gy = bx.y0 + (int)((loc >> kLocBits) & kLocMask);
+    const int gz = bx.z0 + (int)((loc >> (2 * kLocBits)) & kLocMask);
+    const double a = (double)cnt / s3;
+    const double rx = ((double)gx + 0.5) - p.x;
+    const double ry = ((double)gy + 0.5) - p.y;
+    const double rz = ((double)gz + 0.5) - p.z;
+    const double vx = (p.vx + p.wy * rz) - p.wz * ry;
+    const double vy = (p.vy + p.wz * rx) - p.wx * rz;
+    const double vz = (p.vz + p.wx * ry) - p.wy * rx;
+    unsafeAtomicAdd(us_num + cell, (float)(a * vx));
+    unsafeAtomicAdd(us_num + N + cell, (float)(a * vy));
+    unsafeAtomicAdd(us_num + 2 * N + cell, (float)(a * vz));
+  }
+}
+
+// Owner table: zero count_sum at the cells of a previous voxelization's
+// entries, and the numerator planes at those of them that were shared (count
+// sum != the entry's count; a thread that finds the sum already zeroed by
+// another entry of the same cell zeroes the planes too).  The owner slots
+// need no clearing: they are read only where count_sum > 0, and every
+// voxelization rewrites them there.
+__global__ __launch_bounds__(256) void k_clear_owners(const uint32_t* __restrict__ ent_cell,
+                                                      const uint16_t* __restrict__ ent_cnt,
+                                                      const int32_t* __restrict__ chunk_nnz, int chunk_cells, size_t N,
+                                                      uint32_t* __restrict__ csum, float* __restrict__ us_num) {
+  const int c = blockIdx.x;
+  const int nnz = chunk_nnz[c];
+  const size_t out0 = (size_t)c * chunk_cells;
+  for (int k = threadIdx.x; k < nnz; k += blockDim.x) {
+    const uint32_t cell = ent_cell[out0 + k];
+    if (csum[cell] != (uint32_t)ent_cnt[out0 + k]) {
+      us_num[cell] = 0.0f;
+      us_num[N + cell] = 0.0f;
+      us_num[2 * N + cell] = 0.0f;
+    }
+    csum[cell] = 0u;
+  }
 }
 
 // Zero count_sum (and the solid-velocity numerator) at the cells listed in
@@ -833,15 +910,28 @@ int launch_voxelize(const double* parts, const int32_t* boxes, const int32_t* ch
   // us_num: three planes of nx*ny*pz cells, the planes the step kernels and
   // the incremental clear read (count_sum is a single plane)
   const Dims d{nx, ny, nz, pz, (unsigned long long)nx * ny * pz};
-  if (usn == nullptr || usp == 0)
-    k_voxelize<0><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
-                                               chunk_nnz, csum, nullptr, cell_base);
-  else if (usp == 1)
-    k_voxelize<1><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
-                                               chunk_nnz, csum, usn, cell_base);
-  else
-    k_voxelize<2><<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
-                                               chunk_nnz, csum, usn, cell_base);
+#define PSC_VOX(U, S)                                                                                       \
+  k_voxelize<U, S><<<nchunks, kVoxThreads, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, \
+                                                        ent_cnt, chunk_nnz, csum, U ? usn : nullptr, cell_base)
+#define PSC_VOX_S(U)             \
+  switch (s) {                   \
+    case 1: PSC_VOX(U, 1); break; \
+    case 2: PSC_VOX(U, 2); break; \
+    case 3: PSC_VOX(U, 3); break; \
+    case 4: PSC_VOX(U, 4); break; \
+    case 5: PSC_VOX(U, 5); break; \
+    case 6: PSC_VOX(U, 6); break; \
+    default: PSC_VOX(U, 0); break; \
+  }
+  if (usn == nullptr || usp == 0) {
+    PSC_VOX_S(0)
+  } else if (usp == 1) {
+    PSC_VOX_S(1)
+  } else {
+    PSC_VOX_S(2)
+  }
+#undef PSC_VOX_S
+#undef PSC_VOX
   return check_launch("psc_voxelize");
 }
 
@@ -893,6 +983,41 @@ int launch_loads_final(const int32_t* chunk_first, int np, const double* partial
   if (np == 0) return PSC_OK;
   k_loads_final<<<np, 64, 0, stream>>>(chunk_first, partials, loads);
   return check_launch("psc_particle_loads(final)");
+}
+
+int launch_voxelize_owners(const double* parts, const int32_t* boxes, const int32_t* chunks, int nchunks,
+                           int chunk_cells, int nx, int ny, int nz, int s, uint32_t* ent_cell, uint32_t* ent_loc,
+                           uint16_t* ent_cnt, int32_t* chunk_nnz, uint32_t* csum, uint32_t* owner, float* us_num,
+                           hipStream_t stream) {
+  if (nchunks == 0) return PSC_OK;
+  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
+  void* own = owner;
+#define PSC_VOX_OWN(S)                                                                                            \
+  k_voxelize<3, S><<<nchunks, kVoxThreads, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, \
+                                                        ent_cnt, chunk_nnz, csum, own, 0u)
+  switch (s) {
+    case 1: PSC_VOX_OWN(1); break;
+    case 2: PSC_VOX_OWN(2); break;
+    case 3: PSC_VOX_OWN(3); break;
+    case 4: PSC_VOX_OWN(4); break;
+    case 5: PSC_VOX_OWN(5); break;
+    case 6: PSC_VOX_OWN(6); break;
+    default: PSC_VOX_OWN(0); break;
+  }
+#undef PSC_VOX_OWN
+  const int rc = check_launch("psc_voxelize_owners");
+  if (rc) return rc;
+  k_shared_owners<<<nchunks, 256, 0, stream>>>(parts, boxes, chunks, chunk_cells, d, s, ent_cell, ent_loc, ent_cnt,
+                                               chunk_nnz, csum, owner, us_num);
+  return check_launch("psc_voxelize_owners(shared cells)");
+}
+
+int launch_clear_owners(const uint32_t* ent_cell, const uint16_t* ent_cnt, const int32_t* chunk_nnz, int nchunks,
+                        int chunk_cells, int nx, int ny, int nz, uint32_t* csum, float* us_num, hipStream_t stream) {
+  if (nchunks == 0) return PSC_OK;
+  k_clear_owners<<<nchunks, 256, 0, stream>>>(ent_cell, ent_cnt, chunk_nnz, chunk_cells, (size_t)nx * ny * nz, csum,
+                                              us_num);
+  return check_launch("psc_clear_entries_owners");
 }
 
 int launch_clear_entries(const uint32_t* ent_cell, const int32_t* chunk_nnz, int nchunks, int chunk_cells, int nx,

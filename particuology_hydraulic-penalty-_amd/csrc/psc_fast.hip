@@ -79,14 +79,14 @@ struct StepConsts {
 
 // The collision of one cell from its gathered populations g (deviation
 // form) and solid count cnt: post-collision values gn and, at a solid cell,
-// its imb (mx, my, mz; 0 in fluid).  usn/cell/N: the solid-velocity
-// numerator planes and the cell's index, read only by MOVING solid cells.
+// its imb (mx, my, mz; 0 in fluid).  sm/cell/N/d/ix..iz: the solid-velocity
+// inputs, the cell's index and coordinates, read only by MOVING solid cells.
 // Every step kernel calls this one function, so every launch form computes
 // the same bits.
 template <bool MOVING>
 __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, const StepConsts& k,
-                                             const float* __restrict__ usn, uint32_t cell, size_t N, float (&gn)[Q],
-                                             float& mx, float& my, float& mz) {
+                                             const SolidMotion& sm, uint32_t cell, size_t N, const Dims& d, int ix,
+                                             int iy, int iz, float (&gn)[Q], float& mx, float& my, float& mz) {
   const float drho = ((g[0] + (g[1] + g[2])) + ((g[3] + g[4]) + (g[5] + g[6]))) +
                      (((g[7] + g[8]) + (g[9] + g[10])) + ((g[11] + g[12]) + (g[13] + g[14]))) +
                      ((g[15] + g[16]) + (g[17] + g[18]));
@@ -148,10 +148,25 @@ __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, 
     float sx = 0.f, sy = 0.f, sz = 0.f, ssq15 = 0.f;
     if (MOVING && cnt != 0) {
       const uint32_t c = cell;
-      const float ia = 1.0f / ((float)cnt * k.inv_s3);
-      sx = usn[c] * ia;
-      sy = usn[N + c] * ia;
-      sz = usn[2 * N + c] * ia;
+      const uint32_t o = sm.owner != nullptr ? sm.owner[c] : kSharedCell;
+      if (o != kSharedCell) {
+        // the one sphere covering the cell: U + omega x r with the voxelizer's
+        // lever arm (the cell's box coordinate, found from the box origin,
+        // whose box spans at most one period), in FP64, rounded to FP32
+        const double* q = sm.parts + 10 * (size_t)o;
+        const int32_t* b = sm.boxes + 8 * (size_t)o;
+        const double rx = ((double)(b[0] + wrap_near(ix - b[0], d.nx)) + 0.5) - q[0];
+        const double ry = ((double)(b[1] + wrap_near(iy - b[1], d.ny)) + 0.5) - q[1];
+        const double rz = ((double)(b[2] + wrap_near(iz - b[2], d.nz)) + 0.5) - q[2];
+        sx = (float)((q[4] + q[8] * rz) - q[9] * ry);
+        sy = (float)((q[5] + q[9] * rx) - q[7] * rz);
+        sz = (float)((q[6] + q[7] * ry) - q[8] * rx);
+      } else {
+        const float ia = 1.0f / ((float)cnt * k.inv_s3);
+        sx = sm.us_num[c] * ia;
+        sy = sm.us_num[N + c] * ia;
+        sz = sm.us_num[2 * N + c] * ia;
+      }
       ssq15 = 1.5f * (sx * sx + sy * sy + sz * sz);
     }
     {
@@ -209,7 +224,7 @@ __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, 
 // in the same order: results are bit-identical to two pull steps.
 template <bool PULL, bool MOVING, bool IMB, bool NT, bool IMBNT = false, int AA = 0>
 __device__ __forceinline__ void step_cell(const float* gin, float* gout,
-                                          const uint32_t* __restrict__ csum, const float* __restrict__ usn,
+                                          const uint32_t* __restrict__ csum, const SolidMotion& sm,
                                           float* __restrict__ imb, const Dims d, const StepConsts& k, int ix,
                                           int iy, int iz) {
   const size_t plane = (size_t)d.ps * sizeof(float);  // bytes per direction plane
@@ -240,7 +255,7 @@ __device__ __forceinline__ void step_cell(const float* gin, float* gout,
   // collision, so every store keeps the uniform-base + 32-bit-offset form.
   float gn[Q];
   float mx, my, mz;  // imb of a solid cell (0 in fluid)
-  collide_cell<MOVING>(g, cnt, k, usn, cell_id(d, ix, iy, iz), (size_t)d.ps, gn, mx, my, mz);
+  collide_cell<MOVING>(g, cnt, k, sm, cell_id(d, ix, iy, iz), (size_t)d.ps, d, ix, iy, iz, gn, mx, my, mz);
   // imb is defined at solid cells only.  Lanes write it in groups of 16: a
   // group that holds a solid cell writes all 16 lanes (zeros at fluid cells),
   // so that 16 consecutive cells starting at a multiple of 16 fill three
@@ -281,14 +296,14 @@ __device__ __forceinline__ void step_cell(const float* gin, float* gout,
 template <bool PULL, bool MOVING, bool IMB, bool NT, int CPT, bool LIN>
 __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin, float* __restrict__ gout,
                                                   const uint32_t* __restrict__ csum,
-                                                  const float* __restrict__ usn, float* __restrict__ imb, Dims d,
+                                                  SolidMotion sm, float* __restrict__ imb, Dims d,
                                                   StepConsts k, float inv_nz) {
   if (LIN) {
     const uint32_t c = blockIdx.x * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
     if (c >= (uint32_t)d.ny * (uint32_t)d.nz) return;
     uint32_t iy, iz;
     divmod_f(c, (uint32_t)d.nz, inv_nz, iy, iz);
-    step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, usn, imb, d, k, blockIdx.y, (int)iy, (int)iz);
+    step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, sm, imb, d, k, blockIdx.y, (int)iy, (int)iz);
     return;
   }
   const int iz = blockIdx.x * blockDim.x + threadIdx.x;
@@ -299,7 +314,7 @@ __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin,
   for (int c = 0; c < CPT; ++c) {
     const int iy = iy0 + c;
     if (iy < d.ny)
-      step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, usn, imb, d, k, ix, iy, iz);
+      step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, sm, imb, d, k, ix, iy, iz);
   }
 }
 
@@ -368,7 +383,7 @@ __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict_
     if (c >= (uint32_t)d.ny * (uint32_t)d.nz) return;
     uint32_t iy, iz;
     divmod_f(c, (uint32_t)d.nz, inv_nz, iy, iz);
-    step_cell<true, false, true, NT>(gin, gout, csum, nullptr, imb, d, k, ix, (int)iy, (int)iz);
+    step_cell<true, false, true, NT>(gin, gout, csum, SolidMotion{}, imb, d, k, ix, (int)iy, (int)iz);
     return;
   }
   const int t = b / gz;
@@ -381,7 +396,7 @@ __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict_
   for (int c = 0; c < CPT; ++c) {
     const int iy = iy0 + c;
     if (iy < d.ny)
-      step_cell<true, false, true, NT>(gin, gout, csum, nullptr, imb, d, k, ix, iy, iz);
+      step_cell<true, false, true, NT>(gin, gout, csum, SolidMotion{}, imb, d, k, ix, iy, iz);
   }
 }
 
@@ -412,7 +427,7 @@ __global__ __launch_bounds__(256) void k_step_f32_aa(float* g, const uint32_t* _
     iy = (t - ix * gy) * blockDim.y + threadIdx.y;
     if (iz >= d.nz || iy >= d.ny) return;
   }
-  step_cell<true, false, true, NT, false, AA>(g, g, csum, nullptr, imb, d, k, ix, iy, iz);
+  step_cell<true, false, true, NT, false, AA>(g, g, csum, SolidMotion{}, imb, d, k, ix, iy, iz);
 }
 
 // ---------------------------------------------------------------------------
@@ -561,7 +576,7 @@ __global__ __launch_bounds__(BS) void k_step_f32_batched(const float* __restrict
     k.Fy = w.fy;
     k.Fz = w.fz;
     const Dims d{w.nx, w.ny, w.nz, w.pz, (unsigned long long)w.ps};
-    step_cell<true, false, IMB, NT, IMBNT>(gin + w.state_off, gout + w.state_off, csum + w.cell_off, nullptr,
+    step_cell<true, false, IMB, NT, IMBNT>(gin + w.state_off, gout + w.state_off, csum + w.cell_off, SolidMotion{},
                                            IMB ? imb + 3 * w.cell_off : nullptr, d, k, (int)ix, (int)iy, (int)iz);
   }
 }
@@ -571,10 +586,13 @@ __global__ __launch_bounds__(BS) void k_step_f32_batched(const float* __restrict
 constexpr size_t kLdsPerCU = 163840;
 constexpr size_t kLdsHeadroom = 8192;
 
+// owner/parts/boxes (optional, with usn): the owner table of a moving step.
 int launch_step_f32(const float* in, float* out, const uint32_t* csum, const float* usn, float* imb, int layout,
                     int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz, int bz, int by,
-                    int nt, hipStream_t stream) {
+                    int nt, hipStream_t stream, const uint32_t* owner, const double* parts,
+                    const int32_t* boxes) {
   const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
+  const SolidMotion sm{usn, owner, parts, boxes};
   const int cpt = (nt >> 16) & 3 ? (nt >> 16) & 3 : 1;
   const int bpc = (nt >> 8) & 0xff;
   size_t lds = 0;
@@ -610,13 +628,13 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
 #define PSC_F32_CASE(P, M, I, T)                                                                  \
   if (pull == P && moving == M && wimb == I && ntb == T) {                                       \
     if (lin)                                                                                     \
-      k_step_f32<P, M, I, T, 1, true><<<grid, block, lds, stream>>>(in, out, csum, usn, imb, d, k, \
+      k_step_f32<P, M, I, T, 1, true><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,  \
                                                                     inv_nz);                     \
     else if (cpt == 2)                                                                           \
-      k_step_f32<P, M, I, T, 2, false><<<grid, block, lds, stream>>>(in, out, csum, usn, imb, d, k, \
+      k_step_f32<P, M, I, T, 2, false><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,  \
                                                                      inv_nz);                    \
     else                                                                                         \
-      k_step_f32<P, M, I, T, 1, false><<<grid, block, lds, stream>>>(in, out, csum, usn, imb, d, k, \
+      k_step_f32<P, M, I, T, 1, false><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,  \
                                                                      inv_nz);                    \
     return check_launch("psc_step(f32)");                                                        \
   }

@@ -200,7 +200,8 @@ class PscD3Q19:
     def __init__(self, n, particles, tau: float = 0.72, subsamples: int = 4, body_force_z: float = 0.0,
                  static_geometry: bool = False, *, body_force=None, precision: str = "fp64",
                  stream_layout: str = "pull", periodic_geometry: bool = False, device=None,
-                 chunk_cells: int = 2048, block="auto", nontemporal: bool = False, diag: str = "lazy"):
+                 chunk_cells: int = 2048, block="auto", nontemporal: bool = False, diag: str = "lazy",
+                 owner_table=None):
         if isinstance(n, (tuple, list)):
             shape = tuple(int(v) for v in n)
             if len(shape) != 3:
@@ -251,6 +252,10 @@ class PscD3Q19:
         # stream (None = by lattice size and precision)
         self.pipeline_loads = None
         self.periodic_geometry = bool(periodic_geometry)
+        # moving spheres re-voxelized every step, FP32 pull storage: record one owner
+        # particle per covered cell and form its U + omega x r in the step kernel
+        # instead of accumulating solid-velocity planes (None = wherever it applies)
+        self.owner_table = owner_table
         if precision not in ("fp32", "fp64"):
             raise ValueError("precision must be 'fp32' or 'fp64'")
         self.precision = precision
@@ -281,8 +286,10 @@ class PscD3Q19:
         self._imb = self._imb_bufs[0]
         self._partials_alt = None
         self._us = None
+        self._own = None          # owner table (one particle id per covered cell)
+        self._geom_mode = None    # how the current geometry was voxelized: count / planes / owners
         self._ent = None          # (ent_cell u32, ent_loc u32, ent_cnt u16) as int32/int32/int16 tensors
-        self._old_ent_cell = None
+        self._old_ent = None
         self._us_clean = False    # count_sum/us_num are zero outside the current entries
         self._chunk_nnz = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._partials = torch.zeros(6, dtype=torch.float64, device=self.device)
@@ -373,6 +380,19 @@ class PscD3Q19:
                        torch.empty(a.shape, dtype=dt, device=self.device))
         return out
 
+    def _voxel_mode(self, moving: bool) -> str:
+        """How a voxelization stores the solid velocity: "count" (static spheres:
+        none), "owners" (owner table; FP32 pull storage with the geometry
+        rebuilt every step, unless owner_table is False) or "planes"."""
+        if not moving:
+            return "count"
+        fits = self._storage == _lib.PSC_F32_DEV and self._layout == _lib.PSC_PULL and not self.static_geometry
+        if self.owner_table is None:
+            return "owners" if fits else "planes"
+        if self.owner_table and not fits:
+            raise ValueError("owner_table needs FP32 pull storage and static_geometry=False")
+        return "owners" if self.owner_table else "planes"
+
     def rebuild_geometry(self) -> None:
         """Voxelize every sphere (psc3d.py:151-179 / 328-345) on the GPU."""
         torch = _torch()
@@ -383,7 +403,7 @@ class PscD3Q19:
             [tabs.particles, tabs.boxes, tabs.chunks, tabs.chunk_first])
         C = int(tabs.chunks.shape[0])
         cap = max(1, C * tabs.chunk_cells)
-        self._old_ent_cell = self._ent[0] if self._ent is not None else None
+        self._old_ent = self._ent
         old_nnz = self._chunk_nnz
         if self._ent is None or self._ent[0].numel() < cap:
             cap = int(cap * 1.25) + 1
@@ -401,35 +421,53 @@ class PscD3Q19:
             self._loads = torch.zeros((P, 6), dtype=torch.float64, device=dev)
         stream = self._stream()
         N = self._N
-        usp = (1 if self._storage == _lib.PSC_F32_DEV else 2) if tabs.moving else 0
-        if tabs.moving and self._us is None:
+        mode = self._voxel_mode(tabs.moving)
+        if mode != "count" and self._us is None:
             self._us = torch.empty(3 * N, dtype=self._fdt, device=dev)
             _lib.call("psc_clear", self._ptr(self._us), 3 * N * self._us.element_size(), stream)
-        if self._tables is not None and self._ent is not None and self._us_clean:
-            # zero only the cells the previous geometry touched
-            old = self._tables
-            oc = self._old_ent_cell
-            _lib.call("psc_clear_entries", self._ptr(oc), self._ptr(self._chunk_nnz), int(old.chunks.shape[0]),
-                      old.chunk_cells, *self._shape, self._ptr(self._csum),
-                      self._ptr(self._us) if self._us is not None else None,
-                      (1 if self._storage == _lib.PSC_F32_DEV else 2) if self._us is not None else 0, stream)
+        if mode == "owners" and self._own is None:
+            self._own = torch.empty(N, dtype=torch.int32, device=dev)
+        old = self._tables
+        if old is not None and self._old_ent is not None and self._us_clean:
+            # zero only the cells the previous geometry touched (count_sum and us_num
+            # are zero everywhere else)
+            oc, _, on = self._old_ent
+            nold = int(old.chunks.shape[0])
+            if self._geom_mode == "owners":
+                _lib.call("psc_clear_entries_owners", self._ptr(oc), self._ptr(on), self._ptr(self._chunk_nnz), nold,
+                          old.chunk_cells, *self._shape, self._ptr(self._csum), self._ptr(self._us), stream)
+            elif self._geom_mode == "planes":
+                _lib.call("psc_clear_entries", self._ptr(oc), self._ptr(self._chunk_nnz), nold, old.chunk_cells,
+                          *self._shape, self._ptr(self._csum), self._ptr(self._us),
+                          1 if self._storage == _lib.PSC_F32_DEV else 2, stream)
+            else:
+                _lib.call("psc_clear_entries", self._ptr(oc), self._ptr(self._chunk_nnz), nold, old.chunk_cells,
+                          *self._shape, self._ptr(self._csum), None, 0, stream)
         else:
             _lib.call("psc_clear", self._ptr(self._csum), 4 * N, stream)
             if self._us is not None:
                 _lib.call("psc_clear", self._ptr(self._us), 3 * N * self._us.element_size(), stream)
-        us_ptr = self._ptr(self._us) if tabs.moving else None
         nx, ny, nz = self._shape
         ec, el, en = self._ent
-        _lib.call("psc_voxelize", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), C, tabs.chunk_cells,
-                  nx, ny, nz, 0, self._s, self._ptr(ec), self._ptr(el), self._ptr(en), self._ptr(self._chunk_nnz),
-                  self._ptr(self._csum), us_ptr, usp, 0, stream)
+        if mode == "owners":
+            _lib.call("psc_voxelize_owners", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), C,
+                      tabs.chunk_cells, nx, ny, nz, self._s, self._ptr(ec), self._ptr(el), self._ptr(en),
+                      self._ptr(self._chunk_nnz), self._ptr(self._csum), self._ptr(self._own), self._ptr(self._us),
+                      stream)
+        else:
+            us_ptr = self._ptr(self._us) if mode == "planes" else None
+            usp = (1 if self._storage == _lib.PSC_F32_DEV else 2) if mode == "planes" else 0
+            _lib.call("psc_voxelize", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), C, tabs.chunk_cells,
+                      nx, ny, nz, 0, self._s, self._ptr(ec), self._ptr(el), self._ptr(en),
+                      self._ptr(self._chunk_nnz), self._ptr(self._csum), us_ptr, usp, 0, stream)
+        self._geom_mode = mode
         self._tables = tabs
         # keep the device tables alive until the kernels that read them have run
         self._dev_tables = (parts, boxes, chunks, cfirst)
         self._moving = tabs.moving
         self._geometry_built = True
         self._us_clean = True
-        self._old_ent_cell = None
+        self._old_ent = None
         self._plan_epoch += 1
         self._cache.clear()
 
@@ -589,9 +627,15 @@ class PscD3Q19:
         fx, fy, fz = self._force
         us_ptr = self._ptr(self._us) if self._moving else None
         self._imb = self._imb_bufs[0]
-        _lib.call("psc_step", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum), us_ptr,
-                  self._ptr(self._imb), self._storage, self._layout, nx, ny, nz, self._s, self.tau, fx, fy, fz,
-                  self._block[0], self._block[1], self._flags, stream)
+        if self._geom_mode == "owners":
+            parts, boxes = self._dev_tables[0], self._dev_tables[1]
+            _lib.call("psc_step_owners", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum),
+                      self._ptr(self._own), us_ptr, self._ptr(parts), self._ptr(boxes), self._ptr(self._imb),
+                      nx, ny, nz, self._s, self.tau, fx, fy, fz, self._block[0], self._block[1], self._flags, stream)
+        else:
+            _lib.call("psc_step", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum), us_ptr,
+                      self._ptr(self._imb), self._storage, self._layout, nx, ny, nz, self._s, self.tau, fx, fy, fz,
+                      self._block[0], self._block[1], self._flags, stream)
         self._a, self._b = self._b, self._a
         self._step_count += 1
         self._cache.clear()
@@ -766,7 +810,7 @@ class PscD3Q19:
             z = np.zeros(self._shape)
             return z, z.copy(), z.copy()
         asum = self._alpha_sum_np()
-        num = self._np(self._us).astype(np.float64).reshape((3,) + self._shape)
+        num = self._np(self._us_num_planes()).astype(np.float64).reshape((3,) + self._shape)
         mask = asum > 1.0e-14
         out = []
         for a in range(3):
@@ -774,6 +818,28 @@ class PscD3Q19:
             u[mask] /= asum[mask]
             out.append(u)
         return tuple(out)
+
+    def _us_num_planes(self):
+        """The solid-velocity numerator planes sum_p alpha_p (U + omega x r) at
+        every covered cell.  The owner table keeps them at shared cells only,
+        so they are voxelized afresh into scratch buffers for this readout."""
+        if self._geom_mode != "owners":
+            return self._us
+        torch = _torch()
+        N, dev = self._N, self.device
+        tabs = self._tables
+        parts, boxes, chunks, _ = self._dev_tables
+        C = int(tabs.chunks.shape[0])
+        cap = max(1, C * tabs.chunk_cells)
+        csum = torch.zeros(N, dtype=torch.int32, device=dev)
+        us = torch.zeros(3 * N, dtype=self._fdt, device=dev)
+        ent = (torch.empty(cap, dtype=torch.int32, device=dev), torch.empty(cap, dtype=torch.int32, device=dev),
+               torch.empty(cap, dtype=torch.int16, device=dev))
+        nnz = torch.empty(max(1, C), dtype=torch.int32, device=dev)
+        _lib.call("psc_voxelize", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), C, tabs.chunk_cells,
+                  *self._shape, 0, self._s, *(self._ptr(t) for t in ent), self._ptr(nnz), self._ptr(csum),
+                  self._ptr(us), 1, 0, self._stream())
+        return us
 
     def _diagnostics(self) -> dict:
         sums = self.readout_sums()

@@ -134,7 +134,15 @@ def step_kernel_ms(sol, repeats, torch, lib, moving=False, launch_cfg=None):
     h = stream.cuda_stream
     bz, by, flags = launch_cfg if launch_cfg is not None else (sol._block[0], sol._block[1], sol._flags)
 
+    owners = moving and getattr(sol, "_geom_mode", None) == "owners"
+
     def launch(a, b):
+        if owners:  # the kernel the moving solver runs: solid velocity from the owner table
+            parts, boxes = sol._dev_tables[0], sol._dev_tables[1]
+            lib.call("psc_step_owners", a.data_ptr(), b.data_ptr(), sol._csum.data_ptr(), sol._own.data_ptr(),
+                     sol._us.data_ptr(), parts.data_ptr(), boxes.data_ptr(), sol._imb.data_ptr(), nx, ny, nz,
+                     sol._s, sol.tau, fx, fy, fz, bz, by, flags, h)
+            return
         lib.call("psc_step", a.data_ptr(), b.data_ptr(), sol._csum.data_ptr(),
                  sol._us.data_ptr() if moving else None, sol._imb.data_ptr(),
                  sol._storage, sol._layout, nx, ny, nz, sol._s, sol.tau, fx, fy, fz, bz, by, flags, h)
@@ -397,7 +405,12 @@ def bench_moving(args, psc3d, workloads, lib, torch, np):
     log(f"moving window {n}^3: {warm} warm-up + {steps} steps, FP32", args.quiet)
     s32, w32 = run("fp32", steps, True)
     kern = step_kernel_ms(s32, 20, torch, lib, moving=True)
-    nsolid = int((s32._csum != 0).sum().item())   # cells that read us_num and write imb
+    solid = s32._csum != 0
+    nsolid = int(solid.sum().item())   # cells that read their solid velocity and write imb
+    owners = s32._geom_mode == "owners"
+    # owner table: 4 B of owner id at every cell with solid, 12 B of numerators at the shared ones
+    nshared = int(solid.logical_and(s32._own == -1).sum().item()) if owners else nsolid
+    launch_cfg = [int(s32._block[0]), int(s32._block[1]), int(s32._flags)]
     loads32 = s32._loads_np().copy()
     mlups = N * steps / w32 / 1e6
     log(f"fp32: {mlups:.0f} MLUPS, step kernel {kern*1e3:.0f} us", args.quiet)
@@ -411,9 +424,10 @@ def bench_moving(args, psc3d, workloads, lib, torch, np):
     Ts = np.linalg.norm(loads64[:, 3:6], axis=1)
     eF = np.linalg.norm(loads32[:, 0:3] - loads64[:, 0:3], axis=1) / np.maximum(Fs, 1e-3 * Fs.max())
     eT = np.linalg.norm(loads32[:, 3:6] - loads64[:, 3:6], axis=1) / np.maximum(Ts, 1e-3 * Ts.max())
-    # algorithmic bytes: 156 per cell plus, at the cells with solid, 12 of solid velocity read
-    # and 12 of imb written
-    launch_bytes = BYTES_F32_STATIC * N + 24 * nsolid
+    # algorithmic bytes: 156 per cell plus, at the cells with solid, 12 of imb written and the
+    # solid velocity: 12 of numerators (plane form) or 4 of owner id + 12 of numerators at the
+    # shared cells (owner table; the particle and box tables are a few hundred KB, cache-resident)
+    launch_bytes = BYTES_F32_STATIC * N + (16 * nsolid + 12 * nshared if owners else 24 * nsolid)
     achieved = launch_bytes / (kern * 1e-3) / 1e9
     out = {
         "metric": METRIC + " (moving-window configuration)",
@@ -425,9 +439,10 @@ def bench_moving(args, psc3d, workloads, lib, torch, np):
                                "storage, FP64 loads; step = voxelize + collide+stream + loads",
                    "lattice": [n, n, n], "particles": int(base.shape[0])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_step_f32 (pull, moving)",
-                     "bytes_per_cell": BYTES_F32_STATIC, "solid_cells": nsolid, "bytes_per_launch": launch_bytes,
-                     "kernel_us": kern * 1e3},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "k_step_f32 (pull, moving, %s)" % ("owner table" if owners else "solid-velocity planes"),
+                     "bytes_per_cell": BYTES_F32_STATIC, "solid_cells": nsolid, "shared_cells": nshared,
+                     "bytes_per_launch": launch_bytes, "kernel_us": kern * 1e3, "launch": launch_cfg},
         "fp64_MLUPS": N * steps / w64 / 1e6,
         "force_rel_err_fp32_vs_fp64": {"max": float(eF.max()), "median": float(np.median(eF))},
         "torque_rel_err_fp32_vs_fp64": {"max": float(eT.max()), "median": float(np.median(eT))},

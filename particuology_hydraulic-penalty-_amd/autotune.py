@@ -47,7 +47,11 @@ NT = (0, 1)
 BPC = (0, 2, 3, 4, 6, 8)
 
 
-def candidates(nz: int):
+def candidates(nz: int, moving: bool = False):
+    """(block_z, block_y, non-temporal, workgroups per CU cap, flat) of the
+    launch search.  The flat mapping is searched where rows are not a whole
+    number of wavefronts, and for moving spheres, whose flat kernel runs at
+    six wavefronts per SIMD where the tiled one holds five."""
     out = []
     for bz, by in SHAPES:
         if bz > 64 and nz < bz // 2:
@@ -55,7 +59,7 @@ def candidates(nz: int):
         for nt in NT:
             for bpc in BPC:
                 out.append((bz, by, nt, bpc, False))
-    if nz % 64:
+    if nz % 64 or moving:
         for bz in FLAT:
             for nt in NT:
                 for bpc in BPC:
@@ -70,7 +74,7 @@ def _cache_path():
 
 # Part of every cache key: entries chosen from an earlier candidate set or by an
 # earlier selection rule are not reused.
-CANDIDATE_SET = 6
+CANDIDATE_SET = 7
 
 
 def _key(solver):
@@ -78,7 +82,8 @@ def _key(solver):
     name = torch.cuda.get_device_properties(solver.device).name
     arch = getattr(torch.cuda.get_device_properties(solver.device), "gcnArchName", "")
     return "|".join(str(v) for v in (name, arch, solver._shape, solver._storage, solver._layout,
-                                      bool(getattr(solver, "_moving", False)), "c%d" % CANDIDATE_SET))
+                                      bool(getattr(solver, "_moving", False)), getattr(solver, "_geom_mode", None),
+                                      "c%d" % CANDIDATE_SET))
 
 
 def _load_file():
@@ -109,23 +114,34 @@ def flags_of(nt: int, bpc: int, cpt: int = 1, flat: bool = False) -> int:
 
 
 def time_config(solver, bz, by, flags, repeats):
-    """Average psc_step duration (ms) of one launch configuration."""
+    """Average duration (ms) of one launch configuration of the step kernel
+    the solver runs (psc_step, or psc_step_owners for moving spheres with an
+    owner table)."""
     import torch
     nx, ny, nz = solver._shape
     fx, fy, fz = solver.body_force
     stream = torch.cuda.current_stream(solver.device)
     h = stream.cuda_stream
     us = solver._ptr(solver._us) if getattr(solver, "_moving", False) else None
-    args = (solver._ptr(solver._a), solver._ptr(solver._b), solver._ptr(solver._csum), us, solver._ptr(solver._imb),
-            solver._storage, solver._layout, nx, ny, nz, solver._s, solver.tau, fx, fy, fz, bz, by, flags, h)
     lib = _lib.load()
-    rc = lib.psc_step(*args)
-    _lib.check(rc, "psc_step")
+    if getattr(solver, "_geom_mode", None) == "owners":
+        parts, boxes = solver._dev_tables[0], solver._dev_tables[1]
+        fn, name = lib.psc_step_owners, "psc_step_owners"
+        args = (solver._ptr(solver._a), solver._ptr(solver._b), solver._ptr(solver._csum), solver._ptr(solver._own),
+                us, solver._ptr(parts), solver._ptr(boxes), solver._ptr(solver._imb), nx, ny, nz, solver._s,
+                solver.tau, fx, fy, fz, bz, by, flags, h)
+    else:
+        fn, name = lib.psc_step, "psc_step"
+        args = (solver._ptr(solver._a), solver._ptr(solver._b), solver._ptr(solver._csum), us,
+                solver._ptr(solver._imb), solver._storage, solver._layout, nx, ny, nz, solver._s, solver.tau, fx, fy,
+                fz, bz, by, flags, h)
+    rc = fn(*args)
+    _lib.check(rc, name)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(repeats):
-        lib.psc_step(*args)
+        fn(*args)
     e1.record(stream)
     e1.synchronize()
     return e0.elapsed_time(e1) / repeats
@@ -223,7 +239,7 @@ def tune(solver, budget_ms: float = 400.0, use_cache: bool = True, fused: bool =
             # the FP64 step takes no launch flags: workgroup shapes only
             cands = [(bz, by, 0, 0, False) for bz, by in SHAPES + FP64_SHAPES if not (bz > 64 and nz < bz // 2)]
         else:
-            cands = candidates(nz)
+            cands = candidates(nz, bool(getattr(solver, "_moving", False)))
         # one probe launch sizes the repeat count to the time budget
         probe = time_config(solver, 64, 4, 0, 1)
         reps = max(2, min(50, int(budget_ms / max(probe, 1e-3) / len(cands))))

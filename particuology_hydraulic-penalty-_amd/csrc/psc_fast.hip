@@ -293,19 +293,11 @@ __device__ __forceinline__ void step_cell(const float* gin, float* gout,
 // blockDim.x*blockDim.y consecutive cells of an x plane's flat (y, z) index,
 // so rows that are not a whole number of wavefronts leave no idle lanes
 // (136-cell rows keep 136 of 192 lanes busy with one row per z tile).
-template <bool PULL, bool MOVING, bool IMB, bool NT, int CPT, bool LIN>
+template <bool PULL, bool MOVING, bool IMB, bool NT, int CPT>
 __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin, float* __restrict__ gout,
                                                   const uint32_t* __restrict__ csum,
                                                   SolidMotion sm, float* __restrict__ imb, Dims d,
                                                   StepConsts k, float inv_nz) {
-  if (LIN) {
-    const uint32_t c = blockIdx.x * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
-    if (c >= (uint32_t)d.ny * (uint32_t)d.nz) return;
-    uint32_t iy, iz;
-    divmod_f(c, (uint32_t)d.nz, inv_nz, iy, iz);
-    step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, sm, imb, d, k, blockIdx.y, (int)iy, (int)iz);
-    return;
-  }
   const int iz = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy0 = (blockIdx.y * blockDim.y + threadIdx.y) * CPT;
   const int ix = blockIdx.z;
@@ -316,6 +308,21 @@ __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin,
     if (iy < d.ny)
       step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, sm, imb, d, k, ix, iy, iz);
   }
+}
+
+// The flat mapping.  Its register budget is held to six wavefronts per SIMD:
+// the static forms need fewer registers anyway, and the moving form then
+// fits without spilling (77 VGPRs instead of 93 unbounded, five wavefronts).
+template <bool PULL, bool MOVING, bool IMB, bool NT>
+__global__ __launch_bounds__(256, 6) void k_step_f32_flat(const float* __restrict__ gin, float* __restrict__ gout,
+                                                          const uint32_t* __restrict__ csum, SolidMotion sm,
+                                                          float* __restrict__ imb, Dims d, StepConsts k,
+                                                          float inv_nz) {
+  const uint32_t c = blockIdx.x * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
+  if (c >= (uint32_t)d.ny * (uint32_t)d.nz) return;
+  uint32_t iy, iz;
+  divmod_f(c, (uint32_t)d.nz, inv_nz, iy, iz);
+  step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, sm, imb, d, k, blockIdx.y, (int)iy, (int)iz);
 }
 
 // Static pull step with the particle-load reduction of earlier steps in the
@@ -628,13 +635,13 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
 #define PSC_F32_CASE(P, M, I, T)                                                                  \
   if (pull == P && moving == M && wimb == I && ntb == T) {                                       \
     if (lin)                                                                                     \
-      k_step_f32<P, M, I, T, 1, true><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,  \
+      k_step_f32_flat<P, M, I, T><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,      \
                                                                     inv_nz);                     \
     else if (cpt == 2)                                                                           \
-      k_step_f32<P, M, I, T, 2, false><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,  \
+      k_step_f32<P, M, I, T, 2><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,         \
                                                                      inv_nz);                    \
     else                                                                                         \
-      k_step_f32<P, M, I, T, 1, false><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,  \
+      k_step_f32<P, M, I, T, 1><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,         \
                                                                      inv_nz);                    \
     return check_launch("psc_step(f32)");                                                        \
   }

@@ -16,6 +16,10 @@ def test_candidates_and_flags():
     flat = [c for c in odd if c[4]]
     assert {c[0] for c in flat} == set(at.FLAT) and all(c[1] == 1 for c in flat)
     assert len(flat) == len(at.FLAT) * len(at.NT) * len(at.BPC)
+    # moving spheres: the flat mapping is searched at aligned rows too
+    moving = at.candidates(512, moving=True)
+    assert {c for c in moving if not c[4]} == set(at.candidates(512))
+    assert len([c for c in moving if c[4]]) == len(at.FLAT) * len(at.NT) * len(at.BPC)
     # flag layout of psc_step: bit 0 non-temporal, bits 8-15 cap, 16-17 cells per thread, bit 20 flat
     assert at.flags_of(1, 6) == 0x10601
     assert at.flags_of(0, 3, 2) == 0x20300

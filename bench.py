@@ -138,9 +138,8 @@ def step_kernel_ms(sol, repeats, torch, lib, moving=False, launch_cfg=None):
 
     def launch(a, b):
         if owners:  # the kernel the moving solver runs: solid velocity from the owner table
-            parts, boxes = sol._dev_tables[0], sol._dev_tables[1]
             lib.call("psc_step_owners", a.data_ptr(), b.data_ptr(), sol._csum.data_ptr(), sol._own.data_ptr(),
-                     sol._us.data_ptr(), parts.data_ptr(), boxes.data_ptr(), sol._imb.data_ptr(), nx, ny, nz,
+                     sol._us.data_ptr(), sol._motion.data_ptr(), sol._imb.data_ptr(), nx, ny, nz,
                      sol._s, sol.tau, fx, fy, fz, bz, by, flags, h)
             return
         lib.call("psc_step", a.data_ptr(), b.data_ptr(), sol._csum.data_ptr(),
@@ -426,7 +425,7 @@ def bench_moving(args, psc3d, workloads, lib, torch, np):
     eT = np.linalg.norm(loads32[:, 3:6] - loads64[:, 3:6], axis=1) / np.maximum(Ts, 1e-3 * Ts.max())
     # algorithmic bytes: 156 per cell plus, at the cells with solid, 12 of imb written and the
     # solid velocity: 12 of numerators (plane form) or 4 of owner id + 12 of numerators at the
-    # shared cells (owner table; the particle and box tables are a few hundred KB, cache-resident)
+    # shared cells (owner table; the motion records, 64 B per particle, are cache-resident)
     launch_bytes = BYTES_F32_STATIC * N + (16 * nsolid + 12 * nshared if owners else 24 * nsolid)
     achieved = launch_bytes / (kern * 1e-3) / 1e9
     out = {

@@ -125,11 +125,10 @@ def time_config(solver, bz, by, flags, repeats):
     us = solver._ptr(solver._us) if getattr(solver, "_moving", False) else None
     lib = _lib.load()
     if getattr(solver, "_geom_mode", None) == "owners":
-        parts, boxes = solver._dev_tables[0], solver._dev_tables[1]
         fn, name = lib.psc_step_owners, "psc_step_owners"
         args = (solver._ptr(solver._a), solver._ptr(solver._b), solver._ptr(solver._csum), solver._ptr(solver._own),
-                us, solver._ptr(parts), solver._ptr(boxes), solver._ptr(solver._imb), nx, ny, nz, solver._s,
-                solver.tau, fx, fy, fz, bz, by, flags, h)
+                us, solver._ptr(solver._motion), solver._ptr(solver._imb), nx, ny, nz, solver._s, solver.tau, fx, fy,
+                fz, bz, by, flags, h)
     else:
         fn, name = lib.psc_step, "psc_step"
         args = (solver._ptr(solver._a), solver._ptr(solver._b), solver._ptr(solver._csum), us,

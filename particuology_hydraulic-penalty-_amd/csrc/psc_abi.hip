@@ -41,9 +41,9 @@ int launch_step_f64(const double*, double*, const uint32_t*, const double*, doub
                     double, double, double, int, int, hipStream_t);
 int launch_step_f32(const float*, float*, const uint32_t*, const float*, float*, int, int, int, int, int, double,
                     double, double, double, int, int, int, hipStream_t, const uint32_t* = nullptr,
-                    const double* = nullptr, const int32_t* = nullptr);
-int launch_voxelize_owners(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, uint32_t*,
-                           uint32_t*, uint16_t*, int32_t*, uint32_t*, uint32_t*, float*, hipStream_t);
+                    const float* = nullptr);
+int launch_voxelize_owners(const double*, const int32_t*, int, const int32_t*, int, int, int, int, int, int, uint32_t*,
+                           uint32_t*, uint16_t*, int32_t*, uint32_t*, uint32_t*, float*, float*, hipStream_t);
 int launch_clear_owners(const uint32_t*, const uint16_t*, const int32_t*, int, int, int, int, int, uint32_t*, float*,
                         hipStream_t);
 int launch_voxelize(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, int, uint32_t*,
@@ -271,21 +271,24 @@ int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, in
                               d_us_num ? us_precision : 0, (hipStream_t)stream);
 }
 
-int psc_voxelize_owners(const double* d_particles, const int32_t* d_boxes, const int32_t* d_chunks, int32_t nchunks,
-                        int32_t chunk_cells, int32_t nx, int32_t ny, int32_t nz, int32_t subsamples,
-                        uint32_t* d_ent_cell, uint32_t* d_ent_loc, uint16_t* d_ent_cnt, int32_t* d_chunk_nnz,
-                        uint32_t* d_count_sum, uint32_t* d_owner, float* d_us_shared, void* stream) {
+int psc_voxelize_owners(const double* d_particles, const int32_t* d_boxes, int32_t nparticles,
+                        const int32_t* d_chunks, int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny,
+                        int32_t nz, int32_t subsamples, uint32_t* d_ent_cell, uint32_t* d_ent_loc,
+                        uint16_t* d_ent_cnt, int32_t* d_chunk_nnz, uint32_t* d_count_sum, uint32_t* d_owner,
+                        float* d_us_shared, float* d_motion, void* stream) {
   if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_voxelize_owners: bad lattice %dx%dx%d", nx, ny, nz);
   if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
     return set_error(PSC_EINVAL, "psc_voxelize_owners: subsamples %d outside [1, 40]", subsamples);
-  if (nchunks < 0 || chunk_cells < 1) return set_error(PSC_EINVAL, "psc_voxelize_owners: bad chunking");
-  if (nchunks > 0 && (d_particles == nullptr || d_boxes == nullptr || d_chunks == nullptr || d_ent_cell == nullptr ||
-                      d_ent_loc == nullptr || d_ent_cnt == nullptr || d_chunk_nnz == nullptr ||
-                      d_count_sum == nullptr || d_owner == nullptr || d_us_shared == nullptr))
+  if (nchunks < 0 || chunk_cells < 1 || nparticles < 0) return set_error(PSC_EINVAL, "psc_voxelize_owners: sizes");
+  if ((nparticles > 0 || nchunks > 0) && (d_particles == nullptr || d_boxes == nullptr || d_motion == nullptr))
+    return set_error(PSC_EINVAL, "psc_voxelize_owners: null particle, box or motion table");
+  if (nchunks > 0 && (d_chunks == nullptr || d_ent_cell == nullptr || d_ent_loc == nullptr || d_ent_cnt == nullptr ||
+                      d_chunk_nnz == nullptr || d_count_sum == nullptr || d_owner == nullptr ||
+                      d_us_shared == nullptr))
     return set_error(PSC_EINVAL, "psc_voxelize_owners: null pointer");
-  return launch_voxelize_owners(d_particles, d_boxes, d_chunks, nchunks, chunk_cells, nx, ny, nz, subsamples,
-                                d_ent_cell, d_ent_loc, d_ent_cnt, d_chunk_nnz, d_count_sum, d_owner, d_us_shared,
-                                (hipStream_t)stream);
+  return launch_voxelize_owners(d_particles, d_boxes, nparticles, d_chunks, nchunks, chunk_cells, nx, ny, nz,
+                                subsamples, d_ent_cell, d_ent_loc, d_ent_cnt, d_chunk_nnz, d_count_sum, d_owner,
+                                d_us_shared, d_motion, (hipStream_t)stream);
 }
 
 int psc_clear_entries_owners(const uint32_t* d_ent_cell, const uint16_t* d_ent_cnt, const int32_t* d_chunk_nnz,
@@ -301,17 +304,18 @@ int psc_clear_entries_owners(const uint32_t* d_ent_cell, const uint16_t* d_ent_c
 }
 
 int psc_step_owners(const float* d_in, float* d_out, const uint32_t* d_count_sum, const uint32_t* d_owner,
-                    const float* d_us_shared, const double* d_particles, const int32_t* d_boxes, float* d_imb,
-                    int32_t nx, int32_t ny, int32_t nz, int32_t subsamples, double tau, double fx, double fy,
-                    double fz, int32_t block_z, int32_t block_y, int32_t flags, void* stream) {
-  if (d_owner == nullptr || d_us_shared == nullptr || d_particles == nullptr || d_boxes == nullptr)
-    return set_error(PSC_EINVAL, "psc_step_owners: null owner table, numerators or particle tables");
+                    const float* d_us_shared, const float* d_motion, float* d_imb, int32_t nx, int32_t ny,
+                    int32_t nz, int32_t subsamples, double tau, double fx, double fy, double fz, int32_t block_z,
+                    int32_t block_y, int32_t flags, void* stream) {
+  if (d_owner == nullptr || d_us_shared == nullptr || d_motion == nullptr)
+    return set_error(PSC_EINVAL, "psc_step_owners: null owner table, numerators or motion table");
+  if (((uintptr_t)d_motion & 15) != 0) return set_error(PSC_EINVAL, "psc_step_owners: motion table not 16-byte aligned");
   int bz = block_z, by = block_y;
   const int rc = psc::step_args(d_in, d_out, d_count_sum, d_us_shared, PSC_F32_DEV, PSC_PULL, nx, ny, nz, subsamples,
                                 tau, flags, &bz, &by);
   if (rc) return rc;
   return launch_step_f32(d_in, d_out, d_count_sum, d_us_shared, d_imb, PSC_PULL, nx, ny, nz, subsamples, tau, fx, fy,
-                         fz, bz, by, flags, (hipStream_t)stream, d_owner, d_particles, d_boxes);
+                         fz, bz, by, flags, (hipStream_t)stream, d_owner, d_motion);
 }
 
 int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const void* d_us_num, void* d_imb,

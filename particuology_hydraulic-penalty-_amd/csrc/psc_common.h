@@ -74,14 +74,16 @@ constexpr uint32_t kSharedCell = 0xffffffffu;
 // Solid-velocity inputs of a moving FP32 step.  Without an owner table the
 // numerator planes hold sum_p alpha_p (U + omega x r)_p at every covered cell
 // (plane stride N); with one, they hold it at shared cells only, and a cell
-// with a single owner takes that particle's U + omega x r (particle table:
-// 10 doubles x, y, z, r, vx, vy, vz, wx, wy, wz; box table: 8 int32 per
-// particle, the box origin first).
+// with a single owner takes that particle's U + omega x r from its motion
+// record (kMotionFloats floats per particle, four 16-byte rows: box origin
+// x0, y0, z0 as int32; x0 + 1/2 - centre_x, y0 + 1/2 - centre_y,
+// z0 + 1/2 - centre_z, made in FP64 and rounded; U; omega), so that the
+// lever arm of a cell at box coordinate g is (g - x0) + (x0 + 1/2 - centre).
+constexpr int kMotionFloats = 16;
 struct SolidMotion {
   const float* us_num;
   const uint32_t* owner;
-  const double* parts;
-  const int32_t* boxes;
+  const float4* motion;
 };
 
 // Sum of K doubles per thread over the workgroup: wave64 butterfly then LDS across waves.

@@ -501,6 +501,8 @@ __global__ __launch_bounds__(kVoxThreads) void k_voxelize(const double* __restri
   if (lane == 0) chunk_nnz[blockIdx.x] = written;
 }
 
+constexpr int kSharedUnroll = 4;  // entries per thread per round of the owner-table passes
+
 // Owner table, second pass over the entries: a cell whose count sum differs
 // from the count of one of its entries is covered by more than one sphere.
 // Its owner slot is marked shared and each of its spheres adds
@@ -521,28 +523,41 @@ __global__ __launch_bounds__(256) void k_shared_owners(const double* __restrict_
   const size_t N = (size_t)d.ps;
   const double s3 = (double)(s * s * s);
   const int pid = chunks[2 * c];
+  const Box bx = load_box(boxes, pid);  // one sphere per chunk
+  const Particle p = load_particle(parts, pid);
   constexpr uint32_t kLocMask = (1u << kLocBits) - 1u;
-  for (int k = threadIdx.x; k < nnz; k += blockDim.x) {
-    const uint32_t cell = ent_cell[out0 + k];
-    const int cnt = ent_cnt[out0 + k];
-    if (csum[cell] == (uint32_t)cnt) continue;  // one owner
-    owner[cell] = kSharedCell;
-    const Box bx = load_box(boxes, pid);
-    const Particle p = load_particle(parts, pid);
-    const uint32_t loc = ent_loc[out0 + k];
-    const int gx = bx.x0 + (int)(loc & kLocMask);
-    const int gy = bx.y0 + (int)((loc >> kLocBits) & kLocMask);
-    const int gz = bx.z0 + (int)((loc >> (2 * kLocBits)) & kLocMask);
-    const double a = (double)cnt / s3;
-    const double rx = ((double)gx + 0.5) - p.x;
-    const double ry = ((double)gy + 0.5) - p.y;
-    const double rz = ((double)gz + 0.5) - p.z;
-    const double vx = (p.vx + p.wy * rz) - p.wz * ry;
-    const double vy = (p.vy + p.wz * rx) - p.wx * rz;
-    const double vz = (p.vz + p.wx * ry) - p.wy * rx;
-    unsafeAtomicAdd(us_num + cell, (float)(a * vx));
-    unsafeAtomicAdd(us_num + N + cell, (float)(a * vy));
-    unsafeAtomicAdd(us_num + 2 * N + cell, (float)(a * vz));
+  // kSharedUnroll entries per thread per round, their loads issued together
+  for (int k0 = threadIdx.x; k0 < nnz; k0 += kSharedUnroll * blockDim.x) {
+    uint32_t cell[kSharedUnroll], tot[kSharedUnroll];
+    int cnt[kSharedUnroll];
+#pragma unroll
+    for (int j = 0; j < kSharedUnroll; ++j) {
+      const int k = k0 + j * (int)blockDim.x;
+      cell[j] = k < nnz ? ent_cell[out0 + k] : 0u;
+      cnt[j] = k < nnz ? (int)ent_cnt[out0 + k] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kSharedUnroll; ++j) tot[j] = cnt[j] ? csum[cell[j]] : 0u;
+#pragma unroll
+    for (int j = 0; j < kSharedUnroll; ++j) {
+      if (cnt[j] == 0 || tot[j] == (uint32_t)cnt[j]) continue;  // past the entries, or one owner
+      const int k = k0 + j * (int)blockDim.x;
+      owner[cell[j]] = kSharedCell;
+      const uint32_t loc = ent_loc[out0 + k];
+      const int gx = bx.x0 + (int)(loc & kLocMask);
+      const int gy = bx.y0 + (int)((loc >> kLocBits) & kLocMask);
+      const int gz = bx.z0 + (int)((loc >> (2 * kLocBits)) & kLocMask);
+      const double a = (double)cnt[j] / s3;
+      const double rx = ((double)gx + 0.5) - p.x;
+      const double ry = ((double)gy + 0.5) - p.y;
+      const double rz = ((double)gz + 0.5) - p.z;
+      const double vx = (p.vx + p.wy * rz) - p.wz * ry;
+      const double vy = (p.vy + p.wz * rx) - p.wx * rz;
+      const double vz = (p.vz + p.wx * ry) - p.wy * rx;
+      unsafeAtomicAdd(us_num + cell[j], (float)(a * vx));
+      unsafeAtomicAdd(us_num + N + cell[j], (float)(a * vy));
+      unsafeAtomicAdd(us_num + 2 * N + cell[j], (float)(a * vz));
+    }
   }
 }
 
@@ -559,14 +574,27 @@ __global__ __launch_bounds__(256) void k_clear_owners(const uint32_t* __restrict
   const int c = blockIdx.x;
   const int nnz = chunk_nnz[c];
   const size_t out0 = (size_t)c * chunk_cells;
-  for (int k = threadIdx.x; k < nnz; k += blockDim.x) {
-    const uint32_t cell = ent_cell[out0 + k];
-    if (csum[cell] != (uint32_t)ent_cnt[out0 + k]) {
-      us_num[cell] = 0.0f;
-      us_num[N + cell] = 0.0f;
-      us_num[2 * N + cell] = 0.0f;
+  for (int k0 = threadIdx.x; k0 < nnz; k0 += kSharedUnroll * blockDim.x) {
+    uint32_t cell[kSharedUnroll], tot[kSharedUnroll];
+    int cnt[kSharedUnroll];
+#pragma unroll
+    for (int j = 0; j < kSharedUnroll; ++j) {
+      const int k = k0 + j * (int)blockDim.x;
+      cell[j] = k < nnz ? ent_cell[out0 + k] : 0u;
+      cnt[j] = k < nnz ? (int)ent_cnt[out0 + k] : 0;
     }
-    csum[cell] = 0u;
+#pragma unroll
+    for (int j = 0; j < kSharedUnroll; ++j) tot[j] = cnt[j] ? csum[cell[j]] : 0u;
+#pragma unroll
+    for (int j = 0; j < kSharedUnroll; ++j) {
+      if (cnt[j] == 0) continue;  // past the entries
+      if (tot[j] != (uint32_t)cnt[j]) {
+        us_num[cell[j]] = 0.0f;
+        us_num[N + cell[j]] = 0.0f;
+        us_num[2 * N + cell[j]] = 0.0f;
+      }
+      csum[cell[j]] = 0u;
+    }
   }
 }
 
@@ -985,10 +1013,43 @@ int launch_loads_final(const int32_t* chunk_first, int np, const double* partial
   return check_launch("psc_particle_loads(final)");
 }
 
-int launch_voxelize_owners(const double* parts, const int32_t* boxes, const int32_t* chunks, int nchunks,
+// Motion records of the owner table (layout at kMotionFloats): one thread per
+// particle, from the particle and box tables of the voxelization.
+__global__ __launch_bounds__(256) void k_motion_table(const double* __restrict__ parts,
+                                                      const int32_t* __restrict__ boxes, int np,
+                                                      float* __restrict__ motion) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const Particle p = load_particle(parts, i);
+  const Box b = load_box(boxes, i);
+  float* m = motion + (size_t)kMotionFloats * i;
+  m[0] = __int_as_float(b.x0);
+  m[1] = __int_as_float(b.y0);
+  m[2] = __int_as_float(b.z0);
+  m[3] = 0.0f;
+  m[4] = (float)(((double)b.x0 + 0.5) - p.x);
+  m[5] = (float)(((double)b.y0 + 0.5) - p.y);
+  m[6] = (float)(((double)b.z0 + 0.5) - p.z);
+  m[7] = 0.0f;
+  m[8] = (float)p.vx;
+  m[9] = (float)p.vy;
+  m[10] = (float)p.vz;
+  m[11] = 0.0f;
+  m[12] = (float)p.wx;
+  m[13] = (float)p.wy;
+  m[14] = (float)p.wz;
+  m[15] = 0.0f;
+}
+
+int launch_voxelize_owners(const double* parts, const int32_t* boxes, int np, const int32_t* chunks, int nchunks,
                            int chunk_cells, int nx, int ny, int nz, int s, uint32_t* ent_cell, uint32_t* ent_loc,
                            uint16_t* ent_cnt, int32_t* chunk_nnz, uint32_t* csum, uint32_t* owner, float* us_num,
-                           hipStream_t stream) {
+                           float* motion, hipStream_t stream) {
+  if (np > 0) {
+    k_motion_table<<<(np + 255) / 256, 256, 0, stream>>>(parts, boxes, np, motion);
+    const int rc = check_launch("psc_voxelize_owners(motion table)");
+    if (rc) return rc;
+  }
   if (nchunks == 0) return PSC_OK;
   const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
   void* own = owner;

@@ -287,6 +287,7 @@ class PscD3Q19:
         self._partials_alt = None
         self._us = None
         self._own = None          # owner table (one particle id per covered cell)
+        self._motion = None       # per-particle motion records of the owner table
         self._geom_mode = None    # how the current geometry was voxelized: count / planes / owners
         self._ent = None          # (ent_cell u32, ent_loc u32, ent_cnt u16) as int32/int32/int16 tensors
         self._old_ent = None
@@ -427,6 +428,8 @@ class PscD3Q19:
             _lib.call("psc_clear", self._ptr(self._us), 3 * N * self._us.element_size(), stream)
         if mode == "owners" and self._own is None:
             self._own = torch.empty(N, dtype=torch.int32, device=dev)
+        if mode == "owners" and (self._motion is None or self._motion.numel() < 16 * max(1, P)):
+            self._motion = torch.empty(16 * max(1, P), dtype=torch.float32, device=dev)
         old = self._tables
         if old is not None and self._old_ent is not None and self._us_clean:
             # zero only the cells the previous geometry touched (count_sum and us_num
@@ -450,10 +453,10 @@ class PscD3Q19:
         nx, ny, nz = self._shape
         ec, el, en = self._ent
         if mode == "owners":
-            _lib.call("psc_voxelize_owners", self._ptr(parts), self._ptr(boxes), self._ptr(chunks), C,
+            _lib.call("psc_voxelize_owners", self._ptr(parts), self._ptr(boxes), P, self._ptr(chunks), C,
                       tabs.chunk_cells, nx, ny, nz, self._s, self._ptr(ec), self._ptr(el), self._ptr(en),
                       self._ptr(self._chunk_nnz), self._ptr(self._csum), self._ptr(self._own), self._ptr(self._us),
-                      stream)
+                      self._ptr(self._motion), stream)
         else:
             us_ptr = self._ptr(self._us) if mode == "planes" else None
             usp = (1 if self._storage == _lib.PSC_F32_DEV else 2) if mode == "planes" else 0
@@ -628,9 +631,8 @@ class PscD3Q19:
         us_ptr = self._ptr(self._us) if self._moving else None
         self._imb = self._imb_bufs[0]
         if self._geom_mode == "owners":
-            parts, boxes = self._dev_tables[0], self._dev_tables[1]
             _lib.call("psc_step_owners", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum),
-                      self._ptr(self._own), us_ptr, self._ptr(parts), self._ptr(boxes), self._ptr(self._imb),
+                      self._ptr(self._own), us_ptr, self._ptr(self._motion), self._ptr(self._imb),
                       nx, ny, nz, self._s, self.tau, fx, fy, fz, self._block[0], self._block[1], self._flags, stream)
         else:
             _lib.call("psc_step", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum), us_ptr,

@@ -425,7 +425,7 @@ def bench_moving(args, psc3d, workloads, lib, torch, np):
     eT = np.linalg.norm(loads32[:, 3:6] - loads64[:, 3:6], axis=1) / np.maximum(Ts, 1e-3 * Ts.max())
     # algorithmic bytes: 156 per cell plus, at the cells with solid, 12 of imb written and the
     # solid velocity: 12 of numerators (plane form) or 4 of owner id + 12 of numerators at the
-    # shared cells (owner table; the motion records, 64 B per particle, are cache-resident)
+    # shared cells (owner table; the motion records, 96 B per particle, are cache-resident)
     launch_bytes = BYTES_F32_STATIC * N + (16 * nsolid + 12 * nshared if owners else 24 * nsolid)
     achieved = launch_bytes / (kern * 1e-3) / 1e9
     out = {

@@ -150,19 +150,18 @@ int psc_clear_entries(const uint32_t* d_ent_cell, const int32_t* d_chunk_nnz, in
  * alone covers the cell, or 0xffffffff where several spheres cover it; at
  * those shared cells only, d_us_shared (three FP32 planes of N cells,
  * zero elsewhere on entry) accumulates sum_p alpha_p (U + omega x r)_p as
- * psc_voxelize's d_us_num does everywhere.  d_motion (16 floats per
+ * psc_voxelize's d_us_num does everywhere.  d_motion (96 bytes per
  * particle, 16-byte aligned) receives each particle's motion record: box
- * origin (int32 bits), box origin + 1/2 - centre (made in FP64, rounded), U
- * and omega, one 16-byte row each.  psc_step_owners forms the single-owner
- * velocity from it, so the three solid-velocity atomics per covered cell are
- * gone.  d_count_sum must be cleared before the call; the owner table needs
+ * origin (3 int32 + 4 bytes of padding), then centre, U and omega (9 doubles
+ * + 8 bytes of padding).  psc_step_owners forms the single-owner velocity
+ * from it, so the three solid-velocity atomics per covered cell are gone.  d_count_sum must be cleared before the call; the owner table needs
  * no clearing.  Sphere boxes must span at most one period (as the periodic
  * table builder enforces). */
 int psc_voxelize_owners(const double* d_particles, const int32_t* d_boxes, int32_t nparticles,
                         const int32_t* d_chunks, int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny,
                         int32_t nz, int32_t subsamples, uint32_t* d_ent_cell, uint32_t* d_ent_loc,
                         uint16_t* d_ent_cnt, int32_t* d_chunk_nnz, uint32_t* d_count_sum, uint32_t* d_owner,
-                        float* d_us_shared, float* d_motion, void* stream);
+                        float* d_us_shared, void* d_motion, void* stream);
 
 /* The incremental clear of the owner-table form (psc3d.py:209-210 with
  * static_geometry=False): zero d_count_sum at every cell of a previous
@@ -175,12 +174,12 @@ int psc_clear_entries_owners(const uint32_t* d_ent_cell, const uint16_t* d_ent_c
 /* psc_step for FP32 deviation storage, pull layout, moving spheres voxelized
  * by psc_voxelize_owners (psc3d.py:208-246 with the solid velocity of
  * psc3d.py:181-206): at a cell with one owner the solid velocity is that
- * particle's U + omega x r from its motion record (FP32; the lever arm is
- * (box coordinate - box origin) + (box origin + 1/2 - centre)); at a shared
- * cell it is d_us_shared/(count_sum/s^3) as in psc_step.  d_motion: the
+ * particle's U + omega x r, formed from its motion record in FP64 with the
+ * voxelizer's lever arm (box coordinate + 1/2 - centre) and rounded to FP32;
+ * at a shared cell it is d_us_shared/(count_sum/s^3) as in psc_step.  d_motion: the
  * motion records of that voxelization.  Other arguments as psc_step. */
 int psc_step_owners(const float* d_in, float* d_out, const uint32_t* d_count_sum, const uint32_t* d_owner,
-                    const float* d_us_shared, const float* d_motion, float* d_imb, int32_t nx, int32_t ny,
+                    const float* d_us_shared, const void* d_motion, float* d_imb, int32_t nx, int32_t ny,
                     int32_t nz, int32_t subsamples, double tau, double fx, double fy, double fz, int32_t block_z,
                     int32_t block_y, int32_t flags, void* stream);
 

@@ -41,9 +41,9 @@ int launch_step_f64(const double*, double*, const uint32_t*, const double*, doub
                     double, double, double, int, int, hipStream_t);
 int launch_step_f32(const float*, float*, const uint32_t*, const float*, float*, int, int, int, int, int, double,
                     double, double, double, int, int, int, hipStream_t, const uint32_t* = nullptr,
-                    const float* = nullptr);
+                    const void* = nullptr);
 int launch_voxelize_owners(const double*, const int32_t*, int, const int32_t*, int, int, int, int, int, int, uint32_t*,
-                           uint32_t*, uint16_t*, int32_t*, uint32_t*, uint32_t*, float*, float*, hipStream_t);
+                           uint32_t*, uint16_t*, int32_t*, uint32_t*, uint32_t*, float*, void*, hipStream_t);
 int launch_clear_owners(const uint32_t*, const uint16_t*, const int32_t*, int, int, int, int, int, uint32_t*, float*,
                         hipStream_t);
 int launch_voxelize(const double*, const int32_t*, const int32_t*, int, int, int, int, int, int, int, uint32_t*,
@@ -275,7 +275,7 @@ int psc_voxelize_owners(const double* d_particles, const int32_t* d_boxes, int32
                         const int32_t* d_chunks, int32_t nchunks, int32_t chunk_cells, int32_t nx, int32_t ny,
                         int32_t nz, int32_t subsamples, uint32_t* d_ent_cell, uint32_t* d_ent_loc,
                         uint16_t* d_ent_cnt, int32_t* d_chunk_nnz, uint32_t* d_count_sum, uint32_t* d_owner,
-                        float* d_us_shared, float* d_motion, void* stream) {
+                        float* d_us_shared, void* d_motion, void* stream) {
   if (!dims_ok(nx, ny, nz)) return set_error(PSC_EINVAL, "psc_voxelize_owners: bad lattice %dx%dx%d", nx, ny, nz);
   if (subsamples < 1 || subsamples * subsamples * subsamples > 65535)
     return set_error(PSC_EINVAL, "psc_voxelize_owners: subsamples %d outside [1, 40]", subsamples);
@@ -304,7 +304,7 @@ int psc_clear_entries_owners(const uint32_t* d_ent_cell, const uint16_t* d_ent_c
 }
 
 int psc_step_owners(const float* d_in, float* d_out, const uint32_t* d_count_sum, const uint32_t* d_owner,
-                    const float* d_us_shared, const float* d_motion, float* d_imb, int32_t nx, int32_t ny,
+                    const float* d_us_shared, const void* d_motion, float* d_imb, int32_t nx, int32_t ny,
                     int32_t nz, int32_t subsamples, double tau, double fx, double fy, double fz, int32_t block_z,
                     int32_t block_y, int32_t flags, void* stream) {
   if (d_owner == nullptr || d_us_shared == nullptr || d_motion == nullptr)

@@ -75,15 +75,20 @@ constexpr uint32_t kSharedCell = 0xffffffffu;
 // numerator planes hold sum_p alpha_p (U + omega x r)_p at every covered cell
 // (plane stride N); with one, they hold it at shared cells only, and a cell
 // with a single owner takes that particle's U + omega x r from its motion
-// record (kMotionFloats floats per particle, four 16-byte rows: box origin
-// x0, y0, z0 as int32; x0 + 1/2 - centre_x, y0 + 1/2 - centre_y,
-// z0 + 1/2 - centre_z, made in FP64 and rounded; U; omega), so that the
-// lever arm of a cell at box coordinate g is (g - x0) + (x0 + 1/2 - centre).
-constexpr int kMotionFloats = 16;
+// record: box origin, centre, U and omega (FP64), so that the kernel forms
+// the voxelizer's lever arm (box coordinate + 1/2 - centre) and velocity in
+// FP64 and rounds once per cell.  Rounding U, omega or the centre to FP32
+// once per particle would give every cell of a sphere the same error, which
+// adds up in its load instead of averaging out.
+struct alignas(16) MotionRecord {
+  int32_t x0, y0, z0, pad;
+  double cx, cy, cz, vx, vy, vz, wx, wy, wz, pad2;
+};
+static_assert(sizeof(MotionRecord) == 96, "motion record layout");
 struct SolidMotion {
   const float* us_num;
   const uint32_t* owner;
-  const float4* motion;
+  const MotionRecord* motion;
 };
 
 // Sum of K doubles per thread over the workgroup: wave64 butterfly then LDS across waves.

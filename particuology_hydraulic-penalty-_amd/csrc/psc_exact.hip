@@ -1013,40 +1013,39 @@ int launch_loads_final(const int32_t* chunk_first, int np, const double* partial
   return check_launch("psc_particle_loads(final)");
 }
 
-// Motion records of the owner table (layout at kMotionFloats): one thread per
-// particle, from the particle and box tables of the voxelization.
+// Motion records of the owner table (MotionRecord): one thread per particle,
+// from the particle and box tables of the voxelization.
 __global__ __launch_bounds__(256) void k_motion_table(const double* __restrict__ parts,
                                                       const int32_t* __restrict__ boxes, int np,
-                                                      float* __restrict__ motion) {
+                                                      MotionRecord* __restrict__ motion) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const Particle p = load_particle(parts, i);
   const Box b = load_box(boxes, i);
-  float* m = motion + (size_t)kMotionFloats * i;
-  m[0] = __int_as_float(b.x0);
-  m[1] = __int_as_float(b.y0);
-  m[2] = __int_as_float(b.z0);
-  m[3] = 0.0f;
-  m[4] = (float)(((double)b.x0 + 0.5) - p.x);
-  m[5] = (float)(((double)b.y0 + 0.5) - p.y);
-  m[6] = (float)(((double)b.z0 + 0.5) - p.z);
-  m[7] = 0.0f;
-  m[8] = (float)p.vx;
-  m[9] = (float)p.vy;
-  m[10] = (float)p.vz;
-  m[11] = 0.0f;
-  m[12] = (float)p.wx;
-  m[13] = (float)p.wy;
-  m[14] = (float)p.wz;
-  m[15] = 0.0f;
+  MotionRecord m;
+  m.x0 = b.x0;
+  m.y0 = b.y0;
+  m.z0 = b.z0;
+  m.pad = 0;
+  m.cx = p.x;
+  m.cy = p.y;
+  m.cz = p.z;
+  m.vx = p.vx;
+  m.vy = p.vy;
+  m.vz = p.vz;
+  m.wx = p.wx;
+  m.wy = p.wy;
+  m.wz = p.wz;
+  m.pad2 = 0.0;
+  motion[i] = m;
 }
 
 int launch_voxelize_owners(const double* parts, const int32_t* boxes, int np, const int32_t* chunks, int nchunks,
                            int chunk_cells, int nx, int ny, int nz, int s, uint32_t* ent_cell, uint32_t* ent_loc,
                            uint16_t* ent_cnt, int32_t* chunk_nnz, uint32_t* csum, uint32_t* owner, float* us_num,
-                           float* motion, hipStream_t stream) {
+                           void* motion, hipStream_t stream) {
   if (np > 0) {
-    k_motion_table<<<(np + 255) / 256, 256, 0, stream>>>(parts, boxes, np, motion);
+    k_motion_table<<<(np + 255) / 256, 256, 0, stream>>>(parts, boxes, np, (MotionRecord*)motion);
     const int rc = check_launch("psc_voxelize_owners(motion table)");
     if (rc) return rc;
   }

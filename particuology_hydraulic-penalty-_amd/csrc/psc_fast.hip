@@ -150,17 +150,16 @@ __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, 
       const uint32_t c = cell;
       const uint32_t o = sm.owner != nullptr ? sm.owner[c] : kSharedCell;
       if (o != kSharedCell) {
-        // the one sphere covering the cell: U + omega x r, lever arm from the
-        // cell's box coordinate (found from the box origin; a box spans at
-        // most one period) as (g - x0) + (x0 + 1/2 - centre)
-        const float4* m = sm.motion + (kMotionFloats / 4) * (size_t)o;
-        const float4 q0 = m[0], q1 = m[1], q2 = m[2], q3 = m[3];
-        const float rx = (float)wrap_near(ix - __float_as_int(q0.x), d.nx) + q1.x;
-        const float ry = (float)wrap_near(iy - __float_as_int(q0.y), d.ny) + q1.y;
-        const float rz = (float)wrap_near(iz - __float_as_int(q0.z), d.nz) + q1.z;
-        sx = (q2.x + q3.y * rz) - q3.z * ry;
-        sy = (q2.y + q3.z * rx) - q3.x * rz;
-        sz = (q2.z + q3.x * ry) - q3.y * rx;
+        // the one sphere covering the cell: U + omega x r in FP64 with the
+        // voxelizer's lever arm (the cell's box coordinate, found from the
+        // box origin; a box spans at most one period), rounded to FP32
+        const MotionRecord& m = sm.motion[o];
+        const double rx = ((double)(m.x0 + wrap_near(ix - m.x0, d.nx)) + 0.5) - m.cx;
+        const double ry = ((double)(m.y0 + wrap_near(iy - m.y0, d.ny)) + 0.5) - m.cy;
+        const double rz = ((double)(m.z0 + wrap_near(iz - m.z0, d.nz)) + 0.5) - m.cz;
+        sx = (float)((m.vx + m.wy * rz) - m.wz * ry);
+        sy = (float)((m.vy + m.wz * rx) - m.wx * rz);
+        sz = (float)((m.vz + m.wx * ry) - m.wy * rx);
       } else {
         const float ia = 1.0f / ((float)cnt * k.inv_s3);
         sx = sm.us_num[c] * ia;
@@ -596,9 +595,9 @@ constexpr size_t kLdsHeadroom = 8192;
 // owner/motion (optional, with usn): the owner table of a moving step.
 int launch_step_f32(const float* in, float* out, const uint32_t* csum, const float* usn, float* imb, int layout,
                     int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz, int bz, int by,
-                    int nt, hipStream_t stream, const uint32_t* owner, const float* motion) {
+                    int nt, hipStream_t stream, const uint32_t* owner, const void* motion) {
   const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
-  const SolidMotion sm{usn, owner, (const float4*)motion};
+  const SolidMotion sm{usn, owner, (const MotionRecord*)motion};
   const int cpt = (nt >> 16) & 3 ? (nt >> 16) & 3 : 1;
   const int bpc = (nt >> 8) & 0xff;
   size_t lds = 0;

@@ -428,8 +428,8 @@ class PscD3Q19:
             _lib.call("psc_clear", self._ptr(self._us), 3 * N * self._us.element_size(), stream)
         if mode == "owners" and self._own is None:
             self._own = torch.empty(N, dtype=torch.int32, device=dev)
-        if mode == "owners" and (self._motion is None or self._motion.numel() < 16 * max(1, P)):
-            self._motion = torch.empty(16 * max(1, P), dtype=torch.float32, device=dev)
+        if mode == "owners" and (self._motion is None or self._motion.numel() < 12 * max(1, P)):
+            self._motion = torch.empty(12 * max(1, P), dtype=torch.float64, device=dev)  # 96 B records
         old = self._tables
         if old is not None and self._old_ent is not None and self._us_clean:
             # zero only the cells the previous geometry touched (count_sum and us_num

@@ -114,3 +114,38 @@ def test_owner_table_step_matches_planes_and_oracle(psc3d):
     assert fa <= 1e-3 and fab <= 1e-4
     for ua, ub in zip(A.solid_velocity(), B.solid_velocity()):
         np.testing.assert_allclose(ua, ub, rtol=1e-6, atol=1e-10)
+
+
+def test_owner_table_loads_as_accurate_as_planes(psc3d):
+    """Per-particle FP32 loads against the FP64 solver after 600 steps of the
+    moving-window workload at 128^3: the owner table is as accurate as the
+    plane form.  The velocity is rounded to FP32 once per cell; rounding U,
+    omega or the centre to FP32 once per particle instead gives every cell of
+    a sphere the same error, which adds up in its load (about ten times the
+    plane form's error on the particles with small loads)."""
+    from tests.conftest import load_package
+    wl = load_package("workloads")
+    n, steps = 128, 600
+    base = wl.moving_window(seed=0, n=n, count=int(round(2000 * (n / 512) ** 3)))
+    print(f"[owner accuracy] {n}^3, {len(base)} spheres, {steps} steps", flush=True)
+
+    def loads(precision, owner):
+        t = base.copy()
+        s = psc3d.SparsePscD3Q19(n, t, tau=0.8, subsamples=3, precision=precision, periodic_geometry=True,
+                                 device="cuda:0", owner_table=owner)
+        for _ in range(steps):
+            wl.advance_centres(t, n)
+            s.particles = t
+            s.run(1)
+        return s._loads_np()[:, 0:3].copy()
+
+    F64 = loads("fp64", None)
+    mag = np.linalg.norm(F64, axis=1)
+    floor = np.maximum(mag, 1e-3 * mag.max())
+    err = {}
+    for owner in (True, False):
+        e = np.linalg.norm(loads("fp32", owner) - F64, axis=1) / floor
+        err[owner] = (float(e.max()), float(np.median(e)))
+    print(f"  force error vs FP64 (max, median): owners {err[True]}, planes {err[False]}", flush=True)
+    assert err[True][0] <= 2.0 * err[False][0] + 1e-9
+    assert err[True][1] <= 2.0 * err[False][1] + 1e-10

@@ -107,6 +107,43 @@ int main() {
   expect(psc_copy_f32(nullptr, nullptr, 0, nullptr) == 0, "copy of zero");
   expect(psc_copy_f32(nullptr, nullptr, 5, nullptr) != 0, "copy null");
   expect(psc_clear_entries(nullptr, nullptr, 3, 2048, 8, 8, 8, nullptr, nullptr, 0, nullptr) != 0, "clear entries");
+  // owner table
+  expect(psc_voxelize_owners(nullptr, nullptr, 4, nullptr, 0, 2048, 8, 8, 8, 3, nullptr, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, nullptr, nullptr, nullptr) != 0 &&
+             msg_has("motion"),
+         "owner voxelize tables");
+  expect(psc_voxelize_owners(static_cast<const double*>(p64), static_cast<const int32_t*>(p32), 4, nullptr, 3, 2048,
+                             8, 8, 8, 3, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, p16,
+                             nullptr) != 0 &&
+             msg_has("null"),
+         "owner voxelize null");
+  expect(psc_voxelize_owners(nullptr, nullptr, -1, nullptr, 0, 2048, 8, 8, 8, 3, nullptr, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, nullptr, nullptr, nullptr) != 0,
+         "owner voxelize sizes");
+  expect(psc_voxelize_owners(nullptr, nullptr, 0, nullptr, 0, 2048, 8, 8, 8, 41, nullptr, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, nullptr, nullptr, nullptr) != 0 &&
+             msg_has("subsamples"),
+         "owner voxelize subsamples");
+  expect(psc_clear_entries_owners(nullptr, nullptr, nullptr, 3, 2048, 8, 8, 8, nullptr, nullptr, nullptr) != 0 &&
+             msg_has("null"),
+         "owner clear null");
+  expect(psc_clear_entries_owners(nullptr, nullptr, nullptr, 0, 2048, 8, 8, 8, nullptr, nullptr, nullptr) == 0,
+         "owner clear of nothing");
+  expect(psc_step_owners(static_cast<const float*>(p16), static_cast<float*>(p32), cs, nullptr, nullptr, nullptr,
+                         nullptr, 8, 8, 8, 3, 0.8, 0, 0, 0, 64, 1, 0, nullptr) != 0 &&
+             msg_has("owner"),
+         "owner step tables");
+  expect(psc_step_owners(static_cast<const float*>(p16), static_cast<float*>(p32), cs,
+                         static_cast<const uint32_t*>(p32), static_cast<const float*>(p16),
+                         static_cast<const char*>(p16) + 4, nullptr, 8, 8, 8, 3, 0.8, 0, 0, 0, 64, 1, 0,
+                         nullptr) != 0 &&
+             msg_has("aligned"),
+         "owner step motion alignment");
+  expect(psc_step_owners(static_cast<const float*>(p16), static_cast<float*>(p16), cs,
+                         static_cast<const uint32_t*>(p32), static_cast<const float*>(p16), p64, nullptr, 8, 8, 8, 3,
+                         0.8, 0, 0, 0, 64, 1, 0, nullptr) != 0 &&
+             msg_has("alias"),
+         "owner step alias");
   expect(psc_voxelize_binary(nullptr, nullptr, -1, nullptr, 8, 8, 8, nullptr, nullptr) != 0, "binary voxelize");
   expect(psc_smooth_alpha(nullptr, nullptr, nullptr, 0, 64, 8, 8, 8, nullptr, nullptr, nullptr, nullptr, nullptr) != 0,
          "smooth alpha scratch");

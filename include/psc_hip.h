@@ -64,10 +64,10 @@ extern "C" {
                                     /* updating one state buffer in place (AA       */
                                     /* pattern), half the footprint, same bits      */
 #define PSC_STEP_LINEAR 0x100000    /* bit 20: FP32 steps; a workgroup takes        */
-                                    /* block_z*block_y consecutive cells of an x    */
-                                    /* plane's flat (y, z) index instead of a       */
-                                    /* (z, y) tile; one cell per thread,            */
-                                    /* ny*nz < 2^24; no idle lanes at row ends      */
+                                    /* block_z*block_y consecutive cells of the     */
+                                    /* lattice's flat (x, y, z) index instead of a  */
+                                    /* (z, y) tile; one cell per thread; no idle    */
+                                    /* lanes at row ends, whole granules written    */
 
 #define PSC_NSUM 13 /* number of FP64 sums produced by psc_moments           */
 

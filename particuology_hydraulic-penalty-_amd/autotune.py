@@ -40,8 +40,9 @@ SHAPES = [(64, 4), (64, 2), (64, 1), (96, 2), (128, 2), (128, 1), (192, 1), (256
 # Extra workgroup shapes of the FP64 step (narrow z tiles keep more rows in flight)
 FP64_SHAPES = [(32, 4), (32, 8), (16, 16)]
 # Workgroup sizes of the flat mapping (PSC_STEP_LINEAR: consecutive cells of
-# an x plane's flat (y, z) index), tried when z rows are not a whole number of
-# wavefronts: a (z, y) tile then leaves the lanes past each row end idle.
+# the lattice's flat index), tried when z rows are not a whole number of
+# wavefronts: a (z, y) tile then leaves the lanes past each row end idle and
+# splits the 64-byte granules at row ends between workgroups.
 FLAT = (64, 128, 256)
 NT = (0, 1)
 BPC = (0, 2, 3, 4, 6, 8)

@@ -6,9 +6,12 @@ Four translation units:
   psc_exact.hip  FP64 kernels in the reference's operation order, compiled
                  with -ffp-contract=off (no fused multiply-add), plus the
                  voxelizer, particle loads, moments and f import/export;
-  psc_fast.hip   the FP32 deviation step, -ffp-contract=fast;
+  psc_fast.hip   the FP32 deviation step, -ffp-contract=fast-honor-pragmas
+                 (fused multiply-add wherever the source does not switch it
+                 off with #pragma clang fp contract(off); plain
+                 -ffp-contract=fast would ignore the pragma);
   psc_adjoint.hip  smooth solid fraction and the reverse-mode step,
-                 -ffp-contract=fast;
+                 -ffp-contract=fast-honor-pragmas;
   psc_abi.hip    the extern "C" layer.
 Packed FP32 instructions are disabled on every device compile.
 """
@@ -35,8 +38,8 @@ COMMON = [
 ]
 UNITS = [
     ("psc_exact.hip", ["-ffp-contract=off"]),
-    ("psc_fast.hip", ["-ffp-contract=fast"]),
-    ("psc_adjoint.hip", ["-ffp-contract=fast"]),
+    ("psc_fast.hip", ["-ffp-contract=fast-honor-pragmas"]),
+    ("psc_adjoint.hip", ["-ffp-contract=fast-honor-pragmas"]),
     ("psc_abi.hip", ["-ffp-contract=off"]),
 ]
 

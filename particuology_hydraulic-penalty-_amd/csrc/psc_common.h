@@ -46,6 +46,25 @@ __device__ __forceinline__ void divmod_f(uint32_t n, uint32_t d, float inv, uint
   r = (uint32_t)rr;
 }
 
+// Exact n / d for every n < 2^31 by a multiply and a shift (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", 1994,
+// theorem 4.2 with N = 31): with l = ceil(log2 d) and m = ceil(2^(31+l) / d),
+// 2^(31+l) <= m d < 2^(31+l) + 2^l, so floor(n m / 2^(31+l)) = floor(n / d);
+// m < 2^32 because d > 2^(l-1).
+struct Divider {
+  uint32_t m, sh, d;
+};
+inline Divider make_divider(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  const unsigned long long p = 1ull << (31 + l);
+  return Divider{(uint32_t)((p + d - 1) / d), 31 + l, d};
+}
+__device__ __forceinline__ void divmod_exact(uint32_t n, const Divider& v, uint32_t& q, uint32_t& r) {
+  q = (uint32_t)(((unsigned long long)n * v.m) >> v.sh);
+  r = n - q * v.d;
+}
+
 constexpr double kCs2 = 1.0 / 3.0;
 constexpr double kCs2Sq = kCs2 * kCs2;
 

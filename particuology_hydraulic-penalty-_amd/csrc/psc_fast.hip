@@ -52,6 +52,22 @@ struct Dims {
   unsigned long long ps;  // cells per direction plane (>= nx*ny*pz)
 };
 
+// The flat cell mapping: a workgroup takes consecutive cells of the whole
+// lattice's flat index c = (x ny + y) nz + z, so every workgroup's range
+// starts on a 1 KB boundary (256 cells) whatever the row length, and the
+// 64-byte granules at its ends are written by one workgroup only.
+struct FlatMap {
+  Divider nz, ny;
+};
+__device__ __forceinline__ void flat_cell(uint32_t c, const FlatMap& f, int& ix, int& iy, int& iz) {
+  uint32_t q, r, x, y;
+  divmod_exact(c, f.nz, q, r);
+  divmod_exact(q, f.ny, x, y);
+  ix = (int)x;
+  iy = (int)y;
+  iz = (int)r;
+}
+
 __device__ __forceinline__ uint32_t cell_id(const Dims d, int x, int y, int z) {
   return ((uint32_t)x * (uint32_t)d.ny + (uint32_t)y) * (uint32_t)d.pz + (uint32_t)z;
 }
@@ -87,6 +103,12 @@ template <bool MOVING>
 __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, const StepConsts& k,
                                              const SolidMotion& sm, uint32_t cell, size_t N, const Dims& d, int ix,
                                              int iy, int iz, float (&gn)[Q], float& mx, float& my, float& mz) {
+  // No implicit contraction: the fused operations are the explicit fmaf
+  // calls, so the bits depend on this source only, not on how the compiler
+  // fuses the code of each kernel and branch that inlines it (the fluid branch
+  // and the solid branch at B = 0 then agree exactly, and so do all launch
+  // forms, moving ones included).
+#pragma clang fp contract(off)
   const float drho = ((g[0] + (g[1] + g[2])) + ((g[3] + g[4]) + (g[5] + g[6]))) +
                      (((g[7] + g[8]) + (g[9] + g[10])) + ((g[11] + g[12]) + (g[13] + g[14]))) +
                      ((g[15] + g[16]) + (g[17] + g[18]));
@@ -152,7 +174,8 @@ __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, 
       if (o != kSharedCell) {
         // the one sphere covering the cell: U + omega x r in FP64 with the
         // voxelizer's lever arm (the cell's box coordinate, found from the
-        // box origin; a box spans at most one period), rounded to FP32
+        // box origin; a box spans at most one period), rounded to FP32.
+        // Unfused, as the voxelizer computes it.
         const MotionRecord& m = sm.motion[o];
         const double rx = ((double)(m.x0 + wrap_near(ix - m.x0, d.nx)) + 0.5) - m.cx;
         const double ry = ((double)(m.y0 + wrap_near(iy - m.y0, d.ny)) + 0.5) - m.cy;
@@ -296,7 +319,7 @@ template <bool PULL, bool MOVING, bool IMB, bool NT, int CPT>
 __global__ __launch_bounds__(256) void k_step_f32(const float* __restrict__ gin, float* __restrict__ gout,
                                                   const uint32_t* __restrict__ csum,
                                                   SolidMotion sm, float* __restrict__ imb, Dims d,
-                                                  StepConsts k, float inv_nz) {
+                                                  StepConsts k, FlatMap) {
   const int iz = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy0 = (blockIdx.y * blockDim.y + threadIdx.y) * CPT;
   const int ix = blockIdx.z;
@@ -316,12 +339,12 @@ template <bool PULL, bool MOVING, bool IMB, bool NT>
 __global__ __launch_bounds__(256, 6) void k_step_f32_flat(const float* __restrict__ gin, float* __restrict__ gout,
                                                           const uint32_t* __restrict__ csum, SolidMotion sm,
                                                           float* __restrict__ imb, Dims d, StepConsts k,
-                                                          float inv_nz) {
+                                                          FlatMap fm) {
   const uint32_t c = blockIdx.x * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
-  if (c >= (uint32_t)d.ny * (uint32_t)d.nz) return;
-  uint32_t iy, iz;
-  divmod_f(c, (uint32_t)d.nz, inv_nz, iy, iz);
-  step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, sm, imb, d, k, blockIdx.y, (int)iy, (int)iz);
+  if (c >= (uint32_t)d.ps) return;
+  int ix, iy, iz;
+  flat_cell(c, fm, ix, iy, iz);
+  step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, sm, imb, d, k, ix, iy, iz);
 }
 
 // Static pull step with the particle-load reduction of earlier steps in the
@@ -373,23 +396,23 @@ __device__ __forceinline__ bool fused_loads_workgroup(const FusedLoads& fl, cons
   return true;
 }
 
-// LIN: flat mapping as in k_step_f32; gz is then the number of workgroups per
-// x plane and gy is 1.
+// LIN: the flat mapping of k_step_f32_flat (cell workgroup b takes cells
+// b * blockDim.x * blockDim.y onwards); gz is then the number of cell
+// workgroups and gy is 1.
 template <bool NT, int CPT, bool LIN>
 __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict__ gin, float* __restrict__ gout,
                                                         const uint32_t* __restrict__ csum, float* __restrict__ imb,
                                                         Dims d, StepConsts k, FusedLoads fl, int gz, int gy,
-                                                        float inv_nz) {
+                                                        FlatMap fm) {
   int b = blockIdx.x;
   if (fused_loads_workgroup(fl, d, b)) return;
   if (LIN) {
-    const int ix = b / gz;
-    if (ix >= d.nx) return;
-    const uint32_t c = (uint32_t)(b - ix * gz) * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
-    if (c >= (uint32_t)d.ny * (uint32_t)d.nz) return;
-    uint32_t iy, iz;
-    divmod_f(c, (uint32_t)d.nz, inv_nz, iy, iz);
-    step_cell<true, false, true, NT>(gin, gout, csum, SolidMotion{}, imb, d, k, ix, (int)iy, (int)iz);
+    if (b >= gz) return;
+    const uint32_t c = (uint32_t)b * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
+    if (c >= (uint32_t)d.ps) return;
+    int ix, iy, iz;
+    flat_cell(c, fm, ix, iy, iz);
+    step_cell<true, false, true, NT>(gin, gout, csum, SolidMotion{}, imb, d, k, ix, iy, iz);
     return;
   }
   const int t = b / gz;
@@ -412,19 +435,15 @@ __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict_
 template <bool NT, bool LIN, int AA>
 __global__ __launch_bounds__(256) void k_step_f32_aa(float* g, const uint32_t* __restrict__ csum,
                                                      float* __restrict__ imb, Dims d, StepConsts k, FusedLoads fl,
-                                                     int gz, int gy, float inv_nz) {
+                                                     int gz, int gy, FlatMap fm) {
   int b = blockIdx.x;
   if (fused_loads_workgroup(fl, d, b)) return;
   int ix, iy, iz;
   if (LIN) {
-    ix = b / gz;
-    if (ix >= d.nx) return;
-    const uint32_t c = (uint32_t)(b - ix * gz) * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
-    if (c >= (uint32_t)d.ny * (uint32_t)d.nz) return;
-    uint32_t qy, qz;
-    divmod_f(c, (uint32_t)d.nz, inv_nz, qy, qz);
-    iy = (int)qy;
-    iz = (int)qz;
+    if (b >= gz) return;
+    const uint32_t c = (uint32_t)b * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
+    if (c >= (uint32_t)d.ps) return;
+    flat_cell(c, fm, ix, iy, iz);
   } else {
     const int t = b / gz;
     ix = t / gy;
@@ -608,14 +627,14 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
   const bool lin = (nt & PSC_STEP_LINEAR) != 0;
   const dim3 block(bz, by, 1);
   const int rows = (ny + cpt - 1) / cpt;
-  const unsigned per_plane = (unsigned)(((size_t)ny * nz + (size_t)bz * by - 1) / ((size_t)bz * by));
-  const dim3 grid = lin ? dim3(per_plane, nx, 1) : dim3((nz + bz - 1) / bz, (rows + by - 1) / by, nx);
-  const float inv_nz = 1.0f / (float)nz;
+  const size_t ncell = (size_t)nx * ny * nz;
+  const unsigned nflat = (unsigned)((ncell + (size_t)bz * by - 1) / ((size_t)bz * by));
+  const dim3 grid = lin ? dim3(nflat, 1, 1) : dim3((nz + bz - 1) / bz, (rows + by - 1) / by, nx);
+  const FlatMap fm{make_divider((uint32_t)nz), make_divider((uint32_t)ny)};
   if (cpt != 1 && cpt != 2) return set_error(PSC_EINVAL, "psc_step(f32): cells per thread %d", cpt);
-  if ((size_t)nx * ny * nz > ((size_t)1 << 30))
+  if (ncell > ((size_t)1 << 30))
     return set_error(PSC_EINVAL, "psc_step(f32): at most 2^30 cells (32-bit byte offsets in a plane)");
-  if (lin && (cpt != 1 || (size_t)ny * nz >= (1u << 24)))
-    return set_error(PSC_EINVAL, "psc_step(f32): the flat mapping needs one cell per thread and ny*nz < 2^24");
+  if (lin && cpt != 1) return set_error(PSC_EINVAL, "psc_step(f32): the flat mapping takes one cell per thread");
   StepConsts k;
   k.field = s == 0;
   k.inv_s3 = k.field ? 0.0f : (float)(1.0 / (double)(s * s * s));
@@ -634,13 +653,13 @@ int launch_step_f32(const float* in, float* out, const uint32_t* csum, const flo
   if (pull == P && moving == M && wimb == I && ntb == T) {                                       \
     if (lin)                                                                                     \
       k_step_f32_flat<P, M, I, T><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,      \
-                                                                    inv_nz);                     \
+                                                                    fm);                         \
     else if (cpt == 2)                                                                           \
       k_step_f32<P, M, I, T, 2><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,         \
-                                                                     inv_nz);                    \
+                                                                     fm);                        \
     else                                                                                         \
       k_step_f32<P, M, I, T, 1><<<grid, block, lds, stream>>>(in, out, csum, sm, imb, d, k,         \
-                                                                     inv_nz);                    \
+                                                                     fm);                        \
     return check_launch("psc_step(f32)");                                                        \
   }
 #define PSC_F32_CASES(T)                 \
@@ -685,13 +704,12 @@ int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, flo
   if ((size_t)nx * ny * nz > ((size_t)1 << 30))
     return set_error(PSC_EINVAL, "psc_step(f32+loads): at most 2^30 cells (32-bit byte offsets in a plane)");
   const bool lin = (nt & PSC_STEP_LINEAR) != 0;
-  if (lin && (cpt != 1 || (size_t)ny * nz >= (1u << 24)))
-    return set_error(PSC_EINVAL, "psc_step(f32+loads): the flat mapping needs one cell per thread and ny*nz < 2^24");
+  if (lin && cpt != 1) return set_error(PSC_EINVAL, "psc_step(f32+loads): the flat mapping takes one cell per thread");
   const int rows = (ny + cpt - 1) / cpt;
-  const int gz = lin ? (int)(((size_t)ny * nz + (size_t)bz * by - 1) / ((size_t)bz * by)) : (nz + bz - 1) / bz;
+  const int gz = lin ? (int)(((size_t)nx * ny * nz + (size_t)bz * by - 1) / ((size_t)bz * by)) : (nz + bz - 1) / bz;
   const int gy = lin ? 1 : (rows + by - 1) / by;
-  const float inv_nz = 1.0f / (float)nz;
-  const long long ncell = (long long)gz * gy * nx;
+  const FlatMap fm{make_divider((uint32_t)nz), make_divider((uint32_t)ny)};
+  const long long ncell = lin ? (long long)gz : (long long)gz * gy * nx;
   FusedLoads fl{la, imb_prev, part_out, part_prev, loads, 0, 0, 1};
   if (imb_prev != nullptr) {
     if (part_out == nullptr) return set_error(PSC_EINVAL, "psc_step(f32+loads): null partials");
@@ -726,7 +744,7 @@ int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, flo
   if (aa > 0) {
 #define PSC_F32AA_CASE(T, L, A)                                                                        \
     if (ntb == T && lin == L && aa == A) {                                                             \
-      k_step_f32_aa<T, L, A><<<g, block, lds, stream>>>(out, csum, imb, d, k, fl, gz, gy, inv_nz);     \
+      k_step_f32_aa<T, L, A><<<g, block, lds, stream>>>(out, csum, imb, d, k, fl, gz, gy, fm);         \
       return check_launch("psc_step(f32+loads, in place)");                                            \
     }
     PSC_F32AA_CASE(true, false, 1) PSC_F32AA_CASE(true, false, 2) PSC_F32AA_CASE(false, false, 1)
@@ -735,7 +753,7 @@ int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, flo
 #undef PSC_F32AA_CASE
     return set_error(PSC_EINVAL, "psc_step(f32+loads): no in-place kernel variant");
   }
-#define PSC_F32L_ARGS in, out, csum, imb, d, k, fl, gz, gy, inv_nz
+#define PSC_F32L_ARGS in, out, csum, imb, d, k, fl, gz, gy, fm
   if (ntb && lin) k_step_f32_loads<true, 1, true><<<g, block, lds, stream>>>(PSC_F32L_ARGS);
   else if (lin) k_step_f32_loads<false, 1, true><<<g, block, lds, stream>>>(PSC_F32L_ARGS);
   else if (ntb && cpt == 2) k_step_f32_loads<true, 2, false><<<g, block, lds, stream>>>(PSC_F32L_ARGS);

@@ -149,3 +149,34 @@ def test_owner_table_loads_as_accurate_as_planes(psc3d):
     print(f"  force error vs FP64 (max, median): owners {err[True]}, planes {err[False]}", flush=True)
     assert err[True][0] <= 2.0 * err[False][0] + 1e-9
     assert err[True][1] <= 2.0 * err[False][1] + 1e-10
+
+
+def test_owner_table_step_same_bits_tiled_and_flat(psc3d):
+    """The moving owner-table step gives the same bits with every cell
+    mapping the tuner may pick ((z, y) tiles, two cells per thread, the flat
+    mapping over the whole lattice), on a lattice whose row length (37 cells)
+    splits 64-byte granules."""
+    import torch
+    from tests.conftest import load_package
+    lib = load_package("_lib")
+    rng = np.random.default_rng(33)
+    n = 37
+    t = _table(rng, n, 8, 2.5, 6.0, 2e-3)
+    sol = psc3d.SparsePscD3Q19(n, t, tau=0.8, subsamples=3, periodic_geometry=True, precision="fp32",
+                               device="cuda:0", block=(64, 4))
+    for _ in range(3):
+        t[:, 0:3] = np.mod(t[:, 0:3] + t[:, 4:7], n)
+        sol.particles = t
+        sol.collide_stream_psc()
+    assert sol._geom_mode == "owners"
+    h = torch.cuda.current_stream().cuda_stream
+    ref = None
+    for bz, by, flags in [(64, 4, 0), (64, 2, 0x20000), (256, 1, 0x10401), (64, 1, lib.PSC_STEP_LINEAR),
+                          (256, 1, lib.PSC_STEP_LINEAR | 0x401), (128, 2, lib.PSC_STEP_LINEAR)]:
+        lib.call("psc_step_owners", sol._a.data_ptr(), sol._b.data_ptr(), sol._csum.data_ptr(),
+                 sol._own.data_ptr(), sol._us.data_ptr(), sol._motion.data_ptr(), sol._imb.data_ptr(), n, n, n,
+                 sol._s, sol.tau, 0.0, 0.0, 1e-6, bz, by, flags, h)
+        out = (sol._b.cpu().numpy().copy(), sol._imb.cpu().numpy()[sol._csum.cpu().numpy().repeat(3) != 0].copy())
+        if ref is None:
+            ref = out
+        assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), (bz, by, hex(flags))

@@ -298,11 +298,36 @@ def read_traffic(name, launch):
 COPY_PROBE_BYTES = 256 << 20  # each buffer of the calibration copy in the traffic probe
 
 
+def _copy_calibration(lib, torch):
+    """The streaming copy kernel on 256 MiB buffers (read 256 MiB, write
+    256 MiB per dispatch), three times: the counter calibration of a traffic
+    probe."""
+    m = COPY_PROBE_BYTES // 4
+    src = torch.ones(m, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        lib.call("psc_copy_f32", src.data_ptr(), dst.data_ptr(), m, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+
+
 def traffic_probe(args, psc3d, workloads, lib, torch):
     """Child of measure_traffic, run under rocprofv3 --pmc: the timed
     configuration with the launch fixed (--launch, --fuse-period) for
-    12 direct steps, then the streaming copy kernel on 256 MiB buffers
-    (read 256 MiB, write 256 MiB per dispatch) as the counter calibration."""
+    12 direct steps, then the copy calibration.  With --workload windows: the
+    batched windows with --batch-launch and --fuse-period for 12 steps of
+    fused launches (no graph), then the copy calibration."""
+    if args.workload == "windows":
+        batch = importlib.import_module(PKG + ".batch")
+        f_, g_, m_ = args.batch_launch.split(",")
+        wins = workloads.pebble_windows(seed=0)
+        B = batch.BatchedPscD3Q19([(L, t) for L, t, _ in wins], tau=0.8, subsamples=3, body_force=(0.0, 0.0, 1.0e-6),
+                                  device="cuda:0", launch=(int(f_, 0), int(g_), m_))
+        if args.fuse_period is not None:
+            B.fuse_period = args.fuse_period
+        B.run(12, graph=False)
+        torch.cuda.synchronize()
+        _copy_calibration(lib, torch)
+        return
     n = args.n
     bz, by, flags = (int(v) for v in args.launch.split(","))
     sol = psc3d.PscD3Q19(n, workloads.fixed_array(seed=0, n=n), tau=0.8, subsamples=4, body_force=(1.0e-6, 0.0, 0.0),
@@ -314,20 +339,16 @@ def traffic_probe(args, psc3d, workloads, lib, torch):
     sol.rebuild_geometry()
     sol.run(12, graph=False)
     torch.cuda.synchronize()
-    m = COPY_PROBE_BYTES // 4
-    src = torch.ones(m, dtype=torch.float32, device="cuda:0")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        lib.call("psc_copy_f32", src.data_ptr(), dst.data_ptr(), m, torch.cuda.current_stream().cuda_stream)
-    torch.cuda.synchronize()
+    _copy_calibration(lib, torch)
 
 
-def measure_traffic(args, launch, period, kernel_prefix, quiet):
+def measure_traffic(args, launch, period, kernel_prefix, quiet, probe_args=None, timeout=180):
     """HBM traffic per launch of the timed step kernel: two rocprofv3 --pmc
     passes over ``bench.py --traffic-probe`` with the same launch (FETCH_SIZE,
     then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
-    reads for the copy kernel of known size in that run.  Returns a dict, or
-    None when rocprofv3 is missing or a pass fails."""
+    reads for the copy kernel of known size in that run.  probe_args: the
+    probe's arguments when they are not the default workload's.  Returns a
+    dict, or None when rocprofv3 is missing or a pass fails."""
     import csv
     import shutil
     import statistics
@@ -341,12 +362,15 @@ def measure_traffic(args, launch, period, kernel_prefix, quiet):
     env = dict(os.environ, TMPDIR="/tmp")
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="psc_traffic_", dir="/tmp")
+        if probe_args is None:
+            probe_args = ["--n", str(args.n), "--layout", args.layout, "--launch",
+                          ",".join(str(int(v)) for v in launch), "--fuse-period", str(int(period))]
         cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "p", "--", sys.executable, "-u",
-               os.path.join(ROOT, "bench.py"), "--traffic-probe", "--n", str(args.n), "--layout", args.layout,
-               "--launch", ",".join(str(int(v)) for v in launch), "--fuse-period", str(int(period))]
+               os.path.join(ROOT, "bench.py"), "--traffic-probe"] + list(probe_args)
         log(f"traffic: rocprofv3 --pmc {counter} over the probe", quiet)
         try:
-            subprocess.run(cmd, cwd=ROOT, env=env, timeout=180, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            subprocess.run(cmd, cwd=ROOT, env=env, timeout=timeout, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL)
         except (subprocess.TimeoutExpired, OSError):
             shutil.rmtree(d, ignore_errors=True)
             return None
@@ -541,11 +565,7 @@ def bench_windows(args, workloads, torch, np):
                    "windows": len(wins), "total_cells": cells, "particles": int(B.P)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": read_traffic("k_step_f32_batched_loads_windows_pmc.txt" if fused
-                                             else "k_step_f32_batched_windows_pmc.txt",
-                                             (B._flags, B._grid, 512 if B._flags & 0x40000 else 256,
-                                              int(B._mode == "rows"),
-                                              B.effective_fuse_period() if fused else 0)),
+                     "traffic": None,
                      "kernel": kname,
                      "bytes_per_cell": BYTES_F32_STATIC, "bytes_per_launch": launch_bytes,
                      "solid_cells": nsolid, "imb_bytes": ibytes,
@@ -574,6 +594,16 @@ def bench_windows(args, workloads, torch, np):
                      "rel_err": abs(float(k[w]) - k64) / abs(k64)})
         del s64
     out["k_fp32_vs_fp64_smallest_windows"] = errs
+    if fused and not args.no_traffic:
+        # HBM/fabric bytes per fused launch, measured on this launch by two counter passes
+        probe = ["--workload", "windows", "--batch-launch", f"{B._flags:#x},{B._grid},{B._mode}",
+                 "--fuse-period", str(B.effective_fuse_period())]
+        tr = measure_traffic(args, None, None, "k_step_f32_batched<", args.quiet, probe_args=probe, timeout=400)
+        if tr is not None:
+            tr["source"] = "measured in this run: rocprofv3 --pmc over bench.py --traffic-probe, same launch"
+            out["roofline"]["traffic"] = tr["bytes_per_launch"]
+            out["roofline"]["traffic_over_algorithmic"] = tr["bytes_per_launch"] / launch_bytes
+            out["roofline"]["traffic_detail"] = tr
     if not args.no_cpu_baseline:
         from oracle import psc_oracle as O
         w0 = int(np.argmin([L for L, _, _ in wins]))

@@ -139,8 +139,8 @@ def step_kernel_ms(sol, repeats, torch, lib, moving=False, launch_cfg=None):
     def launch(a, b):
         if owners:  # the kernel the moving solver runs: solid velocity from the owner table
             lib.call("psc_step_owners", a.data_ptr(), b.data_ptr(), sol._csum.data_ptr(), sol._own.data_ptr(),
-                     sol._us.data_ptr(), sol._motion.data_ptr(), sol._imb.data_ptr(), nx, ny, nz,
-                     sol._s, sol.tau, fx, fy, fz, bz, by, flags, h)
+                     sol._us.data_ptr(), sol._motion.data_ptr(), int(sol._tables.particles.shape[0]),
+                     sol._imb.data_ptr(), nx, ny, nz, sol._s, sol.tau, fx, fy, fz, bz, by, flags, h)
             return
         lib.call("psc_step", a.data_ptr(), b.data_ptr(), sol._csum.data_ptr(),
                  sol._us.data_ptr() if moving else None, sol._imb.data_ptr(),

@@ -177,11 +177,13 @@ int psc_clear_entries_owners(const uint32_t* d_ent_cell, const uint16_t* d_ent_c
  * particle's U + omega x r, formed from its motion record in FP64 with the
  * voxelizer's lever arm (box coordinate + 1/2 - centre) and rounded to FP32;
  * at a shared cell it is d_us_shared/(count_sum/s^3) as in psc_step.  d_motion: the
- * motion records of that voxelization.  Other arguments as psc_step. */
+ * nparticles motion records of that voxelization; an owner id at or past
+ * nparticles (an owner table that does not belong to them) reads d_us_shared
+ * instead of the records.  Other arguments as psc_step. */
 int psc_step_owners(const float* d_in, float* d_out, const uint32_t* d_count_sum, const uint32_t* d_owner,
-                    const float* d_us_shared, const void* d_motion, float* d_imb, int32_t nx, int32_t ny,
-                    int32_t nz, int32_t subsamples, double tau, double fx, double fy, double fz, int32_t block_z,
-                    int32_t block_y, int32_t flags, void* stream);
+                    const float* d_us_shared, const void* d_motion, int32_t nparticles, float* d_imb, int32_t nx,
+                    int32_t ny, int32_t nz, int32_t subsamples, double tau, double fx, double fy, double fz,
+                    int32_t block_z, int32_t block_y, int32_t flags, void* stream);
 
 /* K2 -- fused PSC collision + streaming, one lattice step.
  * Replaces PscD3Q19.collide_stream_psc (psc3d.py:208-246) with equilibrium

@@ -47,8 +47,8 @@ SIGNATURES = {
     "psc_voxelize_owners": ([_P, _P, _I32, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P,
                              _P], ct.c_int),
     "psc_clear_entries_owners": ([_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P], ct.c_int),
-    "psc_step_owners": ([_P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _I32, _P],
-                        ct.c_int),
+    "psc_step_owners": ([_P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _I32,
+                         _P], ct.c_int),
     "psc_step": ([_P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _I32, _P],
                  ct.c_int),
     "psc_particle_loads": ([_P, _P, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P,

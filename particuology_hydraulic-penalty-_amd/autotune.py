@@ -128,8 +128,8 @@ def time_config(solver, bz, by, flags, repeats):
     if getattr(solver, "_geom_mode", None) == "owners":
         fn, name = lib.psc_step_owners, "psc_step_owners"
         args = (solver._ptr(solver._a), solver._ptr(solver._b), solver._ptr(solver._csum), solver._ptr(solver._own),
-                us, solver._ptr(solver._motion), solver._ptr(solver._imb), nx, ny, nz, solver._s, solver.tau, fx, fy,
-                fz, bz, by, flags, h)
+                us, solver._ptr(solver._motion), int(solver._tables.particles.shape[0]), solver._ptr(solver._imb), nx,
+                ny, nz, solver._s, solver.tau, fx, fy, fz, bz, by, flags, h)
     else:
         fn, name = lib.psc_step, "psc_step"
         args = (solver._ptr(solver._a), solver._ptr(solver._b), solver._ptr(solver._csum), us,

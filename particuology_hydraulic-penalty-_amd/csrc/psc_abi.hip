@@ -41,7 +41,7 @@ int launch_step_f64(const double*, double*, const uint32_t*, const double*, doub
                     double, double, double, int, int, hipStream_t);
 int launch_step_f32(const float*, float*, const uint32_t*, const float*, float*, int, int, int, int, int, double,
                     double, double, double, int, int, int, hipStream_t, const uint32_t* = nullptr,
-                    const void* = nullptr);
+                    const void* = nullptr, int = 0);
 int launch_voxelize_owners(const double*, const int32_t*, int, const int32_t*, int, int, int, int, int, int, uint32_t*,
                            uint32_t*, uint16_t*, int32_t*, uint32_t*, uint32_t*, float*, void*, hipStream_t);
 int launch_clear_owners(const uint32_t*, const uint16_t*, const int32_t*, int, int, int, int, int, uint32_t*, float*,
@@ -304,18 +304,19 @@ int psc_clear_entries_owners(const uint32_t* d_ent_cell, const uint16_t* d_ent_c
 }
 
 int psc_step_owners(const float* d_in, float* d_out, const uint32_t* d_count_sum, const uint32_t* d_owner,
-                    const float* d_us_shared, const void* d_motion, float* d_imb, int32_t nx, int32_t ny,
-                    int32_t nz, int32_t subsamples, double tau, double fx, double fy, double fz, int32_t block_z,
-                    int32_t block_y, int32_t flags, void* stream) {
-  if (d_owner == nullptr || d_us_shared == nullptr || d_motion == nullptr)
+                    const float* d_us_shared, const void* d_motion, int32_t nparticles, float* d_imb, int32_t nx,
+                    int32_t ny, int32_t nz, int32_t subsamples, double tau, double fx, double fy, double fz,
+                    int32_t block_z, int32_t block_y, int32_t flags, void* stream) {
+  if (d_owner == nullptr || d_us_shared == nullptr || (nparticles > 0 && d_motion == nullptr))
     return set_error(PSC_EINVAL, "psc_step_owners: null owner table, numerators or motion table");
+  if (nparticles < 0) return set_error(PSC_EINVAL, "psc_step_owners: nparticles %d", nparticles);
   if (((uintptr_t)d_motion & 15) != 0) return set_error(PSC_EINVAL, "psc_step_owners: motion table not 16-byte aligned");
   int bz = block_z, by = block_y;
   const int rc = psc::step_args(d_in, d_out, d_count_sum, d_us_shared, PSC_F32_DEV, PSC_PULL, nx, ny, nz, subsamples,
                                 tau, flags, &bz, &by);
   if (rc) return rc;
   return launch_step_f32(d_in, d_out, d_count_sum, d_us_shared, d_imb, PSC_PULL, nx, ny, nz, subsamples, tau, fx, fy,
-                         fz, bz, by, flags, (hipStream_t)stream, d_owner, d_motion);
+                         fz, bz, by, flags, (hipStream_t)stream, d_owner, d_motion, nparticles);
 }
 
 int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const void* d_us_num, void* d_imb,

@@ -108,6 +108,7 @@ struct SolidMotion {
   const float* us_num;
   const uint32_t* owner;
   const MotionRecord* motion;
+  uint32_t nmotion;  // records in motion; an owner id at or past it reads the numerator planes
 };
 
 // Sum of K doubles per thread over the workgroup: wave64 butterfly then LDS across waves.

@@ -171,7 +171,7 @@ __device__ __forceinline__ void collide_cell(const float (&g)[Q], uint32_t cnt, 
     if (MOVING && cnt != 0) {
       const uint32_t c = cell;
       const uint32_t o = sm.owner != nullptr ? sm.owner[c] : kSharedCell;
-      if (o != kSharedCell) {
+      if (o < sm.nmotion) {  // kSharedCell, and any id past the motion records, read the planes
         // the one sphere covering the cell: U + omega x r in FP64 with the
         // voxelizer's lever arm (the cell's box coordinate, found from the
         // box origin; a box spans at most one period), rounded to FP32.
@@ -611,12 +611,12 @@ __global__ __launch_bounds__(BS) void k_step_f32_batched(const float* __restrict
 constexpr size_t kLdsPerCU = 163840;
 constexpr size_t kLdsHeadroom = 8192;
 
-// owner/motion (optional, with usn): the owner table of a moving step.
+// owner/motion/nmotion (optional, with usn): the owner table of a moving step.
 int launch_step_f32(const float* in, float* out, const uint32_t* csum, const float* usn, float* imb, int layout,
                     int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz, int bz, int by,
-                    int nt, hipStream_t stream, const uint32_t* owner, const void* motion) {
+                    int nt, hipStream_t stream, const uint32_t* owner, const void* motion, int nmotion) {
   const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
-  const SolidMotion sm{usn, owner, (const MotionRecord*)motion};
+  const SolidMotion sm{usn, owner, (const MotionRecord*)motion, owner != nullptr ? (uint32_t)nmotion : 0u};
   const int cpt = (nt >> 16) & 3 ? (nt >> 16) & 3 : 1;
   const int bpc = (nt >> 8) & 0xff;
   size_t lds = 0;

@@ -632,8 +632,9 @@ class PscD3Q19:
         self._imb = self._imb_bufs[0]
         if self._geom_mode == "owners":
             _lib.call("psc_step_owners", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum),
-                      self._ptr(self._own), us_ptr, self._ptr(self._motion), self._ptr(self._imb),
-                      nx, ny, nz, self._s, self.tau, fx, fy, fz, self._block[0], self._block[1], self._flags, stream)
+                      self._ptr(self._own), us_ptr, self._ptr(self._motion), int(self._tables.particles.shape[0]),
+                      self._ptr(self._imb), nx, ny, nz, self._s, self.tau, fx, fy, fz, self._block[0], self._block[1],
+                      self._flags, stream)
         else:
             _lib.call("psc_step", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum), us_ptr,
                       self._ptr(self._imb), self._storage, self._layout, nx, ny, nz, self._s, self.tau, fx, fy, fz,

@@ -130,18 +130,23 @@ int main() {
   expect(psc_clear_entries_owners(nullptr, nullptr, nullptr, 0, 2048, 8, 8, 8, nullptr, nullptr, nullptr) == 0,
          "owner clear of nothing");
   expect(psc_step_owners(static_cast<const float*>(p16), static_cast<float*>(p32), cs, nullptr, nullptr, nullptr,
-                         nullptr, 8, 8, 8, 3, 0.8, 0, 0, 0, 64, 1, 0, nullptr) != 0 &&
+                         4, nullptr, 8, 8, 8, 3, 0.8, 0, 0, 0, 64, 1, 0, nullptr) != 0 &&
              msg_has("owner"),
          "owner step tables");
   expect(psc_step_owners(static_cast<const float*>(p16), static_cast<float*>(p32), cs,
+                         static_cast<const uint32_t*>(p32), static_cast<const float*>(p16), p64, -1, nullptr, 8, 8, 8,
+                         3, 0.8, 0, 0, 0, 64, 1, 0, nullptr) != 0 &&
+             msg_has("nparticles"),
+         "owner step particle count");
+  expect(psc_step_owners(static_cast<const float*>(p16), static_cast<float*>(p32), cs,
                          static_cast<const uint32_t*>(p32), static_cast<const float*>(p16),
-                         static_cast<const char*>(p16) + 4, nullptr, 8, 8, 8, 3, 0.8, 0, 0, 0, 64, 1, 0,
+                         static_cast<const char*>(p16) + 4, 4, nullptr, 8, 8, 8, 3, 0.8, 0, 0, 0, 64, 1, 0,
                          nullptr) != 0 &&
              msg_has("aligned"),
          "owner step motion alignment");
   expect(psc_step_owners(static_cast<const float*>(p16), static_cast<float*>(p16), cs,
-                         static_cast<const uint32_t*>(p32), static_cast<const float*>(p16), p64, nullptr, 8, 8, 8, 3,
-                         0.8, 0, 0, 0, 64, 1, 0, nullptr) != 0 &&
+                         static_cast<const uint32_t*>(p32), static_cast<const float*>(p16), p64, 4, nullptr, 8, 8, 8,
+                         3, 0.8, 0, 0, 0, 64, 1, 0, nullptr) != 0 &&
              msg_has("alias"),
          "owner step alias");
   expect(psc_voxelize_binary(nullptr, nullptr, -1, nullptr, 8, 8, 8, nullptr, nullptr) != 0, "binary voxelize");

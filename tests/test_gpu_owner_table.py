@@ -174,9 +174,38 @@ def test_owner_table_step_same_bits_tiled_and_flat(psc3d):
     for bz, by, flags in [(64, 4, 0), (64, 2, 0x20000), (256, 1, 0x10401), (64, 1, lib.PSC_STEP_LINEAR),
                           (256, 1, lib.PSC_STEP_LINEAR | 0x401), (128, 2, lib.PSC_STEP_LINEAR)]:
         lib.call("psc_step_owners", sol._a.data_ptr(), sol._b.data_ptr(), sol._csum.data_ptr(),
-                 sol._own.data_ptr(), sol._us.data_ptr(), sol._motion.data_ptr(), sol._imb.data_ptr(), n, n, n,
-                 sol._s, sol.tau, 0.0, 0.0, 1e-6, bz, by, flags, h)
+                 sol._own.data_ptr(), sol._us.data_ptr(), sol._motion.data_ptr(), len(t), sol._imb.data_ptr(), n, n,
+                 n, sol._s, sol.tau, 0.0, 0.0, 1e-6, bz, by, flags, h)
         out = (sol._b.cpu().numpy().copy(), sol._imb.cpu().numpy()[sol._csum.cpu().numpy().repeat(3) != 0].copy())
         if ref is None:
             ref = out
         assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), (bz, by, hex(flags))
+
+
+def test_owner_ids_past_the_motion_records_read_the_planes(psc3d):
+    """An owner id at or past the motion-record count (an owner table that
+    does not belong to the records) is not looked up: such a cell takes its
+    velocity from the numerator planes.  With a count of 0 every cell does,
+    so the owner-table step equals psc_step on the same planes, bit for bit."""
+    import torch
+    from tests.conftest import load_package
+    lib = load_package("_lib")
+    rng = np.random.default_rng(34)
+    n = 32
+    t = _table(rng, n, 8, 2.5, 6.0, 2e-3)
+    sol = psc3d.SparsePscD3Q19(n, t, tau=0.8, subsamples=3, periodic_geometry=True, precision="fp32",
+                               device="cuda:0", block=(64, 4))
+    sol.collide_stream_psc()
+    assert sol._geom_mode == "owners"
+    h = torch.cuda.current_stream().cuda_stream
+    lib.call("psc_step_owners", sol._a.data_ptr(), sol._b.data_ptr(), sol._csum.data_ptr(), sol._own.data_ptr(),
+             sol._us.data_ptr(), sol._motion.data_ptr(), 0, sol._imb.data_ptr(), n, n, n, sol._s, sol.tau, 0.0, 0.0,
+             1e-6, 64, 4, 0, h)
+    got = sol._b.cpu().numpy().copy()
+    lib.call("psc_step", sol._a.data_ptr(), sol._b.data_ptr(), sol._csum.data_ptr(), sol._us.data_ptr(),
+             sol._imb.data_ptr(), lib.PSC_F32_DEV, lib.PSC_PULL, n, n, n, sol._s, sol.tau, 0.0, 0.0, 1e-6, 64, 4, 0, h)
+    assert np.array_equal(got, sol._b.cpu().numpy())
+    lib.call("psc_step_owners", sol._a.data_ptr(), sol._b.data_ptr(), sol._csum.data_ptr(), sol._own.data_ptr(),
+             sol._us.data_ptr(), sol._motion.data_ptr(), len(t), sol._imb.data_ptr(), n, n, n, sol._s, sol.tau, 0.0,
+             0.0, 1e-6, 64, 4, 0, h)
+    assert not np.array_equal(got, sol._b.cpu().numpy())  # with the records the single owners move

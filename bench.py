@@ -133,6 +133,7 @@ def step_kernel_ms(sol, repeats, torch, lib, moving=False, launch_cfg=None):
     stream = torch.cuda.current_stream()
     h = stream.cuda_stream
     bz, by, flags = launch_cfg if launch_cfg is not None else (sol._block[0], sol._block[1], sol._flags)
+    flags |= getattr(sol, "_state_flags", 0)  # the state's plane layout
 
     owners = moving and getattr(sol, "_geom_mode", None) == "owners"
 

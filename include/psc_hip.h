@@ -63,6 +63,12 @@ extern "C" {
                                     /* in the fused form only: steps go in pairs    */
                                     /* updating one state buffer in place (AA       */
                                     /* pattern), half the footprint, same bits      */
+#define PSC_STEP_PLANES64 0x400000  /* bit 22: FP32 steps; each direction plane of  */
+                                    /* the state holds nx*ny*nz rounded up to 64    */
+                                    /* cells (256-byte aligned planes; allocate     */
+                                    /* 19 such planes; psc_moments reads them with  */
+                                    /* pitch_z = nz); count_sum, imb and us_num     */
+                                    /* keep nx*ny*nz cells                          */
 #define PSC_STEP_LINEAR 0x100000    /* bit 20: FP32 steps; a workgroup takes        */
                                     /* block_z*block_y consecutive cells of the     */
                                     /* lattice's flat (x, y, z) index instead of a  */
@@ -309,9 +315,10 @@ int psc_particle_loads(const double* d_particles, const int32_t* d_boxes, const 
  * PSC_ALPHA_FIELD.  d_partials needs psc_moments_blocks(N)*
  * PSC_NSUM doubles.  Deterministic (fixed-order two-pass reduction).
  * d_imb may be NULL (sums 8-10 are then 0).  pitch_z (0 = nz): row pitch of a
- * padded lattice (padding cells are skipped; fields need pitch_z = 0); with
- * pitch_z > 0 each direction plane holds nx*ny*pitch_z cells rounded up to a
- * multiple of 64 (the batched-window layout). */
+ * padded lattice (padding cells are skipped; fields need pitch_z = 0 or nz,
+ * and are written as 4 planes of nx*ny*nz); with pitch_z > 0 each direction
+ * plane holds nx*ny*pitch_z cells rounded up to a multiple of 64 (the
+ * batched-window layout; pitch_z = nz reads a PSC_STEP_PLANES64 state). */
 int psc_moments(const void* d_state, int32_t storage, int32_t layout, const uint32_t* d_count_sum,
                 const void* d_imb, int32_t nx, int32_t ny, int32_t nz, int32_t pitch_z,
                 int32_t subsamples, double fx,

@@ -24,6 +24,7 @@ PSC_BATCH_WG512 = 0x40000
 PSC_BATCH_IMB_NT = 0x80000
 PSC_STEP_LINEAR = 0x100000
 PSC_RUN_INPLACE = 0x200000
+PSC_STEP_PLANES64 = 0x400000
 
 # Names and argument types of every exported entry point, in header order.
 _P = ct.c_void_p

@@ -75,7 +75,7 @@ def _cache_path():
 
 # Part of every cache key: entries chosen from an earlier candidate set or by an
 # earlier selection rule are not reused.
-CANDIDATE_SET = 7
+CANDIDATE_SET = 8
 
 
 def _key(solver):
@@ -117,8 +117,9 @@ def flags_of(nt: int, bpc: int, cpt: int = 1, flat: bool = False) -> int:
 def time_config(solver, bz, by, flags, repeats):
     """Average duration (ms) of one launch configuration of the step kernel
     the solver runs (psc_step, or psc_step_owners for moving spheres with an
-    owner table)."""
+    owner table).  flags: launch flags; the state's layout flag is added."""
     import torch
+    flags = int(flags) | getattr(solver, "_state_flags", 0)
     nx, ny, nz = solver._shape
     fx, fy, fz = solver.body_force
     stream = torch.cuda.current_stream(solver.device)
@@ -169,7 +170,8 @@ def time_fused_config(solver, scratch, bz, by, flags, repeats, period=None, grap
     p.state_a, p.state_b = a.data_ptr(), b.data_ptr()
     p.imb, p.imb_alt = i0.data_ptr(), i1.data_ptr()
     p.partials, p.partials_alt, p.loads = q0.data_ptr(), q1.data_ptr(), ld.data_ptr()
-    p.block_z, p.block_y, p.flags = bz, by, flags  # flags may hold PSC_RUN_INPLACE
+    # flags may hold PSC_RUN_INPLACE; the state's layout flag is added
+    p.block_z, p.block_y, p.flags = bz, by, int(flags) | getattr(solver, "_state_flags", 0)
     if period is not None:
         p.fuse_loads = int(period)
     stream = torch.cuda.current_stream(solver.device)

@@ -164,8 +164,11 @@ int step_args(const void* d_in, const void* d_out, const uint32_t* d_count_sum, 
   // default: 64-cell row tiles, 32 when rows are a multiple of 32 but not of 64
   if (*bz <= 0) *bz = (nz >= 64 && !(nz % 64 != 0 && nz % 32 == 0)) ? 64 : 32;
   if (*by <= 0) *by = 256 / *bz > 0 ? 256 / *bz : 1;
-  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK | PSC_STEP_LINEAR)) != 0)
+  if ((flags & ~(PSC_STEP_NONTEMPORAL | PSC_STEP_BPC_MASK | PSC_STEP_CPT_MASK | PSC_STEP_LINEAR | PSC_STEP_PLANES64)) !=
+      0)
     return set_error(PSC_EINVAL, "psc_step: unknown flags 0x%x", flags);
+  if ((flags & PSC_STEP_PLANES64) && storage != PSC_F32_DEV)
+    return set_error(PSC_EINVAL, "psc_step: 64-cell-aligned planes (PSC_STEP_PLANES64) need FP32 storage");
   if (((flags & PSC_STEP_CPT_MASK) >> 16) > 2) return set_error(PSC_EINVAL, "psc_step: cells per thread > 2");
   if ((flags & PSC_STEP_LINEAR) && ((flags & PSC_STEP_CPT_MASK) >> 16) > 1)
     return set_error(PSC_EINVAL, "psc_step: the flat mapping takes one cell per thread");
@@ -427,7 +430,7 @@ int moments_args(const char* who, const void* d_state, int32_t storage, int32_t 
     return set_error(PSC_EINVAL, "%s: subsamples %d", who, subsamples);
   *pz = pitch_z > 0 ? pitch_z : nz;
   if (*pz < nz || !dims_ok(nx, ny, *pz)) return set_error(PSC_EINVAL, "%s: pitch_z %d < nz %d", who, pitch_z, nz);
-  if (pitch_z > 0 && d_rho_u != nullptr) return set_error(PSC_EINVAL, "%s: fields need an unpadded lattice", who);
+  if (*pz > nz && d_rho_u != nullptr) return set_error(PSC_EINVAL, "%s: fields need unpadded rows", who);
   // explicit pitch: planes of nx*ny*pitch_z cells rounded up to 64 (batched-window layout)
   const unsigned long long n = (unsigned long long)nx * ny * *pz;
   *ps = pitch_z > 0 ? (n + 63ull) / 64ull * 64ull : n;

@@ -108,8 +108,15 @@ struct SolidMotion {
   const float* us_num;
   const uint32_t* owner;
   const MotionRecord* motion;
-  uint32_t nmotion;  // records in motion; an owner id at or past it reads the numerator planes
+  uint32_t nmotion;             // records in motion; an owner id at or past it reads the numerator planes
+  unsigned long long us_plane;  // cells per numerator plane (nx*ny*nz, whatever the state's plane size)
 };
+
+// Cells per direction plane of a single lattice's FP32 state: nx*ny*nz, or,
+// with 64-cell-aligned planes, that rounded up to 64.
+__host__ __device__ inline unsigned long long state_plane(unsigned long long ncell, bool aligned64) {
+  return aligned64 ? (ncell + 63ull) / 64ull * 64ull : ncell;
+}
 
 // Sum of K doubles per thread over the workgroup: wave64 butterfly then LDS across waves.
 // Result valid in thread 0.  blockDim.x must be a multiple of 64, <= 1024.

@@ -750,11 +750,11 @@ __global__ __launch_bounds__(kMomThreads) void k_moments(const TS* __restrict__ 
       const double ux = (moment<0>(f) + hfx) / rho;
       const double uy = (moment<1>(f) + hfy) / rho;
       const double uz = (moment<2>(f) + hfz) / rho;
-      if (rho_u != nullptr) {
+      if (rho_u != nullptr) {  // fields: planes of ncell (no row padding here)
         rho_u[c] = rho;
-        rho_u[N + c] = ux;
-        rho_u[2 * N + c] = uy;
-        rho_u[3 * N + c] = uz;
+        rho_u[ncell + c] = ux;
+        rho_u[2 * ncell + c] = uy;
+        rho_u[3 * ncell + c] = uz;
       }
       const bool finite = isfinite(rho) && isfinite(ux) && isfinite(uy) && isfinite(uz);
       acc[2] += ux;

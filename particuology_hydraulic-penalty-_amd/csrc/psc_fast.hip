@@ -277,7 +277,7 @@ __device__ __forceinline__ void step_cell(const float* gin, float* gout,
   // collision, so every store keeps the uniform-base + 32-bit-offset form.
   float gn[Q];
   float mx, my, mz;  // imb of a solid cell (0 in fluid)
-  collide_cell<MOVING>(g, cnt, k, sm, cell_id(d, ix, iy, iz), (size_t)d.ps, d, ix, iy, iz, gn, mx, my, mz);
+  collide_cell<MOVING>(g, cnt, k, sm, cell_id(d, ix, iy, iz), (size_t)sm.us_plane, d, ix, iy, iz, gn, mx, my, mz);
   // imb is defined at solid cells only.  Lanes write it in groups of 16: a
   // group that holds a solid cell writes all 16 lanes (zeros at fluid cells),
   // so that 16 consecutive cells starting at a multiple of 16 fill three
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256, 6) void k_step_f32_flat(const float* __restric
                                                           float* __restrict__ imb, Dims d, StepConsts k,
                                                           FlatMap fm) {
   const uint32_t c = blockIdx.x * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
-  if (c >= (uint32_t)d.ps) return;
+  if (c >= (uint32_t)d.nx * (uint32_t)d.ny * (uint32_t)d.nz) return;
   int ix, iy, iz;
   flat_cell(c, fm, ix, iy, iz);
   step_cell<PULL, MOVING, IMB, NT>(gin, gout, csum, sm, imb, d, k, ix, iy, iz);
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(256) void k_step_f32_loads(const float* __restrict_
   if (LIN) {
     if (b >= gz) return;
     const uint32_t c = (uint32_t)b * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
-    if (c >= (uint32_t)d.ps) return;
+    if (c >= (uint32_t)d.nx * (uint32_t)d.ny * (uint32_t)d.nz) return;
     int ix, iy, iz;
     flat_cell(c, fm, ix, iy, iz);
     step_cell<true, false, true, NT>(gin, gout, csum, SolidMotion{}, imb, d, k, ix, iy, iz);
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(256) void k_step_f32_aa(float* g, const uint32_t* _
   if (LIN) {
     if (b >= gz) return;
     const uint32_t c = (uint32_t)b * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
-    if (c >= (uint32_t)d.ps) return;
+    if (c >= (uint32_t)d.nx * (uint32_t)d.ny * (uint32_t)d.nz) return;
     flat_cell(c, fm, ix, iy, iz);
   } else {
     const int t = b / gz;
@@ -615,8 +615,9 @@ constexpr size_t kLdsHeadroom = 8192;
 int launch_step_f32(const float* in, float* out, const uint32_t* csum, const float* usn, float* imb, int layout,
                     int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz, int bz, int by,
                     int nt, hipStream_t stream, const uint32_t* owner, const void* motion, int nmotion) {
-  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
-  const SolidMotion sm{usn, owner, (const MotionRecord*)motion, owner != nullptr ? (uint32_t)nmotion : 0u};
+  const unsigned long long N = (unsigned long long)nx * ny * nz;
+  const Dims d{nx, ny, nz, nz, state_plane(N, (nt & PSC_STEP_PLANES64) != 0)};
+  const SolidMotion sm{usn, owner, (const MotionRecord*)motion, owner != nullptr ? (uint32_t)nmotion : 0u, N};
   const int cpt = (nt >> 16) & 3 ? (nt >> 16) & 3 : 1;
   const int bpc = (nt >> 8) & 0xff;
   size_t lds = 0;
@@ -688,7 +689,7 @@ int launch_step_f32_loads(const float* in, float* out, const uint32_t* csum, flo
                           const LoadsArgs& la, double* part_out, const double* part_prev, double* loads, int period,
                           int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz, int bz, int by,
                           int nt, hipStream_t stream, int aa) {
-  const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
+  const Dims d{nx, ny, nz, nz, state_plane((unsigned long long)nx * ny * nz, (nt & PSC_STEP_PLANES64) != 0)};
   const int cpt = (nt >> 16) & 3 ? (nt >> 16) & 3 : 1;
   const int bpc = (nt >> 8) & 0xff;
   size_t lds = 0;

@@ -278,8 +278,15 @@ class PscD3Q19:
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self._fdt = torch.float32 if self._storage == _lib.PSC_F32_DEV else torch.float64
         N = self._N
-        self._a = torch.empty(19 * N, dtype=self._fdt, device=self.device)
-        self._b = torch.empty(19 * N, dtype=self._fdt, device=self.device)
+        # FP32 direction planes start on 256-byte boundaries: planes of N cells with 4N
+        # not a multiple of 64 bytes would leave every plane after the first misaligned,
+        # and the step's stores would then cover partial 64-byte granules, which HBM
+        # completes by read-modify-write (PSC_STEP_PLANES64: planes of N rounded up to 64)
+        planes64 = self._storage == _lib.PSC_F32_DEV and N % 64 != 0
+        self._state_flags = _lib.PSC_STEP_PLANES64 if planes64 else 0
+        self._ps = -(-N // 64) * 64 if planes64 else N  # cells per direction plane of the state
+        self._a = torch.empty(19 * self._ps, dtype=self._fdt, device=self.device)
+        self._b = torch.empty(19 * self._ps, dtype=self._fdt, device=self.device)
         self._csum = torch.zeros(N, dtype=torch.int32, device=self.device)
         # imb of the last step; a second buffer is added for the pipelined loop
         self._imb_bufs = [torch.zeros(3 * N, dtype=self._fdt, device=self.device), None]
@@ -306,10 +313,15 @@ class PscD3Q19:
         self._cache = {}
         self._plan_epoch = 0      # bumped whenever a buffer that run()'s plan holds is replaced
         self._plan_cache = None
-        _lib.call("psc_init_state", self._ptr(self._a), self._storage, N, self._stream())
+        _lib.call("psc_init_state", self._ptr(self._a), self._storage, self._ps, self._stream())
         self.geometry_workers = 1
 
     # ------------------------------------------------------------------ utils
+    @property
+    def _step_flags(self):
+        """Flag word of the step calls: the launch flags and the state layout."""
+        return self._flags | self._state_flags
+
     def _stream(self):
         return _torch().cuda.current_stream(self.device).cuda_stream
 
@@ -586,7 +598,7 @@ class PscD3Q19:
         p.fx, p.fy, p.fz = self._force
         p.block_z, p.block_y = self._block
         inplace = self._inplace_tuned if self.inplace is None else bool(self.inplace)
-        p.flags = self._flags | (_lib.PSC_RUN_INPLACE if fused and inplace else 0)
+        p.flags = self._step_flags | (_lib.PSC_RUN_INPLACE if fused and inplace else 0)
         tabs = self._tables
         if tabs is not None and tabs.particles.shape[0] > 0:
             parts, boxes, chunks, cfirst = self._dev_tables
@@ -634,11 +646,11 @@ class PscD3Q19:
             _lib.call("psc_step_owners", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum),
                       self._ptr(self._own), us_ptr, self._ptr(self._motion), int(self._tables.particles.shape[0]),
                       self._ptr(self._imb), nx, ny, nz, self._s, self.tau, fx, fy, fz, self._block[0], self._block[1],
-                      self._flags, stream)
+                      self._step_flags, stream)
         else:
             _lib.call("psc_step", self._ptr(self._a), self._ptr(self._b), self._ptr(self._csum), us_ptr,
                       self._ptr(self._imb), self._storage, self._layout, nx, ny, nz, self._s, self.tau, fx, fy, fz,
-                      self._block[0], self._block[1], self._flags, stream)
+                      self._block[0], self._block[1], self._step_flags, stream)
         self._a, self._b = self._b, self._a
         self._step_count += 1
         self._cache.clear()
@@ -670,7 +682,8 @@ class PscD3Q19:
             self._rho_u = torch.empty(4 * self._N, dtype=torch.float64, device=self.device)
         fx, fy, fz = self._force
         _lib.call("psc_moments", self._ptr(self._a) if with_state else None, self._storage, self._layout,
-                  self._ptr(self._csum), self._ptr(self._imb), nx, ny, nz, 0, self._s, fx, fy, fz,
+                  self._ptr(self._csum), self._ptr(self._imb), nx, ny, nz, nz if self._state_flags else 0, self._s,
+                  fx, fy, fz,
                   self._ptr(self._rho_u) if fields else None, self._ptr(self._mom_partials), self._ptr(self._sums),
                   self._stream())
         return self._np(self._sums)
@@ -720,7 +733,7 @@ class PscD3Q19:
             torch = _torch()
             out = torch.empty(19 * self._N, dtype=torch.float64, device=self.device)
             nx, ny, nz = self._shape
-            _lib.call("psc_export_f64", self._ptr(self._a), self._storage, self._layout, nx, ny, nz,
+            _lib.call("psc_export_f64", self._ptr(self._compact_state()), self._storage, self._layout, nx, ny, nz,
                       self._ptr(out), self._stream())
             self._cache["f"] = self._np(out).reshape((19,) + self._shape)
         return self._cache["f"]
@@ -731,10 +744,20 @@ class PscD3Q19:
         arr = np.ascontiguousarray(np.asarray(value, dtype=np.float64).reshape(19 * self._N))
         src = torch.from_numpy(arr).to(self.device)
         nx, ny, nz = self._shape
-        _lib.call("psc_import_f64", self._ptr(src), self._storage, self._layout, nx, ny, nz, self._ptr(self._a),
+        dst = self._compact_state() if self._ps != self._N else self._a
+        _lib.call("psc_import_f64", self._ptr(src), self._storage, self._layout, nx, ny, nz, self._ptr(dst),
                   self._stream())
+        if dst is not self._a:
+            self._a.view(19, self._ps)[:, :self._N].copy_(dst.view(19, self._N))
         _torch().cuda.current_stream(self.device).synchronize()
         self._cache.clear()
+
+    def _compact_state(self):
+        """The state with planes of exactly N cells (self._a itself unless its
+        planes are padded to 64 cells)."""
+        if self._ps == self._N:
+            return self._a
+        return self._a.view(19, self._ps)[:, :self._N].contiguous().view(-1)
 
     def _alpha_sum_np(self):
         if "asum" not in self._cache:

@@ -110,6 +110,9 @@ def test_fused_load_entry_points_check_arguments(lib):
     assert rc == -1 and b"fused form" in h.psc_last_error()
     rc = h.psc_step(p16, p32, p64, None, None, 1, 1, 8, 8, 8, 2, 0.8, 0.0, 0.0, 0.0, 64, 1, lib.PSC_RUN_INPLACE, None)
     assert rc == -1 and b"flags" in h.psc_last_error()  # a plan flag, not a psc_step flag
+    rc = h.psc_step(p16, p32, p64, None, None, lib.PSC_F64_PLAIN, 1, 8, 8, 8, 2, 0.8, 0.0, 0.0, 0.0, 64, 1,
+                    lib.PSC_STEP_PLANES64, None)
+    assert rc == -1 and b"FP32" in h.psc_last_error()  # padded planes are an FP32 layout
 
 
 def test_solver_refuses_to_run_without_gpu():

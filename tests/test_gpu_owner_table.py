@@ -175,7 +175,7 @@ def test_owner_table_step_same_bits_tiled_and_flat(psc3d):
                           (256, 1, lib.PSC_STEP_LINEAR | 0x401), (128, 2, lib.PSC_STEP_LINEAR)]:
         lib.call("psc_step_owners", sol._a.data_ptr(), sol._b.data_ptr(), sol._csum.data_ptr(),
                  sol._own.data_ptr(), sol._us.data_ptr(), sol._motion.data_ptr(), len(t), sol._imb.data_ptr(), n, n,
-                 n, sol._s, sol.tau, 0.0, 0.0, 1e-6, bz, by, flags, h)
+                 n, sol._s, sol.tau, 0.0, 0.0, 1e-6, bz, by, flags | sol._state_flags, h)
         out = (sol._b.cpu().numpy().copy(), sol._imb.cpu().numpy()[sol._csum.cpu().numpy().repeat(3) != 0].copy())
         if ref is None:
             ref = out

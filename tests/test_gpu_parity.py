@@ -492,7 +492,7 @@ def test_batched_windows_match_single_solvers(psc3d, launch, row_align):
     loads = B.loads()
     k = B.permeability()
     for w, s in enumerate(singles):
-        assert np.array_equal(B.window_state(w), s._a.cpu().numpy().reshape(B.window_state(w).shape)), w
+        assert np.array_equal(B.window_state(w), s._compact_state().cpu().numpy().reshape(B.window_state(w).shape)), w
         assert np.array_equal(loads[w], s._loads_np()), w
         assert k[w] == pytest.approx(s.permeability(axis=2), rel=1e-12)
 

@@ -190,15 +190,28 @@ struct AdjConsts {
 };
 
 // Load / store of one value at byte offset `off` of a direction plane whose
-// base is uniform across the wavefront (SGPR base + 32-bit VGPR offset).
+// base is uniform across the wavefront, in the saddr form (SGPR base + 32-bit
+// VGPR offset).  The empty asm statements pin the base into an SGPR pair and
+// the offset into a VGPR, which keeps the compiler from folding an offset
+// shared by several planes into 64-bit VGPR addresses (64-bit VALU
+// arithmetic per access, and registers to hold them); they touch no memory.
+// The pointers are cast back to the global address space.
 template <typename R, bool NT>
 __device__ __forceinline__ R ldo(const char* __restrict__ base, uint32_t off) {
-  const R* p = (const R*)(base + off);
+  typedef __attribute__((address_space(1))) const char gc_t;
+  typedef __attribute__((address_space(1))) const R gr_t;
+  asm("" : "+s"(base));
+  asm("" : "+v"(off));
+  gr_t* p = (gr_t*)((gc_t*)base + off);
   return NT ? __builtin_nontemporal_load(p) : *p;
 }
 template <typename R, bool NT>
 __device__ __forceinline__ void sto(char* __restrict__ base, uint32_t off, R v) {
-  R* p = (R*)(base + off);
+  typedef __attribute__((address_space(1))) char gc_t;
+  typedef __attribute__((address_space(1))) R gr_t;
+  asm("" : "+s"(base));
+  asm("" : "+v"(off));
+  gr_t* p = (gr_t*)((gc_t*)base + off);
   if (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }

@@ -72,18 +72,38 @@ __device__ __forceinline__ uint32_t cell_id(const Dims d, int x, int y, int z) {
   return ((uint32_t)x * (uint32_t)d.ny + (uint32_t)y) * (uint32_t)d.pz + (uint32_t)z;
 }
 
-// Load / store of one FP32 value at byte offset `off` of a direction plane.
-// The plane base is uniform (held in SGPRs) and the offset is a 32-bit VGPR,
-// so the compiler emits the saddr form of global_load/store with no 64-bit
-// address arithmetic per access.  NT selects non-temporal (streaming) hints.
+// Load / store of one FP32 value at byte offset `off` of a direction plane,
+// in the saddr form of global_load/store: the plane base (uniform) in an SGPR
+// pair, the offset a 32-bit VGPR, no 64-bit address arithmetic per access.
+// Left to itself the compiler folds an offset that several planes share (a
+// cell's own offset: every store of a pull step, every load and store of the
+// local step of an in-place pair) into one 64-bit VGPR address and adds each
+// plane's base to it with 64-bit VALU operations, 19 to 38 per cell, and
+// holds those addresses in registers (the local in-place step: 109 VGPRs
+// instead of 70).  The empty asm statements only pin the values into an SGPR
+// pair (the base, uniform across the workgroup wherever these are called)
+// and a VGPR (the offset), which hides the shared sum from that folding; they
+// touch no memory.  The pointers are cast back to the global address space so
+// the accesses stay global_load/store.  NT selects non-temporal hints.
+typedef __attribute__((address_space(1))) char gchar_t;
+typedef __attribute__((address_space(1))) float gfloat_t;
+__device__ __forceinline__ const gchar_t* sgpr_base(const char* p) {
+  asm("" : "+s"(p));
+  return (const gchar_t*)p;
+}
+__device__ __forceinline__ gchar_t* sgpr_base(char* p) {
+  asm("" : "+s"(p));
+  return (gchar_t*)p;
+}
 template <bool NT>
 __device__ __forceinline__ float ld(const char* __restrict__ base, uint32_t off) {
-  const float* p = (const float*)(base + off);
+  const gfloat_t* p = (const gfloat_t*)(sgpr_base(base) + off);
   return NT ? __builtin_nontemporal_load(p) : *p;
 }
 template <bool NT>
 __device__ __forceinline__ void st(char* __restrict__ base, uint32_t off, float v) {
-  float* p = (float*)(base + off);
+  asm("" : "+v"(off));
+  gfloat_t* p = (gfloat_t*)(sgpr_base(base) + off);
   if (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }

@@ -102,10 +102,12 @@ def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=N
         sol.fuse_period = period
     sol.rebuild_geometry()
     sol._tune()   # launch-shape autotuning happens before the warm-up
-    # capture the graphs the timed run replays (no steps taken) before the warm-up, so that
-    # the timed run follows the warm-up with the GPU idle only for the synchronisation
-    sol.prepare_run(steps)
     sol.run(warmup)
+    # capture (and upload) the graphs the timed run replays, without stepping, after the
+    # warm-up: an odd warm-up leaves the state in the other buffer, and graphs are captured
+    # for the buffers the run starts from, so graphs prepared before it would not be the
+    # timed run's, which would then capture its own inside the timed region
+    sol.prepare_run(steps)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)

@@ -419,14 +419,17 @@ typedef struct PscStepPlan {
  * Consecutive graph launches leave the GPU idle for a few microseconds, so
  * longer graphs cost less.  Graphs are cached per plan contents, kind, step
  * count and starting parity until psc_graph_cache_clear(); use_graph = 0
- * launches the same kernels directly. */
+ * launches the same kernels directly.  A graph is uploaded to the GPU when it
+ * is created (the host waits for that upload on the library's own capture
+ * stream, never on `stream`), so its first launch costs no more than later
+ * ones. */
 int psc_run_steps(const PscStepPlan* plan, int32_t nsteps, int32_t use_graph, int32_t graph_steps,
                   int32_t* parity, void* stream);
 
 /* Capture, without launching anything, the graphs that psc_run_steps(plan,
- * nsteps, 1, graph_steps, parity, ...) replays, so that the run itself
- * starts with no capture (e.g. a timed run after a warm-up of another
- * length).  Same checks and errors as psc_run_steps; the state is not
+ * nsteps, 1, graph_steps, parity, ...) replays, and upload them, so that
+ * the run itself starts with no capture and no upload (e.g. a timed run after
+ * a warm-up of another length).  Same checks and errors as psc_run_steps; the state is not
  * touched.  No counterpart in the reference, whose loop has no launches. */
 int psc_prepare_steps(const PscStepPlan* plan, int32_t nsteps, int32_t graph_steps, const int32_t* parity);
 

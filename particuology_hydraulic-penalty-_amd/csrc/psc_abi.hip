@@ -904,6 +904,16 @@ int graph_for(const PscStepPlan& p, int steps, int parity, int kind, hipGraphExe
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return set_error((int)e, "psc_run_steps: instantiate: %s", hipGetErrorString(e));
+  // Upload the executable graph now (on the library's capture stream, waited
+  // for here), so that its first launch -- often the first graph of a timed
+  // run, right after two direct steps -- does not carry the upload: without
+  // it the GPU idled ~17 us between those steps and the graph's first kernel.
+  e = hipGraphUpload(exec, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipGraphExecDestroy(exec);
+    return set_error((int)e, "psc_run_steps: graph upload: %s", hipGetErrorString(e));
+  }
   if (g_graphs.size() >= kMaxGraphs) {
     (void)hipGraphExecDestroy(g_graphs.front().exec);
     g_graphs.erase(g_graphs.begin());

@@ -541,7 +541,11 @@ class PscD3Q19:
     def prepare_run(self, steps: int, graph_steps: int = GRAPH_STEPS) -> None:
         """Build the geometry, tune the launch and capture the hipGraphs that
         ``run(steps, graph_steps=graph_steps)`` replays, without stepping
-        (psc_prepare_steps), so that run() itself starts at once."""
+        (psc_prepare_steps), so that run() itself starts at once.  The graphs
+        hold the state buffers the next run starts from: steps taken in
+        between (an odd number of them moves the state to the other buffer)
+        can leave the run needing graphs of its own, so call this right
+        before the run it prepares."""
         import ctypes as ct
         if not self._geometry_built:
             self.rebuild_geometry()

@@ -436,6 +436,35 @@ int psc_prepare_steps(const PscStepPlan* plan, int32_t nsteps, int32_t graph_ste
 /* Destroy every cached hipGraph of psc_run_steps. */
 int psc_graph_cache_clear(void);
 
+/* Bits of *pending in psc_run_steps_deferred / psc_flush_loads. */
+#define PSC_PENDING_CHUNKS 1  /* the chunk pass of the last step's imb is owed        */
+#define PSC_PENDING_FINAL 2   /* the per-particle pass of the step before it is owed  */
+
+/* nsteps collide_stream_psc steps (psc3d.py:208-253) of a fused plan (the
+ * fused form of psc_run_steps: fuse_loads > 0, FP32 pull storage, no solid
+ * velocity, particles, imb_alt and partials_alt set; PSC_RUN_INPLACE allowed)
+ * as direct launches whose particle loads (psc3d.py:255-281) are left
+ * pending: each launch steps the state, writes the imb buffer of its input
+ * parity and reduces, in the same launch, the loads that the steps before it
+ * still owe; *pending says what is owed on entry and on return.  The steps
+ * are those of psc_run_steps (with PSC_RUN_INPLACE: pairs in place, an odd
+ * count's first step an ordinary one), so on return *parity names the buffer
+ * holding the state.  Start a sequence with *pending = 0; do not interleave
+ * other steps on the same buffers (psc_flush_loads, or a reduction of the
+ * last step's loads with psc_particle_loads, ends it).
+ * No counterpart in the reference, whose step reduces the loads before it
+ * returns: for callers stepping one call at a time that read the loads only
+ * now and then, as run_psc_permeability_resolution.py:137-143 does. */
+int psc_run_steps_deferred(const PscStepPlan* plan, int32_t nsteps, int32_t* parity, int32_t* pending,
+                           void* stream);
+
+/* Reduce the loads still pending after psc_run_steps_deferred (the same plan
+ * and the parity it returned), oldest first, so that d_loads then holds the
+ * last step's loads with the bits of psc_particle_loads after that step; sets
+ * *pending to 0.  No counterpart in the reference (see
+ * psc_run_steps_deferred). */
+int psc_flush_loads(const PscStepPlan* plan, int32_t parity, int32_t* pending, void* stream);
+
 /* Plain device copy of n floats, the bandwidth reference for the roofline
  * and for scaling HBM byte counters (no counterpart in the reference). */
 int psc_copy_f32(const float* d_src, float* d_dst, int64_t n, void* stream);

@@ -25,6 +25,8 @@ PSC_BATCH_IMB_NT = 0x80000
 PSC_STEP_LINEAR = 0x100000
 PSC_RUN_INPLACE = 0x200000
 PSC_STEP_PLANES64 = 0x400000
+PSC_PENDING_CHUNKS = 1
+PSC_PENDING_FINAL = 2
 
 # Names and argument types of every exported entry point, in header order.
 _P = ct.c_void_p
@@ -63,6 +65,8 @@ SIGNATURES = {
     "psc_run_steps": ([_P, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_prepare_steps": ([_P, _I32, _I32, _P], ct.c_int),
     "psc_graph_cache_clear": ([], ct.c_int),
+    "psc_run_steps_deferred": ([_P, _I32, _P, _P, _P], ct.c_int),
+    "psc_flush_loads": ([_P, _I32, _P, _P], ct.c_int),
     "psc_copy_f32": ([_P, _P, _I64, _P], ct.c_int),
     "psc_voxelize_binary": ([_P, _P, _I32, _P, _I32, _I32, _I32, _P, _P], ct.c_int),
     "psc_bb_step": ([_P, _P, _P, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _I32, _I32, _I32, _P], ct.c_int),

@@ -1053,6 +1053,51 @@ extern "C" int psc_prepare_steps(const PscStepPlan* plan, int32_t nsteps, int32_
   return PSC_OK;
 }
 
+namespace psc {
+namespace {
+
+int check_deferred(const char* who, const PscStepPlan* plan, int32_t parity, const int32_t* pending,
+                   PscStepPlan* key) {
+  if (plan == nullptr || pending == nullptr) return set_error(PSC_EINVAL, "%s: arguments", who);
+  if (parity != 0 && parity != 1) return set_error(PSC_EINVAL, "%s: parity %d", who, parity);
+  if (*pending & ~(PSC_PENDING_CHUNKS | PSC_PENDING_FINAL))
+    return set_error(PSC_EINVAL, "%s: pending %d", who, *pending);
+  *key = normalized(*plan);
+  if (!fused_plan(*key))
+    return set_error(PSC_EINVAL, "%s: needs a fused plan (FP32 pull, no solid velocity, fuse_loads > 0, particles, "
+                     "alternate imb and partials)", who);
+  return check_run_plan(*key);
+}
+
+}  // namespace
+}  // namespace psc
+
+extern "C" int psc_run_steps_deferred(const PscStepPlan* plan, int32_t nsteps, int32_t* parity, int32_t* pending,
+                                      void* stream) {
+  if (parity == nullptr || nsteps < 0) return set_error(PSC_EINVAL, "psc_run_steps_deferred: arguments");
+  PscStepPlan key;
+  int rc = check_deferred("psc_run_steps_deferred", plan, *parity, pending, &key);
+  if (rc) return rc;
+  int par = *parity;
+  bool pc = (*pending & PSC_PENDING_CHUNKS) != 0, pf = (*pending & PSC_PENDING_FINAL) != 0;
+  rc = run_fused(key, nsteps, &par, &pc, &pf, (hipStream_t)stream);
+  if (rc) return rc;
+  *parity = par;
+  *pending = (pc ? PSC_PENDING_CHUNKS : 0) | (pf ? PSC_PENDING_FINAL : 0);
+  return PSC_OK;
+}
+
+extern "C" int psc_flush_loads(const PscStepPlan* plan, int32_t parity, int32_t* pending, void* stream) {
+  PscStepPlan key;
+  int rc = check_deferred("psc_flush_loads", plan, parity, pending, &key);
+  if (rc) return rc;
+  rc = flush_fused(key, parity, (*pending & PSC_PENDING_CHUNKS) != 0, (*pending & PSC_PENDING_FINAL) != 0,
+                   (hipStream_t)stream);
+  if (rc) return rc;
+  *pending = 0;
+  return PSC_OK;
+}
+
 extern "C" int psc_graph_cache_clear(void) {
   std::lock_guard<std::mutex> lock(g_graph_mu);
   for (GraphEntry& e : g_graphs) (void)hipGraphExecDestroy(e.exec);

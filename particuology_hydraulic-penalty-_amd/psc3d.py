@@ -251,6 +251,13 @@ class PscD3Q19:
         # run() without fused loads: overlap the loads of step t with step t+1 on a side
         # stream (None = by lattice size and precision)
         self.pipeline_loads = None
+        # collide_stream_psc with static geometry: leave each step's loads pending and
+        # reduce them inside the next step's launch, or at the first read of the loads
+        # (None = wherever run() fuses the loads; True = wherever the kernel can)
+        self.defer_loads = None
+        self._pending = None      # (plan, parity, PSC_PENDING_* bits) of deferred steps
+        self._held = None         # plan of a deferred step not launched yet (first of an in-place pair)
+        self._step_plan = None    # (key, plan) of the deferred steps
         self.periodic_geometry = bool(periodic_geometry)
         # moving spheres re-voxelized every step, FP32 pull storage: record one owner
         # particle per covered cell and form its U + omega x r in the step kernel
@@ -409,6 +416,8 @@ class PscD3Q19:
     def rebuild_geometry(self, table=None) -> None:
         """Voxelize every sphere (psc3d.py:151-179 / 328-345) on the GPU.
         ``table``: the packed (P, 10) particle table, if the caller has it."""
+        # loads still pending are those of the geometry being replaced
+        self._flush_loads()
         torch = _torch()
         table = pack_particles(self._particles) if table is None else table
         tabs = build_tables(table, self._shape, self.periodic_geometry, self._chunk_cells)
@@ -508,7 +517,13 @@ class PscD3Q19:
 
     # ------------------------------------------------------------------- step
     def collide_stream_psc(self):
-        """One PSC collide + stream step (psc3d.py:208-253) on the GPU."""
+        """One PSC collide + stream step (psc3d.py:208-253) on the GPU.
+
+        With static geometry where run() fuses the loads, the step's loads are
+        reduced inside the next step's launch or at their first read, and
+        where run() steps in place in pairs the step may be held for the next
+        call (defer_loads); every read through the solver sees every step
+        taken, with the bits of a step followed by its load reduction."""
         self._advance()
         d = LazyDiag(self, self._step_count)
         if self._diag_mode == "eager":
@@ -530,7 +545,10 @@ class PscD3Q19:
                 self._advance()
             return
         import ctypes as ct
+        self._sync_steps()
         plan, fused, pipelined = self._run_plan()
+        # the run reduces its last step's loads, which supersede any left pending
+        self._pending = None
         self.loads_mode = "fused" if fused else ("pipelined" if pipelined else "in line")
         parity = ct.c_int32(0)
         _lib.call("psc_run_steps", ct.byref(plan), steps, 1 if graph else 0, int(graph_steps), ct.byref(parity),
@@ -553,7 +571,7 @@ class PscD3Q19:
         if pipelined or fused:
             self._ensure_pipeline_buffers()
         key = (self._plan_epoch, id(self._a), fused, pipelined, self._block, self._flags, self.inplace,
-               self._inplace_tuned, self.effective_fuse_period())
+               self._inplace_tuned, self.effective_fuse_period(), self.tau, self._force)
         if self._plan_cache is None or self._plan_cache[0] != key:
             self._plan_cache = (key, self._plan(pipelined, fused))
         return self._plan_cache[1], fused, pipelined
@@ -571,6 +589,7 @@ class PscD3Q19:
             self.rebuild_geometry()
         if not self.static_geometry or int(steps) <= 0:
             return
+        self._sync_steps()  # the run starts from the buffer a held step moves the state to
         plan, _, _ = self._run_plan()
         parity = ct.c_int32(0)
         _lib.call("psc_prepare_steps", ct.byref(plan), int(steps), int(graph_steps), ct.byref(parity))
@@ -589,13 +608,26 @@ class PscD3Q19:
         is long enough to hide them (FP64, or FP32 from ~80^3 up); on small lattices
         the fork/join costs more than it hides and the loads run in line."""
         has_loads = self._tables is not None and self._tables.particles.shape[0] > 0
-        fusable = (has_loads and self._storage == _lib.PSC_F32_DEV and self._layout == _lib.PSC_PULL
-                   and not self._moving)
+        fusable = self._fusable()
         fused = fusable and (self._N < FUSE_MAX_CELLS if self.fuse_loads is None else bool(self.fuse_loads))
         auto = self._storage == _lib.PSC_F64_PLAIN or self._N >= PIPELINE_MIN_CELLS
         pipelined = (has_loads and not fused
                      and (auto if self.pipeline_loads is None else bool(self.pipeline_loads)))
         return fused, pipelined
+
+    def _fusable(self) -> bool:
+        """The step kernel can carry the loads: particles, FP32 pull storage,
+        no solid velocity."""
+        return (self._tables is not None and self._tables.particles.shape[0] > 0
+                and self._storage == _lib.PSC_F32_DEV and self._layout == _lib.PSC_PULL and not self._moving)
+
+    def _defer(self) -> bool:
+        """Whether collide_stream_psc leaves its loads pending (defer_loads)."""
+        if not (self.static_geometry and self._geometry_built):
+            return False
+        if self.defer_loads is None:
+            return self._load_modes()[0]
+        return bool(self.defer_loads) and self._fusable()
 
     def _ensure_pipeline_buffers(self):
         torch = _torch()
@@ -607,8 +639,9 @@ class PscD3Q19:
             self._partials_alt = torch.empty(need, dtype=torch.float64, device=self.device)
             self._plan_epoch += 1
 
-    def _plan(self, pipelined: bool = False, fused: bool = False):
-        """PscStepPlan of the current buffers (state in self._a)."""
+    def _plan(self, pipelined: bool = False, fused: bool = False, inplace=None):
+        """PscStepPlan of the current buffers (state in self._a); ``inplace``
+        None = the in-place pairs as tuned or set."""
         p = _lib.PscStepPlan()
         nx, ny, nz = self._shape
         p.state_a, p.state_b = self._ptr(self._a), self._ptr(self._b)
@@ -621,7 +654,8 @@ class PscD3Q19:
         p.tau = self.tau
         p.fx, p.fy, p.fz = self._force
         p.block_z, p.block_y = self._block
-        inplace = self._inplace_tuned if self.inplace is None else bool(self.inplace)
+        if inplace is None:
+            inplace = self._inplace_tuned if self.inplace is None else bool(self.inplace)
         p.flags = self._step_flags | (_lib.PSC_RUN_INPLACE if fused and inplace else 0)
         tabs = self._tables
         if tabs is not None and tabs.particles.shape[0] > 0:
@@ -645,6 +679,7 @@ class PscD3Q19:
         the plain step's."""
         if self._autotune:
             from . import autotune
+            self._flush_loads()  # the unfused timing steps write the solver's own imb
             fused = self.static_geometry and self._geometry_built and self._load_modes()[0]
             if fused:
                 self._ensure_pipeline_buffers()
@@ -660,6 +695,10 @@ class PscD3Q19:
         self._refresh_geometry()
         if self._autotune:
             self._tune()
+        if self._defer():
+            self._step_deferred()
+            return
+        self._sync_steps()
         nx, ny, nz = self._shape
         stream = self._stream()
         fx, fy, fz = self._force
@@ -679,8 +718,85 @@ class PscD3Q19:
         self._cache.clear()
         self.accumulate_particle_loads()
 
+    def _step_deferred(self):
+        """A step whose loads stay pending (psc_run_steps_deferred): its launch
+        also reduces the loads earlier steps left pending, so a step is one
+        launch, and the last step's loads are reduced when they are read
+        (_flush_loads).  Where run() steps in place in pairs, a step is held
+        until the next call and the two launch as a pair on one state buffer;
+        anything that reads the state, imb or loads first launches a held step
+        alone (_sync_steps).  Same bits as a step followed by
+        accumulate_particle_loads."""
+        plan = self._deferred_plan()
+        if self._held is not None and self._held is not plan:
+            self._sync_steps()
+        if (plan.flags & _lib.PSC_RUN_INPLACE) and self._held is None:
+            self._held = plan
+        else:
+            pair = self._held is not None
+            self._held = None
+            self._launch_deferred(plan, 2 if pair else 1)
+        self._step_count += 1
+        self._cache.clear()
+
+    def _deferred_plan(self):
+        """The fused plan of the deferred steps, kept while nothing it holds
+        changed."""
+        self._ensure_pipeline_buffers()
+        key = (self._plan_epoch, self._block, self._flags, self.inplace, self._inplace_tuned,
+               self.effective_fuse_period(), self.tau, self._force)
+        cached = self._step_plan
+        if (cached is None or cached[0] != key
+                or {cached[1].state_a, cached[1].state_b} != {self._ptr(self._a), self._ptr(self._b)}):
+            cached = self._step_plan = (key, self._plan(fused=True))
+        return cached[1]
+
+    def _launch_deferred(self, plan, nsteps: int) -> None:
+        import ctypes as ct
+        parity = ct.c_int32(0 if self._ptr(self._a) == plan.state_a else 1)
+        pend = self._pending
+        if pend is not None and (pend[0] is not plan or pend[1] != parity.value):
+            self._reduce_pending()
+            pend = None
+        bits = ct.c_int32(0 if pend is None else pend[2])
+        _lib.call("psc_run_steps_deferred", ct.byref(plan), nsteps, ct.byref(parity), ct.byref(bits), self._stream())
+        if nsteps & 1:  # an in-place pair leaves the state in its buffer
+            self._a, self._b = self._b, self._a
+        # the last step read parity 1 - p and wrote that imb buffer
+        self._imb = self._imb_bufs[1 - parity.value]
+        self._pending = (plan, parity.value, bits.value)
+
+    def _sync_steps(self) -> None:
+        """Launch a step held back for an in-place pair on its own (an
+        ordinary step to the other buffer), so that the state, imb and
+        pending loads are those of every step taken."""
+        held = self._held
+        if held is not None:
+            self._held = None
+            self._launch_deferred(held, 1)
+
+    def _reduce_pending(self) -> None:
+        pend = self._pending
+        if pend is None:
+            return
+        if pend[2]:
+            import ctypes as ct
+            bits = ct.c_int32(pend[2])
+            _lib.call("psc_flush_loads", ct.byref(pend[0]), pend[1], ct.byref(bits), self._stream())
+        self._pending = None
+
+    def _flush_loads(self) -> None:
+        """Launch a held step and reduce the loads deferred steps left
+        pending (psc_flush_loads), so that the loads buffer holds the last
+        step's."""
+        self._sync_steps()
+        self._reduce_pending()
+
     def accumulate_particle_loads(self) -> None:
         """Node-summed force and torque per particle (psc3d.py:255-281 / 375-407)."""
+        self._sync_steps()
+        # the last step's loads computed afresh: whatever deferred steps left pending is superseded
+        self._pending = None
         tabs = self._tables
         if tabs is None:
             return
@@ -699,6 +815,7 @@ class PscD3Q19:
 
     # --------------------------------------------------------------- readouts
     def _moments(self, fields: bool, with_state: bool = True):
+        self._sync_steps()
         torch = _torch()
         nx, ny, nz = self._shape
         if fields and self._rho_u is None:
@@ -763,6 +880,7 @@ class PscD3Q19:
 
     @f.setter
     def f(self, value):
+        self._sync_steps()  # a held step came before this write
         torch = _torch()
         arr = np.ascontiguousarray(np.asarray(value, dtype=np.float64).reshape(19 * self._N))
         src = torch.from_numpy(arr).to(self.device)
@@ -778,6 +896,7 @@ class PscD3Q19:
     def _compact_state(self):
         """The state with planes of exactly N cells (self._a itself unless its
         planes are padded to 64 cells)."""
+        self._sync_steps()
         if self._ps == self._N:
             return self._a
         return self._a.view(19, self._ps)[:, :self._N].contiguous().view(-1)
@@ -797,6 +916,7 @@ class PscD3Q19:
 
     def _imb_np(self):
         if "imb" not in self._cache:
+            self._sync_steps()
             mask = self._np(self._csum).reshape(self._shape) != 0
             # interleaved on the device: imb[3 cell + axis]
             imb = np.moveaxis(self._np(self._imb).astype(np.float64).reshape(self._shape + (3,)), -1, 0)
@@ -817,6 +937,7 @@ class PscD3Q19:
 
     def _loads_np(self):
         if "loads" not in self._cache:
+            self._flush_loads()
             self._cache["loads"] = self._np(self._loads).reshape(-1, 6)
         return self._cache["loads"]
 

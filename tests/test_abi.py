@@ -115,6 +115,43 @@ def test_fused_load_entry_points_check_arguments(lib):
     assert rc == -1 and b"FP32" in h.psc_last_error()  # padded planes are an FP32 layout
 
 
+def test_deferred_load_entry_points_check_arguments(lib):
+    """psc_run_steps_deferred / psc_flush_loads take a fused plan, a parity of
+    0 or 1 and pending bits of PSC_PENDING_*, and reject anything else before
+    touching a GPU."""
+    h = lib.load()
+    plan = lib.PscStepPlan()
+    plan.state_a, plan.state_b, plan.count_sum = 16, 32, 64
+    plan.imb, plan.imb_alt, plan.partials, plan.partials_alt = 128, 192, 256, 512
+    plan.storage, plan.layout, plan.nx, plan.ny, plan.nz, plan.subsamples = 1, 1, 8, 8, 8, 2
+    plan.tau, plan.nparticles, plan.fuse_loads = 0.8, 2, 5
+    par, pend = ct.c_int32(0), ct.c_int32(0)
+    for fn, call in (("psc_run_steps_deferred", lambda: h.psc_run_steps_deferred(ct.byref(plan), 2, ct.byref(par),
+                                                                                  ct.byref(pend), None)),
+                     ("psc_flush_loads", lambda: h.psc_flush_loads(ct.byref(plan), par.value, ct.byref(pend), None))):
+        par.value, pend.value = 2, 0
+        assert call() == -1 and b"parity" in h.psc_last_error(), fn
+        par.value, pend.value = 1, 4
+        assert call() == -1 and b"pending" in h.psc_last_error(), fn
+        pend.value = lib.PSC_PENDING_CHUNKS | lib.PSC_PENDING_FINAL
+        plan.fuse_loads = 0  # loads not fused
+        assert call() == -1 and b"fused plan" in h.psc_last_error(), fn
+        plan.fuse_loads, plan.storage = 5, lib.PSC_F64_PLAIN
+        assert call() == -1 and b"fused plan" in h.psc_last_error(), fn
+        plan.storage, plan.imb_alt = 1, 128
+        assert call() == -1 and b"alias" in h.psc_last_error(), fn
+        plan.imb_alt, plan.tau = 192, 0.5
+        assert call() == -1 and b"tau" in h.psc_last_error(), fn
+        plan.tau, plan.flags = 0.8, lib.PSC_RUN_INPLACE | 0x20000  # in place with two cells per thread
+        assert call() == -1 and b"one cell per thread" in h.psc_last_error(), fn
+        plan.flags = 0
+        assert par.value == 1 and pend.value == 3, fn  # a rejected call leaves its outputs alone
+    assert h.psc_run_steps_deferred(ct.byref(plan), -1, ct.byref(par), ct.byref(pend), None) == -1
+    assert h.psc_run_steps_deferred(None, 1, ct.byref(par), ct.byref(pend), None) == -1
+    assert h.psc_run_steps_deferred(ct.byref(plan), 1, None, ct.byref(pend), None) == -1
+    assert h.psc_flush_loads(ct.byref(plan), 0, None, None) == -1
+
+
 def test_solver_refuses_to_run_without_gpu():
     torch = pytest.importorskip("torch")
     if torch.cuda.is_available():

@@ -161,6 +161,7 @@ def test_native_runner_and_graph_replays_match_python_steps(psc3d, precision, bl
     a, _ = _solver(psc3d, g, precision, block=block)
     b, _ = _solver(psc3d, g, precision, block=block)
     c, _ = _solver(psc3d, g, precision, block=block)
+    a.defer_loads = False  # the reference: loads reduced after every step
     b.pipeline_loads = c.pipeline_loads = pipeline
     b.fuse_loads = c.fuse_loads = fuse
     b.inplace = c.inplace = inplace
@@ -206,6 +207,7 @@ def test_fused_loads_match_separate_reduction(psc3d, n, block, flags, period, in
         return s
 
     a, b = make(), make()
+    a.defer_loads = False  # the reference: loads reduced after every step
     b.fuse_loads, b.fuse_period, b.inplace = True, period, inplace
     print(f"[fused] n {n} block {block} flags {flags:#x} period {period} inplace {inplace}", flush=True)
     for chunk in (1, 3, 24, 9):
@@ -241,6 +243,7 @@ def test_inplace_pairs_on_non_cubic_lattices(psc3d, shape, block, flags):
         return s
 
     a, b = make(), make()
+    a.defer_loads = False  # the reference: loads reduced after every step
     b.fuse_loads, b.inplace = True, True
     print(f"[inplace] shape {shape} block {block} flags {flags:#x}", flush=True)
     for chunk in (2, 3, 18, 7, 1):
@@ -282,6 +285,7 @@ def test_fused_tuning_leaves_the_solver_unchanged(psc3d, tmp_path, monkeypatch):
     at._MEM.clear()
     (tmp_path / "tune.json").unlink()
     b._autotune = True
+    b.hydro_forces  # the loads of b's last step, which it left pending
     before = (b._a.clone(), b._b.clone(), b._loads.clone())
     b._tune()  # both caches emptied: times every candidate again
     torch.cuda.synchronize()

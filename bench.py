@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--fuse-period", type=int, default=None,
                     help="placement period of the fused load workgroups (default: tuned; with --launch: 5)")
     ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--per-call-steps", type=int, default=240,
+                    help="fixed_array: also time this many collide_stream_psc() calls one at a time, reading the "
+                         "diag dict every 60 (0: skip)")
     ap.add_argument("--no-large-check", action="store_true",
                     help="skip the 256^3 HBM-only rate of the plain step kernel")
     ap.add_argument("--workload", default="fixed_array",
@@ -124,6 +127,31 @@ def timed_run(psc3d, table, n, precision, layout, steps, warmup, torch, launch=N
     k = sol.permeability(axis=0)
     readout_ms = (time.perf_counter() - r0) * 1e3
     return sol, k, wall, gpu_ms, readout_ms
+
+
+def per_call_rate(sol, calls, read_every, torch):
+    """collide_stream_psc() one call at a time, as the reference's drivers step
+    (run_psc_permeability_resolution.py:137-143 reads the diag dict every 60
+    calls): HIP-event time per step on the solver's stream over `calls` calls
+    (a multiple of read_every, so the last call's read leaves nothing held or
+    owed), and the wall time per call."""
+    stream = torch.cuda.current_stream()
+    for _ in range(4):
+        sol.collide_stream_psc()
+    sol.hydro_forces
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for k in range(calls):
+        d = sol.collide_stream_psc()
+        if (k + 1) % read_every == 0:
+            d["mean_uz"]
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) * 1e3 / calls, wall * 1e6 / calls
 
 
 def step_kernel_ms(sol, repeats, torch, lib, moving=False, launch_cfg=None):
@@ -948,6 +976,21 @@ def main():
     porosity = 1.0 - float(sol32.solid.mean())
     fused = sol32.loads_mode == "fused"
     inplace = fused and (sol32._inplace_tuned if sol32.inplace is None else bool(sol32.inplace))
+    per_call = None
+    if args.per_call_steps > 0:
+        calls = -(-args.per_call_steps // 60) * 60
+        us, wall_us = per_call_rate(sol32, calls, 60, torch)
+        sol32.defer_loads = False
+        us_plain, wall_plain = per_call_rate(sol32, calls, 60, torch)
+        sol32.defer_loads = None
+        per_call = {"calls": calls, "read_every": 60, "gpu_us_per_step": us, "wall_us_per_step": wall_us,
+                    "undeferred_gpu_us_per_step": us_plain, "undeferred_wall_us_per_step": wall_plain,
+                    "note": "collide_stream_psc() in a Python loop reading the diag dict every 60 calls, after "
+                            "the timed run; default: loads reduced in the next step's launch"
+                            + (", steps launched as in-place pairs" if inplace else "")
+                            + "; undeferred: defer_loads=False, a step launch and two load launches per call"}
+        log(f"fp32 one call at a time: {us:.1f} us/step ({wall_us:.1f} us wall), undeferred {us_plain:.1f} us/step",
+            args.quiet)
     # the plain step kernel's own best launch (the timed loop's launch is tuned for the fused,
     # possibly in-place, form); with --launch both use the given one
     kern_launch = None
@@ -1106,6 +1149,7 @@ def main():
         "gpu_ms_timed_region": gpu_ms32,
         "gpu_us_per_step": gpu_ms32 * 1e3 / args.steps,
         "k_readout_ms": readout_ms,
+        "per_call": per_call,
         "nonfinite_cells": int(nonfinite),
     }
     print(json.dumps(out))

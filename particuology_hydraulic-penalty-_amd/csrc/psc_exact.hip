@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "psc_common.h"
+#include "psc_div.h"
 #include "psc_loads.h"
 #include "../../include/psc_hip.h"
 
@@ -44,10 +45,17 @@ __device__ __forceinline__ double moment(const double (&v)[Q]) {
   return acc;
 }
 
+// Reciprocals of the constant divisors, correctly rounded (compile-time IEEE
+// division), for the divisions of psc_div.h: every quotient below is IEEE
+// division's, bit for bit.
+constexpr double kInvCs2 = 1.0 / kCs2;
+constexpr double kInvCs2Sq = 1.0 / kCs2Sq;
+
 template <int I>
 __device__ __forceinline__ double feq_term(double rho, double ux, double uy, double uz, double uu) {
   const double cu = cdot<I>(ux, uy, uz);
-  return (w_of(I) * rho) * (((1.0 + cu / kCs2) + ((0.5 * cu) * cu) / kCs2Sq) - (0.5 * uu) / kCs2);
+  return (w_of(I) * rho) * (((1.0 + div_pos(cu, kCs2, kInvCs2)) + div_pos((0.5 * cu) * cu, kCs2Sq, kInvCs2Sq)) -
+                            div_pos(0.5 * uu, kCs2, kInvCs2));
 }
 
 template <int... Is>
@@ -73,7 +81,7 @@ __device__ __forceinline__ double guo_term(double pref, double uf, double ux, do
                                            double fx, double fy, double fz) {
   const double cu = cdot<I>(ux, uy, uz);
   const double cf = cdot<I>(fx, fy, fz);
-  return (pref * w_of(I)) * (((cf - uf) / kCs2) + ((cu * cf) / kCs2Sq));
+  return (pref * w_of(I)) * (div_pos(cf - uf, kCs2, kInvCs2) + div_pos(cu * cf, kCs2Sq, kInvCs2Sq));
 }
 
 template <int... Is>
@@ -134,8 +142,8 @@ template <bool PULL, bool MOVING, bool FORCE, bool IMB>
 __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin, double* __restrict__ fout,
                                                   const uint32_t* __restrict__ csum,
                                                   const double* __restrict__ usn, double* __restrict__ imb,
-                                                  Dims d, double s3, double tau, double pref, double fx,
-                                                  double fy, double fz) {
+                                                  Dims d, double s3, double ys3, double tau, double ytau,
+                                                  double pref, double fx, double fy, double fz) {
   int ix, iy, iz;
   if (!thread_cell(d, ix, iy, iz)) return;
   const size_t N = (size_t)d.ps;
@@ -147,13 +155,16 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
   double rho = f[0];
 #pragma unroll
   for (int i = 1; i < Q; ++i) rho = rho + f[i];
-  const double ux = (moment<0>(f) + 0.5 * fx) / rho;
-  const double uy = (moment<1>(f) + 0.5 * fy) / rho;
-  const double uz = (moment<2>(f) + 0.5 * fz) / rho;
+  // every division is IEEE's, bit for bit (psc_div.h): by rho through one
+  // division for its reciprocal, by tau, s^3 and the constants through theirs
+  const double yrho = 1.0 / rho;
+  const double ux = div_any(moment<0>(f) + 0.5 * fx, rho, yrho);
+  const double uy = div_any(moment<1>(f) + 0.5 * fy, rho, yrho);
+  const double uz = div_any(moment<2>(f) + 0.5 * fz, rho, yrho);
   // second half-force shift (psc3d.py:217-219)
-  const double uxe = ux + 0.5 * fx / rho;
-  const double uye = uy + 0.5 * fy / rho;
-  const double uze = uz + 0.5 * fz / rho;
+  const double uxe = ux + div_any(0.5 * fx, rho, yrho);
+  const double uye = uy + div_any(0.5 * fy, rho, yrho);
+  const double uze = uz + div_any(0.5 * fz, rho, yrho);
 
   double feqf[Q];
   equilibrium_all(Dirs{}, rho, uxe, uye, uze, feqf);
@@ -162,7 +173,7 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
   // given FP64 solid-fraction field in place of the sub-sample counts
   const bool field = s3 == 0.0;
   const uint32_t cnt = field ? 0u : csum[cell];
-  const double asum = field ? ((const double*)csum)[cell] : (double)cnt / s3;
+  const double asum = field ? ((const double*)csum)[cell] : div_pos((double)cnt, s3, ys3);
   const double solid = asum < 1.0 ? asum : 1.0;
   const double b = solid / ((solid + tau * (1.0 - solid)) + 1.0e-14);
 
@@ -174,7 +185,7 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
     // B = 0: collision = 1*Omega_bgk + 0*Omega_s
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const double obgk = -((f[i] - feqf[i]) / tau);
+      const double obgk = -div_pos(f[i] - feqf[i], tau, ytau);
       fnew[i] = FORCE ? f[i] + (obgk + gterm[i]) : f[i] + obgk;
     }
   } else {
@@ -183,9 +194,10 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
       // solid velocity: numerator / alpha_sum (psc3d.py:202-205)
       double usx = usn[cell], usy = usn[N + cell], usz = usn[2 * N + cell];
       if (asum > 1.0e-14) {
-        usx = usx / asum;
-        usy = usy / asum;
-        usz = usz / asum;
+        const double yas = 1.0 / asum;
+        usx = div_pos(usx, asum, yas);
+        usy = div_pos(usy, asum, yas);
+        usz = div_pos(usz, asum, yas);
       }
       equilibrium_all(Dirs{}, rho, usx, usy, usz, feqs);
     } else {
@@ -197,7 +209,7 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const int o = opp_of(i);
-      const double obgk = -((f[i] - feqf[i]) / tau);
+      const double obgk = -div_pos(f[i] - feqf[i], tau, ytau);
       const double os = ((f[o] - f[i]) + feqs[i]) - feqf[o];
       bos[i] = b * os;
       const double coll = omb * obgk + bos[i];
@@ -225,7 +237,7 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
 template <bool FORCE>
 __global__ __launch_bounds__(256) void k_bb_step_f64(const double* __restrict__ fin, double* __restrict__ fout,
                                                      const uint8_t* __restrict__ solid, Dims d, double tau,
-                                                     double pref, double fx, double fy, double fz) {
+                                                     double ytau, double pref, double fx, double fy, double fz) {
   int ix, iy, iz;
   if (!thread_cell(d, ix, iy, iz)) return;
   const size_t N = (size_t)d.ps;
@@ -240,9 +252,10 @@ __global__ __launch_bounds__(256) void k_bb_step_f64(const double* __restrict__ 
   double rho = f[0];
 #pragma unroll
   for (int i = 1; i < Q; ++i) rho = rho + f[i];
-  double ux = (fx != 0.0 ? moment<0>(f) + 0.5 * fx : moment<0>(f)) / rho;
-  double uy = (fy != 0.0 ? moment<1>(f) + 0.5 * fy : moment<1>(f)) / rho;
-  double uz = (fz != 0.0 ? moment<2>(f) + 0.5 * fz : moment<2>(f)) / rho;
+  const double yrho = 1.0 / rho;  // IEEE divisions through the reciprocal (psc_div.h)
+  double ux = div_any(fx != 0.0 ? moment<0>(f) + 0.5 * fx : moment<0>(f), rho, yrho);
+  double uy = div_any(fy != 0.0 ? moment<1>(f) + 0.5 * fy : moment<1>(f), rho, yrho);
+  double uz = div_any(fz != 0.0 ? moment<2>(f) + 0.5 * fz : moment<2>(f), rho, yrho);
   if (s) ux = uy = uz = 0.0;
   double feq[Q];
   equilibrium_all(Dirs{}, rho, ux, uy, uz, feq);
@@ -251,7 +264,7 @@ __global__ __launch_bounds__(256) void k_bb_step_f64(const double* __restrict__ 
   double out[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const double v = f[i] - (f[i] - feq[i]) / tau;
+    const double v = f[i] - div_pos(f[i] - feq[i], tau, ytau);
     out[i] = FORCE ? v + g[i] : v;
   }
   store_all<true>(Dirs{}, fout, N, d, ix, iy, iz, out);
@@ -879,10 +892,11 @@ int launch_bb_step_f64(const double* in, double* out, const uint8_t* solid, int 
   const dim3 block(bz, by, 1);
   const dim3 grid((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
   const double pref = 1.0 - 0.5 / tau;
+  const double ytau = 1.0 / tau;  // RN(1/tau): host IEEE division
   if (fx != 0.0 || fy != 0.0 || fz != 0.0)
-    k_bb_step_f64<true><<<grid, block, 0, stream>>>(in, out, solid, d, tau, pref, fx, fy, fz);
+    k_bb_step_f64<true><<<grid, block, 0, stream>>>(in, out, solid, d, tau, ytau, pref, fx, fy, fz);
   else
-    k_bb_step_f64<false><<<grid, block, 0, stream>>>(in, out, solid, d, tau, pref, fx, fy, fz);
+    k_bb_step_f64<false><<<grid, block, 0, stream>>>(in, out, solid, d, tau, ytau, pref, fx, fy, fz);
   return check_launch("psc_bb_step(f64)");
 }
 
@@ -900,6 +914,7 @@ int launch_step_f64(const double* in, double* out, const uint32_t* csum, const d
   const dim3 block(bz, by, 1);
   const dim3 grid((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
   const double s3 = (double)(s * s * s);  // 0: csum is an FP64 solid-fraction field
+  const double ys3 = 1.0 / s3, ytau = 1.0 / tau;  // RN(1/s^3), RN(1/tau): host IEEE divisions
   const double pref = 1.0 - 0.5 / tau;
   const bool force = (fx != 0.0 || fy != 0.0 || fz != 0.0);
   const bool moving = usn != nullptr;
@@ -907,7 +922,8 @@ int launch_step_f64(const double* in, double* out, const uint32_t* csum, const d
   const bool wimb = imb != nullptr;
 #define PSC_F64_CASE(P, M, F, I)                                                                      \
   if (pull == P && moving == M && force == F && wimb == I) {                                        \
-    k_step_f64<P, M, F, I><<<grid, block, 0, stream>>>(in, out, csum, usn, imb, d, s3, tau, pref, fx, fy, fz); \
+    k_step_f64<P, M, F, I><<<grid, block, 0, stream>>>(in, out, csum, usn, imb, d, s3, ys3, tau, ytau, pref, fx, fy, \
+                                                       fz);                                                        \
     return check_launch("psc_step(f64)");                                                           \
   }
   PSC_F64_CASE(false, false, false, false)

@@ -102,6 +102,41 @@ def test_deferred_loads_match_reduction_after_every_step(psc3d, n, block, flags,
     assert int(a._tables.chunks.shape[0]) > 20
 
 
+@pytest.mark.parametrize("shape,block,flags", [((20, 28, 36), (64, 1), 0x00000), ((36, 20, 52), (128, 1), 0x100601),
+                                               ((24, 16, 3), (64, 4), 0x00301)])
+def test_held_pairs_on_non_cubic_lattices(psc3d, shape, block, flags):
+    """Held in-place pairs index x, y and z separately (a 3-cell axis where
+    z - 1 and z + 2 coincide): odd and even runs of calls, with native runs
+    and reads between them, give the bits of a step followed by its loads."""
+    rng = np.random.default_rng(9)
+    nx, ny, nz = shape
+    spheres = [(rng.uniform(0, nx), rng.uniform(0, ny), rng.uniform(0, nz), rng.uniform(2.0, 5.0)) for _ in range(8)]
+
+    def make(defer):
+        s = psc3d.SparsePscD3Q19(shape, [psc3d.Particle3D(*p) for p in spheres], tau=0.8, subsamples=3,
+                                 body_force=(1e-5, -2e-5, 3e-5), static_geometry=True, precision="fp32",
+                                 device="cuda:0", block=block, chunk_cells=256)
+        s._flags = flags
+        s.defer_loads, s.inplace = defer, True
+        return s
+
+    a, b = make(False), make(True)
+    print(f"[held] shape {shape} block {block} flags {flags:#x}", flush=True)
+    for calls, native in ((5, 2), (4, 3), (1, 0), (6, 1), (3, 4)):
+        for _ in range(calls):
+            a.collide_stream_psc()
+            b.collide_stream_psc()
+        assert (b._held is not None) == bool(calls & 1) or calls == 0
+        if native:
+            a.run(native)
+            b.run(native)
+        assert np.array_equal(a.f, b.f)
+        assert np.array_equal(a.hydro_forces, b.hydro_forces)
+        assert np.array_equal(a.hydro_torques, b.hydro_torques)
+        assert np.array_equal(np.stack([a.imb_mx, a.imb_my, a.imb_mz]), np.stack([b.imb_mx, b.imb_my, b.imb_mz]))
+    assert np.abs(a.hydro_forces).max() > 0
+
+
 def test_steps_that_cannot_carry_the_loads_reduce_them_at_once(psc3d):
     """FP64 storage, dynamic geometry, or no particles: nothing is deferred."""
     spheres = [psc3d.Particle3D(8.3, 7.6, 8.1, 3.2)]

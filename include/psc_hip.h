@@ -54,7 +54,8 @@ extern "C" {
 #define PSC_PULL 1
 
 /* psc_step flags */
-#define PSC_STEP_NONTEMPORAL 1      /* bit 0: non-temporal loads/stores (FP32)      */
+#define PSC_STEP_NONTEMPORAL 1      /* bit 0: non-temporal state loads/stores; the  */
+                                    /* only flag the FP64 step uses                 */
 #define PSC_STEP_BPC_MASK 0xff00    /* bits 8-15: cap on resident workgroups per CU */
 #define PSC_STEP_CPT_MASK 0x30000   /* bits 16-17: cells per thread along y (1, 2)  */
 #define PSC_BATCH_WG512 0x40000     /* psc_step_batched: 512-thread workgroups      */
@@ -203,10 +204,11 @@ int psc_step_owners(const float* d_in, float* d_out, const uint32_t* d_count_sum
  * untouched or set to 0 (FP32 storage writes every lane of a wavefront that
  * holds a solid cell), so readers mask imb by count_sum > 0.  d_in and d_out
  * must not alias.  block_z/block_y select the
- * workgroup shape (0 = default); flags (FP32 storage): PSC_STEP_NONTEMPORAL
- * streams the distributions with non-temporal loads/stores, bits 8-15 cap
- * the resident workgroups per CU (through an LDS reservation, 0 = no cap),
- * bits 16-17 set the cells per thread (1 or 2);
+ * workgroup shape (0 = default); flags: PSC_STEP_NONTEMPORAL streams the
+ * distributions with non-temporal loads/stores (both storages); with FP32
+ * storage bits 8-15 also cap the resident workgroups per CU (through an LDS
+ * reservation, 0 = no cap) and bits 16-17 set the cells per thread (1 or 2);
+ * the FP64 step ignores those;
  * storage/layout as above.  Precision of d_in/d_out/d_us_num/d_imb follows
  * `storage`.  subsamples = PSC_ALPHA_FIELD reads the solid fraction from a
  * field instead (no solid velocity: d_us_num must be NULL); a cell is fluid

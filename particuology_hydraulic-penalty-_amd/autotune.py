@@ -238,8 +238,9 @@ def tune(solver, budget_ms: float = 400.0, use_cache: bool = True, fused: bool =
                 return cfg
         nz = solver._shape[2]
         if solver._storage == _lib.PSC_F64_PLAIN:
-            # the FP64 step takes no launch flags: workgroup shapes only
-            cands = [(bz, by, 0, 0, False) for bz, by in SHAPES + FP64_SHAPES if not (bz > 64 and nz < bz // 2)]
+            # the FP64 step's only launch flag is non-temporal state access: shapes x that
+            cands = [(bz, by, nt, 0, False) for bz, by in SHAPES + FP64_SHAPES for nt in NT
+                     if not (bz > 64 and nz < bz // 2)]
         else:
             cands = candidates(nz, bool(getattr(solver, "_moving", False)))
         # one probe launch sizes the repeat count to the time budget

@@ -38,7 +38,7 @@ int check_launch(const char* what) {
 
 // launchers defined in the kernel translation units
 int launch_step_f64(const double*, double*, const uint32_t*, const double*, double*, int, int, int, int, int, double,
-                    double, double, double, int, int, hipStream_t);
+                    double, double, double, int, int, int, hipStream_t);
 int launch_step_f32(const float*, float*, const uint32_t*, const float*, float*, int, int, int, int, int, double,
                     double, double, double, int, int, int, hipStream_t, const uint32_t* = nullptr,
                     const void* = nullptr, int = 0);
@@ -335,7 +335,7 @@ int psc_step(const void* d_in, void* d_out, const uint32_t* d_count_sum, const v
                            layout, nx, ny, nz, subsamples, tau, fx, fy, fz, bz, by, flags, s);
   if (storage == PSC_F64_PLAIN)
     return launch_step_f64((const double*)d_in, (double*)d_out, d_count_sum, (const double*)d_us_num,
-                           (double*)d_imb, layout, nx, ny, nz, subsamples, tau, fx, fy, fz, bz, by, s);
+                           (double*)d_imb, layout, nx, ny, nz, subsamples, tau, fx, fy, fz, bz, by, flags, s);
   return set_error(PSC_EINVAL, "psc_step: storage %d", storage);
 }
 

@@ -123,22 +123,33 @@ __device__ __forceinline__ uint32_t shifted(const Dims d, int ix, int iy, int iz
                  wrap_step(iz, s * cz_of(I), d.nz));
 }
 
-template <bool PULL, int... Is>
-__device__ __forceinline__ void load_all(Seq<Is...>, const double* __restrict__ in, size_t N, const Dims d,
-                                         int ix, int iy, int iz, double (&f)[Q]) {
-  ((f[Is] = in[(size_t)Is * N + (PULL ? shifted<Is>(d, ix, iy, iz, -1) : cell_id(d, ix, iy, iz))]), ...);
+// NT: non-temporal loads and stores (the state streams through once per step).
+template <bool NT>
+__device__ __forceinline__ double ld64(const double* p) {
+  return NT ? __builtin_nontemporal_load(p) : *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st64(double* p, double v) {
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
 }
 
-template <bool PULL, int... Is>
+template <bool PULL, bool NT = false, int... Is>
+__device__ __forceinline__ void load_all(Seq<Is...>, const double* __restrict__ in, size_t N, const Dims d,
+                                         int ix, int iy, int iz, double (&f)[Q]) {
+  ((f[Is] = ld64<NT>(in + (size_t)Is * N + (PULL ? shifted<Is>(d, ix, iy, iz, -1) : cell_id(d, ix, iy, iz)))), ...);
+}
+
+template <bool PULL, bool NT = false, int... Is>
 __device__ __forceinline__ void store_all(Seq<Is...>, double* __restrict__ out, size_t N, const Dims d, int ix,
                                           int iy, int iz, const double (&f)[Q]) {
-  ((out[(size_t)Is * N + (PULL ? cell_id(d, ix, iy, iz) : shifted<Is>(d, ix, iy, iz, 1))] = f[Is]), ...);
+  (st64<NT>(out + (size_t)Is * N + (PULL ? cell_id(d, ix, iy, iz) : shifted<Is>(d, ix, iy, iz, 1)), f[Is]), ...);
 }
 
 // ---------------------------------------------------------------------------
 // K2 (FP64 plain): psc3d.py:211-242 in the reference's operation order.
 // ---------------------------------------------------------------------------
-template <bool PULL, bool MOVING, bool FORCE, bool IMB>
+template <bool PULL, bool MOVING, bool FORCE, bool IMB, bool NT>
 __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin, double* __restrict__ fout,
                                                   const uint32_t* __restrict__ csum,
                                                   const double* __restrict__ usn, double* __restrict__ imb,
@@ -149,7 +160,7 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
   const size_t N = (size_t)d.ps;
   const uint32_t cell = cell_id(d, ix, iy, iz);
   double f[Q];
-  load_all<PULL>(Dirs{}, fin, N, d, ix, iy, iz, f);
+  load_all<PULL, NT>(Dirs{}, fin, N, d, ix, iy, iz, f);
 
   // rho, u with the half-force shift (psc3d.py:211-214)
   double rho = f[0];
@@ -221,7 +232,7 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
       imb[3 * (size_t)cell + 2] = moment<2>(bos);
     }
   }
-  store_all<PULL>(Dirs{}, fout, N, d, ix, iy, iz, fnew);
+  store_all<PULL, NT>(Dirs{}, fout, N, d, ix, iy, iz, fnew);
 }
 
 // ---------------------------------------------------------------------------
@@ -909,7 +920,7 @@ int launch_voxelize_binary(const double* spheres, const int32_t* boxes, int nsph
 
 int launch_step_f64(const double* in, double* out, const uint32_t* csum, const double* usn, double* imb,
                     int layout, int nx, int ny, int nz, int s, double tau, double fx, double fy, double fz,
-                    int bz, int by, hipStream_t stream) {
+                    int bz, int by, int flags, hipStream_t stream) {
   const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
   const dim3 block(bz, by, 1);
   const dim3 grid((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
@@ -920,11 +931,16 @@ int launch_step_f64(const double* in, double* out, const uint32_t* csum, const d
   const bool moving = usn != nullptr;
   const bool pull = layout == PSC_PULL;
   const bool wimb = imb != nullptr;
-#define PSC_F64_CASE(P, M, F, I)                                                                      \
-  if (pull == P && moving == M && force == F && wimb == I) {                                        \
-    k_step_f64<P, M, F, I><<<grid, block, 0, stream>>>(in, out, csum, usn, imb, d, s3, ys3, tau, ytau, pref, fx, fy, \
-                                                       fz);                                                        \
-    return check_launch("psc_step(f64)");                                                           \
+  const bool nt = (flags & PSC_STEP_NONTEMPORAL) != 0;  // the only launch flag of the FP64 step
+#define PSC_F64_CASE(P, M, F, I)                                                                             \
+  if (pull == P && moving == M && force == F && wimb == I) {                                               \
+    if (nt)                                                                                                \
+      k_step_f64<P, M, F, I, true><<<grid, block, 0, stream>>>(in, out, csum, usn, imb, d, s3, ys3, tau, ytau, \
+                                                               pref, fx, fy, fz);                           \
+    else                                                                                                   \
+      k_step_f64<P, M, F, I, false><<<grid, block, 0, stream>>>(in, out, csum, usn, imb, d, s3, ys3, tau,     \
+                                                                ytau, pref, fx, fy, fz);                    \
+    return check_launch("psc_step(f64)");                                                                  \
   }
   PSC_F64_CASE(false, false, false, false)
   PSC_F64_CASE(false, false, false, true)

@@ -322,9 +322,9 @@ def test_every_launch_configuration_gives_identical_results(psc3d):
 
 
 def test_every_fp64_launch_gives_identical_results(psc3d):
-    """The tuner also picks the FP64 step's workgroup shape: every shape it
-    may pick computes the same bits (the reference's operation order per
-    cell, no cross-cell arithmetic)."""
+    """The tuner also picks the FP64 step's workgroup shape and whether it
+    streams non-temporally: every launch it may pick computes the same bits
+    (the reference's operation order per cell, no cross-cell arithmetic)."""
     at = load_package("autotune")
     lib = load_package("_lib")
     import torch
@@ -335,12 +335,13 @@ def test_every_fp64_launch_gives_identical_results(psc3d):
     h = torch.cuda.current_stream().cuda_stream
     ref = None
     for bz, by in at.SHAPES + at.FP64_SHAPES:
-        lib.call("psc_step", s._a.data_ptr(), s._b.data_ptr(), s._csum.data_ptr(), None, s._imb.data_ptr(),
-                 lib.PSC_F64_PLAIN, 1, nx, ny, nz, s._s, s.tau, 0.0, 0.0, float(g["force"][2]), bz, by, 0, h)
-        out = s._b.cpu().numpy().copy()
-        if ref is None:
-            ref = out
-        assert np.array_equal(out, ref), (bz, by)
+        for flags in (0, lib.PSC_STEP_NONTEMPORAL):
+            lib.call("psc_step", s._a.data_ptr(), s._b.data_ptr(), s._csum.data_ptr(), None, s._imb.data_ptr(),
+                     lib.PSC_F64_PLAIN, 1, nx, ny, nz, s._s, s.tau, 0.0, 0.0, float(g["force"][2]), bz, by, flags, h)
+            out = s._b.cpu().numpy().copy()
+            if ref is None:
+                ref = out
+            assert np.array_equal(out, ref), (bz, by, flags)
 
 
 def _moving_table(rng, n, count, rmin, rmax, umax=2e-3):

@@ -1020,8 +1020,14 @@ def main():
         kern64 = step_kernel_ms(sol64, max(50, args.kernel_repeats // 4), torch, lib)
         mlups64 = N * args.steps / wall64 / 1e6
         fp64 = {"MLUPS": mlups64, "ms_per_step": wall64 * 1e3 / args.steps, "k": k64,
-                "step_kernel_us": kern64 * 1e3, "launch": [sol64._block[0], sol64._block[1]],
+                "step_kernel_us": kern64 * 1e3, "launch": [sol64._block[0], sol64._block[1], sol64._flags],
                 "step_kernel_GBps": BYTES_F64_STATIC * N / (kern64 * 1e-3) / 1e9}
+        if args.per_call_steps > 0:
+            calls = -(-args.per_call_steps // 60) * 60
+            us64, wall_us64 = per_call_rate(sol64, calls, 60, torch)
+            fp64["per_call"] = {"calls": calls, "read_every": 60, "gpu_us_per_step": us64,
+                                "wall_us_per_step": wall_us64}
+            log(f"fp64 one call at a time: {us64:.1f} us/step ({wall_us64:.1f} us wall)", args.quiet)
         log(f"fp64: {mlups64:.0f} MLUPS, step kernel {kern64*1e3:.1f} us, k={k64:.9e}", args.quiet)
         del sol64
     copy_gbps = None

@@ -278,7 +278,8 @@ class PscD3Q19:
         if self._chunk_cells < 64 or self._chunk_cells & (self._chunk_cells - 1):
             raise ValueError("chunk_cells must be a power of two >= 64")
         # launch configuration of psc_step: "auto" = measured by the autotuner
-        # at the first step (FP32 storage; FP64 uses the default shape)
+        # at the first step (FP32: shape, non-temporal access, occupancy cap, cells per thread,
+        # flat mapping; FP64: shape and non-temporal access)
         self._autotune = isinstance(block, str) and block == "auto"
         self._block = (0, 0) if self._autotune else (int(block[0]), int(block[1]))
         self._flags = _lib.PSC_STEP_NONTEMPORAL if nontemporal else 0

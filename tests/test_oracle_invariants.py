@@ -365,3 +365,20 @@ def test_permeability_readout_closed_form():
                                                                                                    rel=1e-14)
     assert O.permeability(np.full((n,) * 3, 0.5), np.full((n,) * 3, U), n, tau, F) == pytest.approx(
         0.5 * nu * U / F, rel=1e-14)
+
+
+def test_numpy_sums_one_cell_pairwise():
+    """The premise of the one-cell tolerance of the GPU edge cases: NumPy's
+    sum(axis=0) over a (19, 1, 1, 1) state (contiguous along the axis) does
+    not add row by row, while over a (19, 2, 2, 2) state it does."""
+    rng = np.random.default_rng(0)
+    differ = 0
+    for _ in range(200):
+        v = rng.uniform(0.0, 0.4, 19)
+        seq = v[0]
+        for x in v[1:]:
+            seq = seq + x
+        differ += v.reshape(19, 1, 1, 1).sum(axis=0)[0, 0, 0] != seq
+        wide = np.broadcast_to(v[:, None, None, None], (19, 2, 2, 2)).copy()
+        assert np.all(wide.sum(axis=0) == seq)
+    assert differ > 20

@@ -76,10 +76,11 @@ def test_fp64_matches_reference_trajectory(psc3d, name, layout):
 
 
 @pytest.mark.parametrize("name", ["smoke12", "moving20", "spin16", "sparse16"])
-def test_fp32_deviation_close_to_reference(psc3d, name):
+@pytest.mark.parametrize("layout", ["pull", "push"])
+def test_fp32_deviation_close_to_reference(psc3d, name, layout):
     g = G.load(name)
-    s, parts = _solver(psc3d, g, "fp32")
-    print(f"[{name}] running fp32", flush=True)
+    s, parts = _solver(psc3d, g, "fp32", layout)
+    print(f"[{name}/{layout}] running fp32", flush=True)
     F, T, D = _run(s, parts, name, int(g["steps"]))
     n = int(g["n"])
     if "f" in g:

@@ -553,3 +553,37 @@ def test_fluid_statistics_readout_matches_oracle(psc3d):
     assert s.permeability(axis=2) == pytest.approx(O.permeability(o.solid, o.uz, n, o.tau, o.body_force[2]),
                                                    rel=1e-12)
     assert s.mean_uz() == pytest.approx(float(np.mean(o.uz)), rel=1e-12)
+
+
+@pytest.mark.parametrize("launch", ["auto", (1, 0, "spans"), (1 | 0x40000, 0, "rows")])
+def test_batched_degenerate_windows_match_single_solvers(psc3d, launch):
+    """Windows of 1, 2 and 3 cells per edge (a cell whose upstream and
+    downstream neighbours coincide or are itself; fluid only, since a periodic
+    sphere box with its one-cell margin must fit in the window) batched with
+    5- and 6-cell windows whose spheres' boxes fill the window or wrap two
+    faces equal standalone FP32 solvers of those windows bit for bit, state
+    and loads, over graph-replayed fused steps."""
+    batch = load_package("batch")
+    print(f"[batched degenerate windows] launch {launch}", flush=True)
+
+    def table(rows):
+        t = np.zeros((len(rows), 10))
+        for i, r in enumerate(rows):
+            t[i, :len(r)] = r
+        return t
+
+    wins = [(1, table([])),
+            (2, table([])),
+            (3, table([])),
+            (5, table([(2.5, 2.5, 2.5, 0.9)])),
+            (6, table([(0.3, 4.6, 2.5, 1.2), (3.5, 1.5, 4.8, 0.8)]))]
+    kw = dict(tau=0.8, subsamples=3, body_force=(1e-6, -2e-6, 3e-6), device="cuda:0")
+    B = batch.BatchedPscD3Q19(wins, launch=launch, **kw)
+    B.run(13)
+    loads = B.loads()
+    for w, (L, t) in enumerate(wins):
+        s = psc3d.SparsePscD3Q19(L, t, static_geometry=True, periodic_geometry=True, precision="fp32", **kw)
+        s.run(13, graph=False)
+        assert np.array_equal(B.window_state(w), s._compact_state().cpu().numpy().reshape(B.window_state(w).shape)), w
+        assert np.array_equal(loads[w], s._loads_np()), w
+    assert loads[0].size == 0 and np.abs(loads[3]).max() > 0 and np.abs(loads[4]).max() > 0

@@ -78,7 +78,7 @@ int launch_adjoint_step(const void*, const void*, void*, const void*, double*, i
                         double, double, int, int, int, hipStream_t);
 
 int launch_bb_step_f64(const double*, double*, const uint8_t*, int, int, int, double, double, double, double, int, int,
-                       hipStream_t);
+                       int, hipStream_t);
 int launch_bb_step_f32(const float*, float*, const uint8_t*, int, int, int, double, double, double, double, int, int,
                        int, hipStream_t);
 int launch_voxelize_binary(const double*, const int32_t*, int, const double*, int, int, int, uint8_t*, hipStream_t);
@@ -533,7 +533,8 @@ int psc_bb_step(const void* d_in, void* d_out, const uint8_t* d_solid, int32_t s
     return launch_bb_step_f32((const float*)d_in, (float*)d_out, d_solid, nx, ny, nz, tau, fx, fy, fz, bz, by, flags,
                               s);
   if (storage == PSC_F64_PLAIN)
-    return launch_bb_step_f64((const double*)d_in, (double*)d_out, d_solid, nx, ny, nz, tau, fx, fy, fz, bz, by, s);
+    return launch_bb_step_f64((const double*)d_in, (double*)d_out, d_solid, nx, ny, nz, tau, fx, fy, fz, bz, by, flags,
+                              s);
   return set_error(PSC_EINVAL, "psc_bb_step: storage %d", storage);
 }
 

@@ -245,7 +245,8 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ fin
 //   u = (sum f c + F/2)/rho (the half shift only on components with F != 0),
 //   f' = (f - (f - feq)/tau) + Guo(u).
 // ---------------------------------------------------------------------------
-template <bool FORCE>
+// NT: non-temporal state loads and stores (the same bits).
+template <bool FORCE, bool NT>
 __global__ __launch_bounds__(256) void k_bb_step_f64(const double* __restrict__ fin, double* __restrict__ fout,
                                                      const uint8_t* __restrict__ solid, Dims d, double tau,
                                                      double ytau, double pref, double fx, double fy, double fz) {
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(256) void k_bb_step_f64(const double* __restrict__ 
   const size_t N = (size_t)d.ps;
   const uint32_t cell = cell_id(d, ix, iy, iz);
   double f[Q];
-  load_all<true>(Dirs{}, fin, N, d, ix, iy, iz, f);
+  load_all<true, NT>(Dirs{}, fin, N, d, ix, iy, iz, f);
   const bool s = solid[cell] != 0;
   if (s) {
 #pragma unroll
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(256) void k_bb_step_f64(const double* __restrict__ 
     const double v = f[i] - div_pos(f[i] - feq[i], tau, ytau);
     out[i] = FORCE ? v + g[i] : v;
   }
-  store_all<true>(Dirs{}, fout, N, d, ix, iy, iz, out);
+  store_all<true, NT>(Dirs{}, fout, N, d, ix, iy, iz, out);
 }
 
 // Binary voxelization (scripts/run_3d_window_lbm.py:73-92,
@@ -898,16 +899,18 @@ int grid_for(size_t n, int threads, int cap = 8192) {
 
 // ------------------------------- launchers ---------------------------------
 int launch_bb_step_f64(const double* in, double* out, const uint8_t* solid, int nx, int ny, int nz, double tau,
-                       double fx, double fy, double fz, int bz, int by, hipStream_t stream) {
+                       double fx, double fy, double fz, int bz, int by, int flags, hipStream_t stream) {
   const Dims d{nx, ny, nz, nz, (unsigned long long)nx * ny * nz};
   const dim3 block(bz, by, 1);
   const dim3 grid((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
   const double pref = 1.0 - 0.5 / tau;
   const double ytau = 1.0 / tau;  // RN(1/tau): host IEEE division
-  if (fx != 0.0 || fy != 0.0 || fz != 0.0)
-    k_bb_step_f64<true><<<grid, block, 0, stream>>>(in, out, solid, d, tau, ytau, pref, fx, fy, fz);
-  else
-    k_bb_step_f64<false><<<grid, block, 0, stream>>>(in, out, solid, d, tau, ytau, pref, fx, fy, fz);
+  const bool force = fx != 0.0 || fy != 0.0 || fz != 0.0;
+  const bool nt = (flags & PSC_STEP_NONTEMPORAL) != 0;  // the only launch flag of the FP64 step
+  if (force && nt) k_bb_step_f64<true, true><<<grid, block, 0, stream>>>(in, out, solid, d, tau, ytau, pref, fx, fy, fz);
+  else if (force) k_bb_step_f64<true, false><<<grid, block, 0, stream>>>(in, out, solid, d, tau, ytau, pref, fx, fy, fz);
+  else if (nt) k_bb_step_f64<false, true><<<grid, block, 0, stream>>>(in, out, solid, d, tau, ytau, pref, fx, fy, fz);
+  else k_bb_step_f64<false, false><<<grid, block, 0, stream>>>(in, out, solid, d, tau, ytau, pref, fx, fy, fz);
   return check_launch("psc_bb_step(f64)");
 }
 

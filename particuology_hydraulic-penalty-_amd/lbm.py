@@ -139,11 +139,11 @@ class BounceBackD3Q19:
         self._partials = torch.empty(int(_lib.load().psc_moments_blocks(self.N)) * _lib.PSC_NSUM,
                                      dtype=torch.float64, device=dev)
         self._sums = torch.empty(_lib.PSC_NSUM, dtype=torch.float64, device=dev)
-        # "auto": the launch (workgroup shape; for FP32 also non-temporal access and the
+        # "auto": the launch (workgroup shape and non-temporal access; for FP32 also the
         # occupancy cap) is timed on the first step; explicit blocks are used as given
         self._autotune = isinstance(block, str) and block == "auto"
         self.block = (0, 0) if isinstance(block, str) else tuple(int(v) for v in block)
-        self.flags = _lib.PSC_STEP_NONTEMPORAL if (nontemporal and precision == "fp32") else 0
+        self.flags = _lib.PSC_STEP_NONTEMPORAL if nontemporal else 0
         self._graph = None
         self.steps = 0
 
@@ -174,10 +174,10 @@ class BounceBackD3Q19:
             e1.synchronize()
             return e0.elapsed_time(e1) / reps
 
-        if self.storage == _lib.PSC_F64_PLAIN:  # the FP64 step takes no launch flags
+        if self.storage == _lib.PSC_F64_PLAIN:  # the FP64 step's only launch flag is non-temporal access
             nz = self.shape[2]
-            cands = [(bz, by, 0, 0) for bz, by in autotune.SHAPES + autotune.FP64_SHAPES
-                     if not (bz > 64 and nz <= bz // 2)]
+            cands = [(bz, by, nt, 0) for bz, by in autotune.SHAPES + autotune.FP64_SHAPES
+                     if not (bz > 64 and nz <= bz // 2) for nt in autotune.NT]
         else:
             cands = [(bz, by, nt, bpc) for bz, by, nt, bpc, flat in autotune.candidates(self.shape[2]) if not flat]
         probe = timed(64, 4, 0, 1)

@@ -92,3 +92,30 @@ def test_every_bounce_back_launch_gives_identical_results():
         if ref is None:
             ref = out
         assert np.array_equal(out, ref), (bz, by, nt, bpc)
+
+
+def test_fp64_bounce_back_nontemporal_gives_the_same_bits():
+    """The FP64 bounce-back step's one launch flag, non-temporal state access,
+    leaves every bit of the step unchanged (with and without a force, over
+    the FP64 workgroup shapes the tuner tries)."""
+    lbm = load_package("lbm")
+    at = load_package("autotune")
+    import torch
+    rng = np.random.default_rng(6)
+    n = 36
+    pts = rng.uniform(0.0, 1.0, (20, 3))
+    solid, _ = BB.voxelize(pts, np.array([0.5, 0.5, 0.5]), np.array([1.0, 1.0, 1.0]), 0.1, n)
+    stream = torch.cuda.current_stream().cuda_stream
+    for force in (0.0, 1.0e-5):
+        print(f"[bb fp64 nt] force {force}", flush=True)
+        s = lbm.BounceBackD3Q19(solid, tau=0.8, force_z=force, precision="fp64", device="cuda:0", block=(64, 4))
+        s.run(3, graph=False)
+        ref = None
+        for bz, by in [(64, 4), (128, 2)] + list(at.FP64_SHAPES):
+            for nt in (0, 1):
+                s.block, s.flags = (bz, by), at.flags_of(nt, 0, 0)
+                s._launch(s._a, s._b, stream)
+                out = s._b.cpu().numpy().copy()
+                if ref is None:
+                    ref = out
+                assert np.array_equal(out, ref), (force, bz, by, nt)
